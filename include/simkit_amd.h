@@ -1,0 +1,229 @@
+/*
+ * simkit_amd — C ABI of the MI355X (gfx950) implicit-FEM backward-Euler path.
+ *
+ * Conventions
+ *   - All array arguments are DEVICE pointers (hipMalloc'd or torch tensors'
+ *     data_ptr()) unless the name ends in _host. Row-major, C-contiguous.
+ *   - Indices are int32, reals FP64 unless stated.
+ *   - x / U vectors are flattened vertex positions (x[3*v + i]).
+ *   - F is row-major per element (F[9*e + 3*i + c], i spatial, c material);
+ *     9x9 Hessians index row-major vec(F).
+ *   - Every call is ordered on the given stream (NULL = the null stream).
+ *     Calls that return host scalars synchronise that stream.
+ *   - Return value: SK_OK (0) or a negative sk_status; sk_last_error() gives
+ *     the calling thread's last message.
+ *
+ * The reference (simkit v0.1.4) is pure Python/numpy/scipy; each entry point
+ * names the reference function it replaces as file:line.
+ */
+#ifndef SIMKIT_AMD_H
+#define SIMKIT_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SK_VERSION 1
+
+typedef void* sk_stream_t; /* hipStream_t */
+
+enum sk_status {
+  SK_OK = 0,
+  SK_ERR_INVALID = -1,
+  SK_ERR_HIP = -2,
+  SK_ERR_LS_FAIL = -3,       /* Armijo search found no step (backtracking_line_search.py:66-67 returns t=0) */
+  SK_ERR_PCG_BREAKDOWN = -4, /* p^T A p <= 0: system not positive definite */
+  SK_ERR_PCG_MAXIT = -5,     /* PCG hit its iteration limit before rtol */
+  SK_ERR_NONFINITE = -6,     /* NaN/Inf in a reduction */
+  SK_ERR_INVERTED = -7,      /* reserved: inverted element count reported */
+  SK_ERR_NEWTON_MAXIT = -8,  /* Newton hit its iteration limit before the gradient tolerance */
+  SK_ERR_UNSUPPORTED = -9
+};
+
+enum sk_material { SK_SNH = 0, SK_ARAP = 1 };
+enum sk_psd_method { SK_PSD_NONE = 0, SK_PSD_PROJ = 1, SK_PSD_ABS = 2 };
+
+const char* sk_last_error(void);
+const char* sk_status_string(int status);
+int sk_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Kinematics and rest measures                                              */
+/* ------------------------------------------------------------------------ */
+
+/* Per-tet gradient operator D_e = (H Dm^-1)^T, stored D[12*e + 4*c + v], and
+ * signed rest volume det(Dm)/6 (vol may be NULL).
+ * Replaces deformation_jacobian.py:58-61 (and deformation_gradient.py:43-62),
+ * tetrahedron_volumes.py:26-27 / volume.py:14-41. */
+int sk_tet_operator(const double* X, const int32_t* T, int64_t nv, int64_t nt, double* D, double* vol,
+                    sk_stream_t stream);
+
+/* F[e] = (D_e U[T_e])^T. Replaces deformation_gradient.py:65-67 (and J @ x,
+ * stable_neo_hookean.py:471,499,531). */
+int sk_deformation_gradient(const double* D, const int32_t* T, const double* U, int64_t nt, double* F,
+                            sk_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* Element tier                                                              */
+/* ------------------------------------------------------------------------ */
+
+/* psi (nt), P (nt x 9), H (nt x 81) of material on F; any output may be NULL;
+ * lam is ignored (may be NULL) for ARAP.
+ * Replaces stable_neo_hookean_{energy,gradient,hessian}_element_F
+ * (energies/stable_neo_hookean.py:65-443) and arap_{energy,gradient,hessian}_element_F
+ * (energies/arap.py:71-145). */
+int sk_element_F(int material, const double* F, const double* mu, const double* lam, int64_t nt, double* psi,
+                 double* P, double* H, sk_stream_t stream);
+
+/* Batched eigenvalue projection of n symmetric d x d matrices (lower triangle
+ * read), method SK_PSD_PROJ (floor 1e-6) or SK_PSD_ABS. d in 1..12.
+ * Replaces psd_project.py:12-47. */
+int sk_psd_project(const double* H, int64_t n, int d, int method, double* out, sk_stream_t stream);
+
+/* Signed SVD F = U diag(S) V^T (U, V rotations, S[2] carries sign det F) and
+ * R = U V^T; any output may be NULL.
+ * Replaces svd_rv.py:8-53 and polar_svd.py:8-89. */
+int sk_polar_svd(const double* F, int64_t n, double* U, double* S, double* V, double* R, sk_stream_t stream);
+
+/* dR/dF (n x 81). Replaces rotation_gradient.py:12-75 (3D branch). */
+int sk_rotation_gradient(const double* F, int64_t n, double* dRdF, sk_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* Mesh handle: sorted CSR pattern of the vertex-pair x 3x3 block structure  */
+/* and the element -> nnz map used for atomic-free assembly.                 */
+/* ------------------------------------------------------------------------ */
+typedef struct sk_mesh sk_mesh;
+
+/* T_host: nt x 4 connectivity (host); D: device operator from sk_tet_operator
+ * (or extracted from a user's deformation Jacobian). Replaces the pattern
+ * work of sp.sparse.block_diag + J.T @ H @ J (stable_neo_hookean.py:536-537). */
+int sk_mesh_create(const int32_t* T_host, const double* D, int64_t nv, int64_t nt, sk_mesh** out);
+int sk_mesh_destroy(sk_mesh* mesh);
+int sk_mesh_info(const sk_mesh* mesh, int64_t* nv, int64_t* nt, int64_t* nnz, int64_t* nblocks);
+
+/* Scalar CSR pattern of the 3nv x 3nv Hessian: indptr (3nv+1), indices (nnz),
+ * columns sorted within each row. */
+int sk_mesh_csr_pattern(const sk_mesh* mesh, int32_t* indptr, int32_t* indices, sk_stream_t stream);
+
+/* elem_nnz[144*e + 12*(3*i+p) + 3*j+q] = CSR position of entry
+ * (3*T[e][i]+p, 3*T[e][j]+q): where K_e's entries land. */
+int sk_mesh_element_nnz(const sk_mesh* mesh, int32_t* elem_nnz, sk_stream_t stream);
+
+/* Lumped per-vertex mass (sum_e vol_e rho_e)/4; rho per element, may be NULL
+ * (rho = 1). Replaces massmatrix.py:16-50 (diagonal of M) and, scaled by the
+ * acceleration, gravity_force.py:8-42. */
+int sk_mesh_lumped_mass(sk_mesh* mesh, const double* vol, const double* rho, double* mass, sk_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* Global x tier on a mesh (mu, lam, vol are per-element device arrays).     */
+/* F0 (nt x 9, may be NULL) is added to every F = J x: with x = u and        */
+/* F0 = J x_bar these are the u tier (stable_neo_hookean.py:544-648,         */
+/* arap.py:331-424).                                                         */
+/* ------------------------------------------------------------------------ */
+
+/* E = sum_e vol_e psi(F_e(x)) -> *energy_host; elem_energy (nt) optional.
+ * Replaces stable_neo_hookean_energy_x (:449-474) / arap_energy_x (arap.py:242-265). */
+int sk_mesh_energy(sk_mesh* mesh, int material, const double* x, const double* F0, const double* mu, const double* lam,
+                   const double* vol, double* energy_host, double* elem_energy, sk_stream_t stream);
+
+/* g = J^T vec(vol P) (3nv). Replaces stable_neo_hookean_gradient_x (:477-503) /
+ * arap_gradient_x (arap.py:268-292). */
+int sk_mesh_gradient(sk_mesh* mesh, int material, const double* x, const double* F0, const double* mu, const double* lam,
+                     const double* vol, double* g, sk_stream_t stream);
+
+/* CSR values (nnz, pattern of sk_mesh_csr_pattern) of sum_e B_e^T psd(vol_e H_e) B_e.
+ * psd: SK_PSD_NONE / SK_PSD_PROJ / SK_PSD_ABS. Replaces stable_neo_hookean_hessian_x
+ * (:506-538) / arap_hessian_x (arap.py:295-325) including psd_project and the
+ * block_diag / J.T @ H @ J assembly. */
+int sk_mesh_hessian(sk_mesh* mesh, int material, const double* x, const double* F0, const double* mu, const double* lam,
+                    const double* vol, int psd, double* values, sk_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* Kinetic (inertial) terms                                                  */
+/* ------------------------------------------------------------------------ */
+
+/* y = A x for a general CSR matrix (M @ d in kinetic.py:107-119). */
+int sk_csr_spmv(int64_t n, const int32_t* indptr, const int32_t* indices, const double* values, const double* x,
+                double* y, sk_stream_t stream);
+
+/* With x_curr/x_prev: x~ = x_c + h (x_c - x_p)/h, v = (x_c - x_p)/h, d = x - x~;
+ * without: d = x. Then energy = 0.5 c d.(M d) / h^2 (host) and
+ * grad = c (M d) / h^2. Every output may be NULL.
+ * Replaces kinetic.py:41-60 (velocity_be), :87-90 (be_target), :107-119
+ * (kinetic_energy / _gradient) and :125-175 (the _be variants). */
+int sk_kinetic(int64_t n, const double* x, const double* x_curr, const double* x_prev, double h, double c,
+               const int32_t* M_indptr, const int32_t* M_indices, const double* M_values, double* energy_host,
+               double* grad, double* x_tilde, double* velocity, sk_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* Linear solve                                                              */
+/* ------------------------------------------------------------------------ */
+
+/* Jacobi-preconditioned CG on a general SPD CSR matrix, x0 = 0, stop at
+ * ||r|| <= rtol ||b||. Replaces spsolve in newton.py:52 for the generic
+ * (user-closure) Newton path. *iters and *relres are host outputs. */
+int sk_pcg_csr(int64_t n, const int32_t* indptr, const int32_t* indices, const double* values, const double* b,
+               double* x, double rtol, int64_t maxit, int64_t* iters, double* relres, sk_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* Device-resident backward-Euler simulator                                  */
+/* ------------------------------------------------------------------------ */
+typedef struct sk_sim sk_sim;
+
+typedef struct sk_sim_params {
+  int material;          /* SK_SNH / SK_ARAP */
+  int psd;               /* SK_PSD_PROJ (reference default psd=True) or SK_PSD_NONE */
+  double h;              /* time step */
+  double newton_rtol;    /* stop when ||g_free|| <= newton_rtol * ||g_free(x0)|| */
+  int newton_max_iter;
+  double pcg_rtol;       /* block-Jacobi PCG relative residual */
+  int pcg_max_iter;
+  double ls_alpha;       /* Armijo constants, backtracking_line_search.py:8-17 */
+  double ls_beta;
+  int ls_max_iter;
+  double ls_threshold;
+  int timing;            /* 1: record per-phase times in sk_step_info */
+} sk_sim_params;
+
+typedef struct sk_step_info {
+  int status;            /* sk_status of the step */
+  int newton_iters;
+  int64_t pcg_iters;
+  int ls_trials;
+  double g0_norm;
+  double g_norm;
+  double ms_gradient;    /* per-phase device times (timing=1) */
+  double ms_hessian;
+  double ms_pcg;
+  double ms_line_search;
+} sk_step_info;
+
+/* mu, lam, vol: per element; mass: per DOF (3nv, M = kron(M_lumped, I3));
+ * f_ext: per DOF external force (may be NULL); fixed: per DOF 0/1 (may be
+ * NULL). Arrays are copied. Replaces the closure set-up of backward_euler.py:76-83
+ * (with kinetic.py:125-194). */
+int sk_sim_create(sk_mesh* mesh, const sk_sim_params* params, const double* mu, const double* lam,
+                  const double* vol, const double* mass, const double* f_ext, const uint8_t* fixed, sk_sim** out);
+int sk_sim_destroy(sk_sim* sim);
+
+/* One backward-Euler step: Newton on E(x) = V(x) + 1/(2h^2)|x - x~|_M^2 - f_ext.x
+ * from x0 = x~ = x_curr + h (x_curr - x_prev)/h, fixed DOFs held at x_bc
+ * (or at x_curr when x_bc is NULL). Writes x_next; info is a host struct.
+ * Replaces backward_euler.py:27-91 -> newton.py:7-75 -> backtracking_line_search.py:8-67. */
+int sk_sim_step(sk_sim* sim, const double* x_curr, const double* x_prev, const double* x_bc, double* x_next,
+                sk_step_info* info, sk_stream_t stream);
+
+/* Measurement hook, no counterpart in the reference: re-initialises PCG
+ * scratch vectors on the system assembled by the last sk_sim_step (RHS = the
+ * last gradient) and enqueues `reps` block-CSR SpMV launches back to back on
+ * `stream`, so a caller can bracket them with events. The simulator's state
+ * (positions, gradient, assembled values) is not modified. */
+int sk_sim_spmv_reps(sk_sim* sim, int reps, sk_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SIMKIT_AMD_H */

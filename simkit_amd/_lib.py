@@ -1,0 +1,143 @@
+"""ctypes binding of libsimkit_amd.so (the C ABI declared in include/simkit_amd.h).
+
+The library is loaded from this package directory only. If it is missing or
+no GPU is visible, every entry point raises: there is no CPU fallback.
+"""
+
+import ctypes
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsimkit_amd.so")
+
+SK_OK = 0
+SK_ERR_INVALID = -1
+SK_ERR_HIP = -2
+SK_ERR_LS_FAIL = -3
+SK_ERR_PCG_BREAKDOWN = -4
+SK_ERR_PCG_MAXIT = -5
+SK_ERR_NONFINITE = -6
+SK_ERR_INVERTED = -7
+SK_ERR_NEWTON_MAXIT = -8
+SK_ERR_UNSUPPORTED = -9
+
+SK_SNH = 0
+SK_ARAP = 1
+SK_PSD_NONE = 0
+SK_PSD_PROJ = 1
+SK_PSD_ABS = 2
+
+MATERIALS = {"snh": SK_SNH, "stable_neo_hookean": SK_SNH, "arap": SK_ARAP}
+
+
+class SimkitAmdError(RuntimeError):
+    """A non-zero status from the HIP library."""
+
+    def __init__(self, status, message):
+        super().__init__(f"simkit_amd status {status}: {message}")
+        self.status = status
+
+
+class SkSimParams(ctypes.Structure):
+    _fields_ = [
+        ("material", ctypes.c_int),
+        ("psd", ctypes.c_int),
+        ("h", ctypes.c_double),
+        ("newton_rtol", ctypes.c_double),
+        ("newton_max_iter", ctypes.c_int),
+        ("pcg_rtol", ctypes.c_double),
+        ("pcg_max_iter", ctypes.c_int),
+        ("ls_alpha", ctypes.c_double),
+        ("ls_beta", ctypes.c_double),
+        ("ls_max_iter", ctypes.c_int),
+        ("ls_threshold", ctypes.c_double),
+        ("timing", ctypes.c_int),
+    ]
+
+
+class SkStepInfo(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int),
+        ("newton_iters", ctypes.c_int),
+        ("pcg_iters", ctypes.c_int64),
+        ("ls_trials", ctypes.c_int),
+        ("g0_norm", ctypes.c_double),
+        ("g_norm", ctypes.c_double),
+        ("ms_gradient", ctypes.c_double),
+        ("ms_hessian", ctypes.c_double),
+        ("ms_pcg", ctypes.c_double),
+        ("ms_line_search", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_D = ctypes.c_double
+_INT = ctypes.c_int
+
+# name -> argtypes (restype is int unless listed in _RESTYPES)
+SIGNATURES = {
+    "sk_last_error": [],
+    "sk_status_string": [_INT],
+    "sk_version": [],
+    "sk_tet_operator": [_P, _P, _I64, _I64, _P, _P, _P],
+    "sk_deformation_gradient": [_P, _P, _P, _I64, _P, _P],
+    "sk_element_F": [_INT, _P, _P, _P, _I64, _P, _P, _P, _P],
+    "sk_psd_project": [_P, _I64, _INT, _INT, _P, _P],
+    "sk_polar_svd": [_P, _I64, _P, _P, _P, _P, _P],
+    "sk_rotation_gradient": [_P, _I64, _P, _P],
+    "sk_mesh_create": [_P, _P, _I64, _I64, ctypes.POINTER(_P)],
+    "sk_mesh_destroy": [_P],
+    "sk_mesh_info": [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(_I64), ctypes.POINTER(_I64)],
+    "sk_mesh_csr_pattern": [_P, _P, _P, _P],
+    "sk_mesh_element_nnz": [_P, _P, _P],
+    "sk_mesh_lumped_mass": [_P, _P, _P, _P, _P],
+    "sk_mesh_energy": [_P, _INT, _P, _P, _P, _P, _P, ctypes.POINTER(_D), _P, _P],
+    "sk_mesh_gradient": [_P, _INT, _P, _P, _P, _P, _P, _P, _P],
+    "sk_mesh_hessian": [_P, _INT, _P, _P, _P, _P, _P, _INT, _P, _P],
+    "sk_csr_spmv": [_I64, _P, _P, _P, _P, _P, _P],
+    "sk_kinetic": [_I64, _P, _P, _P, _D, _D, _P, _P, _P, ctypes.POINTER(_D), _P, _P, _P, _P],
+    "sk_pcg_csr": [_I64, _P, _P, _P, _P, _P, _D, _I64, ctypes.POINTER(_I64), ctypes.POINTER(_D), _P],
+    "sk_sim_create": [_P, ctypes.POINTER(SkSimParams), _P, _P, _P, _P, _P, _P, ctypes.POINTER(_P)],
+    "sk_sim_destroy": [_P],
+    "sk_sim_step": [_P, _P, _P, _P, _P, ctypes.POINTER(SkStepInfo), _P],
+    "sk_sim_spmv_reps": [_P, _INT, _P],
+}
+_RESTYPES = {"sk_last_error": ctypes.c_char_p, "sk_status_string": ctypes.c_char_p}
+
+_lock = threading.Lock()
+_handle = None
+
+
+def load():
+    """Load the library (once). Raises if it has not been built."""
+    global _handle
+    with _lock:
+        if _handle is None:
+            if not os.path.exists(LIB_PATH):
+                raise ImportError(
+                    f"simkit_amd: HIP library not found at {LIB_PATH}; build it with "
+                    "`python -m simkit_amd.build` (there is no CPU fallback)")
+            h = ctypes.CDLL(LIB_PATH)
+            for name, args in SIGNATURES.items():
+                fn = getattr(h, name)
+                fn.argtypes = args
+                fn.restype = _RESTYPES.get(name, ctypes.c_int)
+            _handle = h
+    return _handle
+
+
+def check(status):
+    if status != SK_OK:
+        msg = load().sk_last_error().decode(errors="replace")
+        raise SimkitAmdError(status, msg)
+    return status
+
+
+def call(name, *args):
+    """Call an entry point and raise SimkitAmdError on a non-zero status."""
+    return check(getattr(load(), name)(*args))

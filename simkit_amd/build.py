@@ -1,0 +1,49 @@
+"""Build the simkit_amd HIP library in-tree: simkit_amd/libsimkit_amd.so (gfx950).
+
+Usage: ``python -m simkit_amd.build`` (also called by ``__graft_entry__.build()``).
+Cross-compiles without a GPU. Packed-FP32 instructions are disabled for all
+device code (``-Xclang -target-feature -Xclang -packed-fp32-ops``).
+"""
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libsimkit_amd.so")
+SOURCES = ["sk_element.hip", "sk_mesh.hip", "sk_solver.hip", "sk_aux.hip"]
+ARCH = os.environ.get("SIMKIT_AMD_ARCH", "gfx950")
+
+
+def hipcc():
+    for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", "hipcc"):
+        if c and (os.path.isabs(c) and os.path.exists(c) or not os.path.isabs(c)):
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+def build(verbose=False, extra_flags=()):
+    """Compile every HIP source into one shared library; returns its path."""
+    tmp = LIB + ".tmp"
+    cmd = [
+        hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+        "-I", os.path.join(ROOT, "include"), "-Wall", "-Wno-unused-function",
+        "-o", tmp,
+    ] + list(extra_flags) + [os.path.join(CSRC, s) for s in SOURCES]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError(f"hipcc failed with exit code {res.returncode}")
+    if verbose and (res.stdout or res.stderr):
+        sys.stderr.write(res.stdout + res.stderr)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv, extra_flags=[a for a in sys.argv[1:] if a != "-v"]))

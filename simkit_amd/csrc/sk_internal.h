@@ -1,0 +1,109 @@
+// Internal structures shared by the mesh / x-tier and solver translation units.
+#pragma once
+
+#include "sk_common.h"
+
+#include <vector>
+
+namespace sk {
+
+// Owning device buffer.
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  int alloc(size_t count) {
+    release();
+    if (count == 0) return SK_OK;
+    hipError_t e = hipMalloc(&p, count * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      set_error(std::string("hipMalloc failed: ") + hipGetErrorString(e));
+      return SK_ERR_HIP;
+    }
+    n = count;
+    return SK_OK;
+  }
+  int upload(const std::vector<T>& h) {
+    SK_TRY(alloc(h.size()));
+    if (!h.empty()) SK_HIP_CHECK(hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return SK_OK;
+  }
+};
+
+// Upper-triangle 3x3-block index of local pair (i <= j) of a tet: 0..9.
+__host__ __device__ constexpr int ublock(int i, int j) { return i * (7 - i) / 2 + j; }
+
+constexpr int kKst = 90;  // staged doubles per element: 10 upper 3x3 blocks of K_e
+constexpr int kFst = 12;  // staged doubles per element: f_e = vol P D
+
+// Grid-reduction scratch: partial sums, a ticket counter, result slots.
+struct Reducer {
+  DevBuf<double> partials;  // up to 8 values x kMaxBlocks
+  DevBuf<unsigned> counter;
+  DevBuf<double> result;    // 8 doubles
+  static constexpr int kMaxBlocks = 65536;
+  int init() {
+    SK_TRY(partials.alloc(8 * (size_t)kMaxBlocks));
+    SK_TRY(counter.alloc(1));
+    SK_TRY(result.alloc(8));
+    SK_HIP_CHECK(hipMemset(counter.p, 0, sizeof(unsigned)));
+    SK_HIP_CHECK(hipMemset(result.p, 0, 8 * sizeof(double)));
+    return SK_OK;
+  }
+};
+
+}  // namespace sk
+
+struct sk_mesh {
+  int64_t nv = 0, nt = 0, nblk = 0;
+  sk::DevBuf<int32_t> T;      // SoA [4][nt]
+  sk::DevBuf<double> D;       // SoA [12][nt], (4*c + v)*nt + e
+  sk::DevBuf<int32_t> bptr;   // [nv+1] block-row pointer
+  sk::DevBuf<int32_t> bcol;   // [nblk] sorted neighbour vertex per block
+  sk::DevBuf<int32_t> brow;   // [nblk] row vertex per block
+  sk::DevBuf<int32_t> emap;   // [16][nt] block of local pair (i, j)
+  sk::DevBuf<int32_t> cptr;   // [nblk+1] contribution list per block
+  sk::DevBuf<int32_t> clist;  // [16 nt] (e << 4) | (4 i + j), e ascending per block
+  sk::DevBuf<int32_t> vptr;   // [nv+1] element incidences per vertex
+  sk::DevBuf<int32_t> vlist;  // [4 nt] (e << 2) | i, e ascending per vertex
+  sk::DevBuf<double> Kst;     // [nt][90] staged element stiffness blocks
+  sk::DevBuf<double> fst;     // [nt][12] staged element forces
+  sk::Reducer red;
+  double* pinned = nullptr;   // 16 host-pinned doubles for scalar readback
+};
+
+namespace sk {
+
+// Element-pass launchers shared by the x tier and the simulator.
+int launch_elem_energy(sk_mesh* m, int material, const double* x, const double* F0, const double* dx, double t,
+                       const double* mu,
+                       const double* lam, const double* vol, double* elem_energy, const double* mass,
+                       const double* xt, const double* fext, const uint8_t* fixed, double inv_h2,
+                       hipStream_t st);
+int launch_elem_grad(sk_mesh* m, int material, const double* x, const double* F0, const double* mu,
+                     const double* lam, const double* vol, hipStream_t st);
+int launch_elem_hess(sk_mesh* m, int material, const double* x, const double* F0, const double* mu,
+                     const double* lam, const double* vol, int psd, hipStream_t st);
+// Gathers staged element forces into g; with mass != nullptr adds
+// m (x - xt) * inv_h2 - fext, zeroes fixed DOFs and reduces ||g||^2 into red.result[0].
+int launch_gather_grad(sk_mesh* m, double* g, const double* mass, const double* x, const double* xt,
+                       const double* fext, const uint8_t* fixed, double inv_h2, hipStream_t st);
+// Gathers staged element blocks into CSR values; with mass != nullptr adds
+// mass * inv_h2 on the diagonal and writes the fixed-DOF-masked inverse of each
+// diagonal 3x3 block into binv (9 per vertex).
+int launch_gather_hess(sk_mesh* m, double* values, const double* mass, double inv_h2, const uint8_t* fixed,
+                       double* binv, hipStream_t st);
+// Copies n doubles from device to the mesh's pinned buffer and waits.
+int read_scalars(sk_mesh* m, const double* dev, int n, hipStream_t st);
+
+}  // namespace sk

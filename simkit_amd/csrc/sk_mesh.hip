@@ -1,0 +1,622 @@
+// Mesh handle, sparsity pattern and the global x tier.
+//
+// Assembly is atomic-free and sort-free at run time. Set-up (host, once per
+// mesh) builds the sorted vertex-pair block pattern, the element -> block map
+// and, per block, the list of (element, local pair) contributions in
+// ascending element order. Each Hessian evaluation is then two launches:
+//   1. element pass: one lane per tet computes F, the 9x9 Hessian, its
+//      vol-weighted PSD projection and the 10 upper 3x3 blocks of
+//      K_e = B_e^T H+ B_e, staged per element (kKst doubles);
+//   2. block gather: one lane per CSR block sums its contributions in list
+//      order and writes the 3 row segments of the block. No two lanes write the
+//      same value and the summation order is fixed, so values are
+//      bit-reproducible.
+#include "sk_internal.h"
+#include "sk_materials.h"
+
+#include <algorithm>
+#include <numeric>
+
+namespace sk {
+
+// ---------------------------------------------------------------------------
+// Element helpers (SoA mesh layout)
+// ---------------------------------------------------------------------------
+struct ElemView {
+  int64_t nt;
+  const int32_t* __restrict__ T;  // [4][nt]
+  const double* __restrict__ D;   // [12][nt]
+};
+
+__device__ __forceinline__ void load_elem(const ElemView& m, int64_t e, int32_t (&v)[4], double (&D)[12]) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) v[k] = m.T[k * m.nt + e];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) D[k] = m.D[k * m.nt + e];
+}
+
+__device__ __forceinline__ void gather_x(const double* __restrict__ x, const int32_t (&v)[4], double (&xe)[12]) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) xe[3 * k + i] = x[3 * (int64_t)v[k] + i];
+}
+
+// F[3i+c] = sum_v xe[3v+i] D[4c+v]
+__device__ __forceinline__ void make_F(const double (&xe)[12], const double (&D)[12], double (&F)[9]) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      F[3 * i + c] = xe[i] * D[4 * c] + xe[3 + i] * D[4 * c + 1] + xe[6 + i] * D[4 * c + 2] + xe[9 + i] * D[4 * c + 3];
+}
+
+__device__ __forceinline__ void add_F0(const double* __restrict__ F0, int64_t e, double (&F)[9]) {
+  if (F0) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) F[k] += F0[9 * e + k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Element passes
+// ---------------------------------------------------------------------------
+struct EnergyArgs {
+  ElemView m;
+  int64_t nv;
+  const double* x;
+  const double* F0;  // optional per-element offset added to F (u tier: F = J u + J x_bar)
+  const double* dx;  // non-null: evaluate E(x + t dx) - E(x)
+  double t;
+  const double* mu;
+  const double* lam;
+  const double* vol;
+  double* elem_energy;
+  const double* mass;  // per DOF; non-null adds the kinetic and load difference
+  const double* xt;
+  const double* fext;
+  const uint8_t* fixed;
+  double inv_h2;
+  double* partials;
+  unsigned* counter;
+  double* result;
+};
+
+template <class M>
+__global__ __launch_bounds__(kBlock) void elem_energy_kernel(EnergyArgs a) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double acc = 0.0;
+  if (e < a.m.nt) {
+    int32_t v[4];
+    double D[12], xe[12], F[9];
+    load_elem(a.m, e, v, D);
+    gather_x(a.x, v, xe);
+    make_F(xe, D, F);
+    add_F0(a.F0, e, F);
+    const double mu = a.mu[e], lam = a.lam ? a.lam[e] : 0.0, vol = a.vol[e];
+    double val = vol * M::energy(F, mu, lam);
+    if (a.dx) {
+      double de[12], F1[9];
+      gather_x(a.dx, v, de);
+#pragma unroll
+      for (int k = 0; k < 12; ++k) de[k] = xe[k] + a.t * de[k];
+      make_F(de, D, F1);
+      add_F0(a.F0, e, F1);
+      val = vol * M::energy(F1, mu, lam) - val;
+    }
+    if (a.elem_energy) a.elem_energy[e] = val;
+    acc = val;
+  }
+  if (a.mass && e < a.nv) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int64_t k = 3 * e + c;
+      if (a.fixed && a.fixed[k]) continue;
+      const double x0 = a.x[k];
+      const double x1 = x0 + a.t * a.dx[k];
+      const double d0 = x0 - a.xt[k], d1 = x1 - a.xt[k];
+      acc += 0.5 * a.inv_h2 * a.mass[k] * (d1 - d0) * (d1 + d0);
+      if (a.fext) acc -= a.fext[k] * (x1 - x0);
+    }
+  }
+  double v1[1] = {acc}, out[1];
+  if (grid_sum<1>(v1, a.partials, a.counter, out) && threadIdx.x == 0) a.result[0] = out[0];
+}
+
+struct ElemArgs {
+  ElemView m;
+  const double* x;
+  const double* F0;
+  const double* mu;
+  const double* lam;
+  const double* vol;
+  double* fst;
+  double* Kst;
+  int psd;
+};
+
+template <class M>
+__global__ __launch_bounds__(kBlock) void elem_grad_kernel(ElemArgs a) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= a.m.nt) return;
+  int32_t v[4];
+  double D[12], xe[12], F[9], P[9];
+  load_elem(a.m, e, v, D);
+  gather_x(a.x, v, xe);
+  make_F(xe, D, F);
+  add_F0(a.F0, e, F);
+  M::gradient(F, a.mu[e], a.lam ? a.lam[e] : 0.0, P);
+  const double vol = a.vol[e];
+  double* out = a.fst + kFst * e;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      out[3 * k + i] = vol * (P[3 * i] * D[k] + P[3 * i + 1] * D[4 + k] + P[3 * i + 2] * D[8 + k]);
+}
+
+template <class M, int PSD>
+__global__ __launch_bounds__(kBlock) void elem_hess_kernel(ElemArgs a) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= a.m.nt) return;
+  int32_t v[4];
+  double D[12], xe[12], F[9], H[45];
+  load_elem(a.m, e, v, D);
+  gather_x(a.x, v, xe);
+  make_F(xe, D, F);
+  add_F0(a.F0, e, F);
+  M::hessian(F, a.mu[e], a.lam ? a.lam[e] : 0.0, H);
+  const double vol = a.vol[e];
+#pragma unroll
+  for (int k = 0; k < 45; ++k) H[k] *= vol;
+  if (PSD != SK_PSD_NONE) psd_project_sym<9, double>(H, PSD, 1e-6);
+  // K_e[3i+p][3j+q] = sum_{c,c'} D[c][i] H[3p+c][3q+c'] D[c'][j], upper blocks i <= j.
+  double* out = a.Kst + (int64_t)kKst * e;
+#pragma unroll
+  for (int p = 0; p < 3; ++p)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      double G[3][4];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          G[c][j] = H[Sym<9>::idx(3 * p + c, 3 * q + 0)] * D[j] + H[Sym<9>::idx(3 * p + c, 3 * q + 1)] * D[4 + j] +
+                    H[Sym<9>::idx(3 * p + c, 3 * q + 2)] * D[8 + j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = i; j < 4; ++j)
+          out[ublock(i, j) * 9 + 3 * p + q] = D[i] * G[0][j] + D[4 + i] * G[1][j] + D[8 + i] * G[2][j];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Gathers
+// ---------------------------------------------------------------------------
+struct GatherGArgs {
+  int64_t nv;
+  const int32_t* vptr;
+  const int32_t* vlist;
+  const double* fst;
+  double* g;
+  const double* mass;
+  const double* x;
+  const double* xt;
+  const double* fext;
+  const uint8_t* fixed;
+  double inv_h2;
+  double* partials;
+  unsigned* counter;
+  double* result;
+};
+
+__global__ __launch_bounds__(kBlock) void gather_grad_kernel(GatherGArgs a) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double nrm = 0.0;
+  if (v < a.nv) {
+    double acc[3] = {0.0, 0.0, 0.0};
+    const int32_t k0 = a.vptr[v], k1 = a.vptr[v + 1];
+    for (int32_t k = k0; k < k1; ++k) {
+      const int32_t pk = a.vlist[k];
+      const double* f = a.fst + (int64_t)kFst * (pk >> 2) + 3 * (pk & 3);
+      acc[0] += f[0];
+      acc[1] += f[1];
+      acc[2] += f[2];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int64_t k = 3 * v + c;
+      double gk = acc[c];
+      if (a.mass) {
+        gk += a.mass[k] * (a.x[k] - a.xt[k]) * a.inv_h2;
+        if (a.fext) gk -= a.fext[k];
+        if (a.fixed && a.fixed[k]) gk = 0.0;
+        nrm += gk * gk;
+      }
+      a.g[k] = gk;
+    }
+  }
+  if (a.mass) {
+    double v1[1] = {nrm}, out[1];
+    if (grid_sum<1>(v1, a.partials, a.counter, out) && threadIdx.x == 0) a.result[0] = out[0];
+  }
+}
+
+struct GatherHArgs {
+  int64_t nblk;
+  const int32_t* bptr;
+  const int32_t* bcol;
+  const int32_t* brow;
+  const int32_t* cptr;
+  const int32_t* clist;
+  const double* Kst;
+  double* values;
+  const double* mass;
+  double inv_h2;
+  const uint8_t* fixed;
+  double* binv;
+};
+
+__global__ __launch_bounds__(kBlock) void gather_hess_kernel(GatherHArgs a) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= a.nblk) return;
+  double acc[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) acc[k] = 0.0;
+  const int32_t k0 = a.cptr[b], k1 = a.cptr[b + 1];
+  for (int32_t k = k0; k < k1; ++k) {
+    const int32_t pk = a.clist[k];
+    const int i = (pk >> 2) & 3, j = pk & 3;
+    const int64_t e = pk >> 4;
+    if (i <= j) {
+      const double* s = a.Kst + kKst * e + 9 * ublock(i, j);
+#pragma unroll
+      for (int m = 0; m < 9; ++m) acc[m] += s[m];
+    } else {
+      const double* s = a.Kst + kKst * e + 9 * ublock(j, i);
+#pragma unroll
+      for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) acc[3 * p + q] += s[3 * q + p];
+    }
+  }
+  const int32_t v = a.brow[b], w = a.bcol[b];
+  const int32_t b0 = a.bptr[v], deg = a.bptr[v + 1] - b0, r = (int32_t)(b - b0);
+  if (v == w && a.mass) {
+#pragma unroll
+    for (int p = 0; p < 3; ++p) acc[4 * p] += a.mass[3 * (int64_t)v + p] * a.inv_h2;
+  }
+  double* out = a.values + 9 * (int64_t)b0 + 3 * r;
+#pragma unroll
+  for (int p = 0; p < 3; ++p)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) out[3 * p * deg + q] = acc[3 * p + q];
+  if (v == w && a.binv) {
+    // Inverse of diag(f) A diag(f) + diag(1 - f), then masked to the free DOFs.
+    double f[3];
+#pragma unroll
+    for (int p = 0; p < 3; ++p) f[p] = (a.fixed && a.fixed[3 * (int64_t)v + p]) ? 0.0 : 1.0;
+    double A[9];
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) A[3 * p + q] = f[p] * f[q] * acc[3 * p + q] + (p == q ? 1.0 - f[p] : 0.0);
+    double c[9];
+    cofactor3(A, c);
+    const double det = A[0] * c[0] + A[1] * c[1] + A[2] * c[2];
+    const double inv = 1.0 / det;
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) a.binv[9 * (int64_t)v + 3 * p + q] = f[p] * f[q] * c[3 * q + p] * inv;
+  }
+}
+
+__global__ void csr_pattern_kernel(int64_t nv, int64_t nblk, const int32_t* bptr, const int32_t* bcol,
+                                   const int32_t* brow, int32_t* indptr, int32_t* indices) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (indptr && b <= nv) {
+    if (b == nv) {
+      indptr[3 * nv] = (int32_t)(9 * nblk);
+    } else {
+      const int32_t b0 = bptr[b], deg = bptr[b + 1] - b0;
+#pragma unroll
+      for (int p = 0; p < 3; ++p) indptr[3 * b + p] = 9 * b0 + 3 * p * deg;
+    }
+  }
+  if (indices && b < nblk) {
+    const int32_t v = brow[b], w = bcol[b];
+    const int32_t b0 = bptr[v], deg = bptr[v + 1] - b0, r = (int32_t)(b - b0);
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) indices[9 * (int64_t)b0 + 3 * p * deg + 3 * r + q] = 3 * w + q;
+  }
+}
+
+__global__ void element_nnz_kernel(int64_t nt, const int32_t* T, const int32_t* bptr, const int32_t* emap,
+                                   int32_t* out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nt) return;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int32_t v = T[i * nt + e];
+    const int32_t b0 = bptr[v], deg = bptr[v + 1] - b0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int32_t r = emap[(4 * i + j) * nt + e] - b0;
+#pragma unroll
+      for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) out[144 * e + 12 * (3 * i + p) + 3 * j + q] = 9 * b0 + 3 * p * deg + 3 * r + q;
+    }
+  }
+}
+
+__global__ void aos_to_soa_kernel(const double* __restrict__ D, int64_t nt, double* __restrict__ Ds) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nt) return;
+#pragma unroll
+  for (int k = 0; k < 12; ++k) Ds[k * nt + e] = D[12 * e + k];
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+static ElemView view(sk_mesh* m) { return ElemView{m->nt, m->T.p, m->D.p}; }
+
+int launch_elem_energy(sk_mesh* m, int material, const double* x, const double* F0, const double* dx, double t,
+                       const double* mu,
+                       const double* lam, const double* vol, double* elem_energy, const double* mass,
+                       const double* xt, const double* fext, const uint8_t* fixed, double inv_h2,
+                       hipStream_t st) {
+  EnergyArgs a{view(m), m->nv, x, F0, dx, t, mu, lam, vol, elem_energy, mass, xt, fext, fixed, inv_h2,
+               m->red.partials.p, m->red.counter.p, m->red.result.p};
+  const int64_t n = std::max<int64_t>(m->nt, mass ? m->nv : 0);
+  const int g = grid_for(n);
+  SK_REQUIRE(g <= Reducer::kMaxBlocks, "mesh too large for the energy reduction");
+  if (material == SK_SNH) elem_energy_kernel<SNH<double>><<<g, kBlock, 0, st>>>(a);
+  else if (material == SK_ARAP) elem_energy_kernel<ARAP<double>><<<g, kBlock, 0, st>>>(a);
+  else SK_REQUIRE(false, "unknown material");
+  SK_HIP_CHECK(hipGetLastError());
+  return SK_OK;
+}
+
+int launch_elem_grad(sk_mesh* m, int material, const double* x, const double* F0, const double* mu,
+                     const double* lam, const double* vol, hipStream_t st) {
+  ElemArgs a{view(m), x, F0, mu, lam, vol, m->fst.p, m->Kst.p, 0};
+  const int g = grid_for(m->nt);
+  if (material == SK_SNH) elem_grad_kernel<SNH<double>><<<g, kBlock, 0, st>>>(a);
+  else if (material == SK_ARAP) elem_grad_kernel<ARAP<double>><<<g, kBlock, 0, st>>>(a);
+  else SK_REQUIRE(false, "unknown material");
+  SK_HIP_CHECK(hipGetLastError());
+  return SK_OK;
+}
+
+template <class M>
+static void hess_dispatch(int psd, int g, hipStream_t st, const ElemArgs& a) {
+  if (psd == SK_PSD_PROJ) elem_hess_kernel<M, SK_PSD_PROJ><<<g, kBlock, 0, st>>>(a);
+  else if (psd == SK_PSD_ABS) elem_hess_kernel<M, SK_PSD_ABS><<<g, kBlock, 0, st>>>(a);
+  else elem_hess_kernel<M, SK_PSD_NONE><<<g, kBlock, 0, st>>>(a);
+}
+
+int launch_elem_hess(sk_mesh* m, int material, const double* x, const double* F0, const double* mu,
+                     const double* lam, const double* vol, int psd, hipStream_t st) {
+  SK_REQUIRE(psd == SK_PSD_NONE || psd == SK_PSD_PROJ || psd == SK_PSD_ABS, "invalid psd method");
+  ElemArgs a{view(m), x, F0, mu, lam, vol, m->fst.p, m->Kst.p, psd};
+  const int g = grid_for(m->nt);
+  if (material == SK_SNH) hess_dispatch<SNH<double>>(psd, g, st, a);
+  else if (material == SK_ARAP) hess_dispatch<ARAP<double>>(psd, g, st, a);
+  else SK_REQUIRE(false, "unknown material");
+  SK_HIP_CHECK(hipGetLastError());
+  return SK_OK;
+}
+
+int launch_gather_grad(sk_mesh* m, double* g, const double* mass, const double* x, const double* xt,
+                       const double* fext, const uint8_t* fixed, double inv_h2, hipStream_t st) {
+  GatherGArgs a{m->nv, m->vptr.p, m->vlist.p, m->fst.p, g, mass, x, xt, fext, fixed, inv_h2,
+                m->red.partials.p, m->red.counter.p, m->red.result.p};
+  const int gr = grid_for(m->nv);
+  SK_REQUIRE(gr <= Reducer::kMaxBlocks, "mesh too large for the gradient reduction");
+  gather_grad_kernel<<<gr, kBlock, 0, st>>>(a);
+  SK_HIP_CHECK(hipGetLastError());
+  return SK_OK;
+}
+
+int launch_gather_hess(sk_mesh* m, double* values, const double* mass, double inv_h2, const uint8_t* fixed,
+                       double* binv, hipStream_t st) {
+  GatherHArgs a{m->nblk, m->bptr.p, m->bcol.p, m->brow.p, m->cptr.p, m->clist.p, m->Kst.p,
+                values, mass, inv_h2, fixed, binv};
+  gather_hess_kernel<<<grid_for(m->nblk), kBlock, 0, st>>>(a);
+  SK_HIP_CHECK(hipGetLastError());
+  return SK_OK;
+}
+
+int read_scalars(sk_mesh* m, const double* dev, int n, hipStream_t st) {
+  SK_HIP_CHECK(hipMemcpyAsync(m->pinned, dev, n * sizeof(double), hipMemcpyDeviceToHost, st));
+  SK_HIP_CHECK(hipStreamSynchronize(st));
+  return SK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Host pattern build (once per mesh)
+// ---------------------------------------------------------------------------
+static int build_mesh(const int32_t* T, int64_t nv, int64_t nt, sk_mesh* m) {
+  SK_REQUIRE(nt < (int64_t(1) << 27), "sk_mesh_create: at most 2^27 tets");
+  SK_REQUIRE(nv < (int64_t(1) << 29), "sk_mesh_create: too many vertices");
+  for (int64_t k = 0; k < 4 * nt; ++k)
+    SK_REQUIRE(T[k] >= 0 && T[k] < nv, "sk_mesh_create: tet index out of range");
+
+  // vertex -> (element, local corner), ascending element order
+  std::vector<int32_t> vptr(nv + 1, 0), vlist(4 * nt);
+  for (int64_t k = 0; k < 4 * nt; ++k) vptr[T[k] + 1]++;
+  std::partial_sum(vptr.begin(), vptr.end(), vptr.begin());
+  {
+    std::vector<int32_t> fill(vptr.begin(), vptr.end() - 1);
+    for (int64_t e = 0; e < nt; ++e)
+      for (int i = 0; i < 4; ++i) vlist[fill[T[4 * e + i]]++] = (int32_t)((e << 2) | i);
+  }
+  // sorted neighbour blocks per vertex (including the vertex itself)
+  std::vector<int32_t> bptr(nv + 1, 0), bcol;
+  bcol.reserve(16 * nv);
+  std::vector<int32_t> nb;
+  for (int64_t v = 0; v < nv; ++v) {
+    nb.clear();
+    nb.push_back((int32_t)v);
+    for (int32_t k = vptr[v]; k < vptr[v + 1]; ++k) {
+      const int64_t e = vlist[k] >> 2;
+      for (int j = 0; j < 4; ++j) nb.push_back(T[4 * e + j]);
+    }
+    std::sort(nb.begin(), nb.end());
+    nb.erase(std::unique(nb.begin(), nb.end()), nb.end());
+    bcol.insert(bcol.end(), nb.begin(), nb.end());
+    bptr[v + 1] = (int32_t)bcol.size();
+  }
+  const int64_t nblk = (int64_t)bcol.size();
+  SK_REQUIRE(9 * nblk < (int64_t(1) << 31), "sk_mesh_create: nnz exceeds int32 indexing");
+  std::vector<int32_t> brow(nblk);
+  for (int64_t v = 0; v < nv; ++v)
+    for (int32_t b = bptr[v]; b < bptr[v + 1]; ++b) brow[b] = (int32_t)v;
+  // element -> block map and per-block contribution lists
+  std::vector<int32_t> emap(16 * nt), cptr(nblk + 1, 0), clist(16 * nt);
+  for (int64_t e = 0; e < nt; ++e)
+    for (int i = 0; i < 4; ++i) {
+      const int32_t v = T[4 * e + i];
+      const int32_t* lo = bcol.data() + bptr[v];
+      const int32_t* hi = bcol.data() + bptr[v + 1];
+      for (int j = 0; j < 4; ++j) {
+        const int32_t b = (int32_t)(std::lower_bound(lo, hi, T[4 * e + j]) - bcol.data());
+        emap[(4 * i + j) * nt + e] = b;
+        cptr[b + 1]++;
+      }
+    }
+  std::partial_sum(cptr.begin(), cptr.end(), cptr.begin());
+  {
+    std::vector<int32_t> fill(cptr.begin(), cptr.end() - 1);
+    for (int64_t e = 0; e < nt; ++e)
+      for (int ij = 0; ij < 16; ++ij) clist[fill[emap[ij * nt + e]]++] = (int32_t)((e << 4) | ij);
+  }
+  std::vector<int32_t> Tsoa(4 * nt);
+  for (int64_t e = 0; e < nt; ++e)
+    for (int k = 0; k < 4; ++k) Tsoa[k * nt + e] = T[4 * e + k];
+
+  m->nv = nv;
+  m->nt = nt;
+  m->nblk = nblk;
+  SK_TRY(m->T.upload(Tsoa));
+  SK_TRY(m->bptr.upload(bptr));
+  SK_TRY(m->bcol.upload(bcol));
+  SK_TRY(m->brow.upload(brow));
+  SK_TRY(m->emap.upload(emap));
+  SK_TRY(m->cptr.upload(cptr));
+  SK_TRY(m->clist.upload(clist));
+  SK_TRY(m->vptr.upload(vptr));
+  SK_TRY(m->vlist.upload(vlist));
+  SK_TRY(m->D.alloc(12 * (size_t)nt));
+  SK_TRY(m->Kst.alloc((size_t)kKst * nt));
+  SK_TRY(m->fst.alloc((size_t)kFst * nt));
+  SK_TRY(m->red.init());
+  SK_HIP_CHECK(hipHostMalloc((void**)&m->pinned, 16 * sizeof(double), hipHostMallocDefault));
+  return SK_OK;
+}
+
+}  // namespace sk
+
+using namespace sk;
+
+static hipStream_t as_stream(sk_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+extern "C" {
+
+int sk_mesh_create(const int32_t* T_host, const double* D, int64_t nv, int64_t nt, sk_mesh** out) {
+  SK_REQUIRE(out, "sk_mesh_create: null out");
+  *out = nullptr;
+  SK_REQUIRE(nv > 0 && nt > 0, "sk_mesh_create: empty mesh");
+  SK_REQUIRE(T_host && D, "sk_mesh_create: null T or D");
+  sk_mesh* m = new sk_mesh();
+  int s = build_mesh(T_host, nv, nt, m);
+  if (s == SK_OK) {
+    aos_to_soa_kernel<<<grid_for(nt), kBlock>>>(D, nt, m->D.p);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+      set_error(std::string("sk_mesh_create: ") + hipGetErrorString(e));
+      s = SK_ERR_HIP;
+    }
+  }
+  if (s != SK_OK) {
+    sk_mesh_destroy(m);
+    return s;
+  }
+  *out = m;
+  return SK_OK;
+}
+
+int sk_mesh_destroy(sk_mesh* m) {
+  if (!m) return SK_OK;
+  if (m->pinned) (void)hipHostFree(m->pinned);
+  delete m;
+  return SK_OK;
+}
+
+int sk_mesh_info(const sk_mesh* m, int64_t* nv, int64_t* nt, int64_t* nnz, int64_t* nblocks) {
+  SK_REQUIRE(m, "sk_mesh_info: null mesh");
+  if (nv) *nv = m->nv;
+  if (nt) *nt = m->nt;
+  if (nnz) *nnz = 9 * m->nblk;
+  if (nblocks) *nblocks = m->nblk;
+  return SK_OK;
+}
+
+int sk_mesh_csr_pattern(const sk_mesh* m, int32_t* indptr, int32_t* indices, sk_stream_t s) {
+  SK_REQUIRE(m, "sk_mesh_csr_pattern: null mesh");
+  const int64_t n = std::max<int64_t>(m->nv + 1, m->nblk);
+  csr_pattern_kernel<<<grid_for(n), kBlock, 0, as_stream(s)>>>(m->nv, m->nblk, m->bptr.p, m->bcol.p, m->brow.p,
+                                                                indptr, indices);
+  SK_HIP_CHECK(hipGetLastError());
+  return SK_OK;
+}
+
+int sk_mesh_element_nnz(const sk_mesh* m, int32_t* elem_nnz, sk_stream_t s) {
+  SK_REQUIRE(m && elem_nnz, "sk_mesh_element_nnz: null argument");
+  element_nnz_kernel<<<grid_for(m->nt), kBlock, 0, as_stream(s)>>>(m->nt, m->T.p, m->bptr.p, m->emap.p, elem_nnz);
+  SK_HIP_CHECK(hipGetLastError());
+  return SK_OK;
+}
+
+int sk_mesh_energy(sk_mesh* m, int material, const double* x, const double* F0, const double* mu,
+                   const double* lam, const double* vol, double* energy_host, double* elem_energy, sk_stream_t s) {
+  SK_REQUIRE(m && x && mu && vol, "sk_mesh_energy: null argument");
+  SK_REQUIRE(material == SK_ARAP || lam, "sk_mesh_energy: stable neo-Hookean needs lam");
+  hipStream_t st = as_stream(s);
+  SK_TRY(launch_elem_energy(m, material, x, F0, nullptr, 0.0, mu, lam, vol, elem_energy, nullptr, nullptr, nullptr,
+                            nullptr, 0.0, st));
+  if (energy_host) {
+    SK_TRY(read_scalars(m, m->red.result.p, 1, st));
+    *energy_host = m->pinned[0];
+  }
+  return SK_OK;
+}
+
+int sk_mesh_gradient(sk_mesh* m, int material, const double* x, const double* F0, const double* mu,
+                     const double* lam, const double* vol, double* g, sk_stream_t s) {
+  SK_REQUIRE(m && x && mu && vol && g, "sk_mesh_gradient: null argument");
+  SK_REQUIRE(material == SK_ARAP || lam, "sk_mesh_gradient: stable neo-Hookean needs lam");
+  hipStream_t st = as_stream(s);
+  SK_TRY(launch_elem_grad(m, material, x, F0, mu, lam, vol, st));
+  SK_TRY(launch_gather_grad(m, g, nullptr, nullptr, nullptr, nullptr, nullptr, 0.0, st));
+  return SK_OK;
+}
+
+int sk_mesh_hessian(sk_mesh* m, int material, const double* x, const double* F0, const double* mu,
+                    const double* lam, const double* vol, int psd, double* values, sk_stream_t s) {
+  SK_REQUIRE(m && x && mu && vol && values, "sk_mesh_hessian: null argument");
+  SK_REQUIRE(material == SK_ARAP || lam, "sk_mesh_hessian: stable neo-Hookean needs lam");
+  hipStream_t st = as_stream(s);
+  SK_TRY(launch_elem_hess(m, material, x, F0, mu, lam, vol, psd, st));
+  SK_TRY(launch_gather_hess(m, values, nullptr, 0.0, nullptr, nullptr, st));
+  return SK_OK;
+}
+
+}  // extern "C"
