@@ -132,7 +132,7 @@ class BEProblem:
         self.m = np.repeat(fem.lumped_mass(self.X, self.T, rho), 3)
         self.f_ext = fem.gravity_force(self.X, self.T, gravity, rho).reshape(-1) if gravity else np.zeros(3 * self.nv)
         self.fixed = np.zeros(3 * self.nv, bool)
-        fv = np.asarray(fixed_vertices)
+        fv = np.asarray(fixed_vertices, dtype=np.int64).reshape(-1)
         for c in range(3):
             self.fixed[3 * fv + c] = True
         self.free = ~self.fixed

@@ -77,9 +77,12 @@ def test_x_tier_matches_reference_golden(ref_assembly, mat):
         ref = sps.csr_matrix((d[f"{mat}_{name}_data"], d[f"{mat}_{name}_indices"], d[f"{mat}_{name}_indptr"]),
                              shape=Q.shape)
         assert rel_err(Q.toarray(), ref.toarray()) < TOL
-        assert rel_err(g, d[f"{mat}_{name}_g"]) < TOL
+        # At rest the gradient is zero up to roundoff on both sides; judge it on the
+        # force scale of the same mesh under the deformed state.
+        gscale = max(np.abs(d[f"{mat}_{name}_g"]).max(), np.abs(d[f"{mat}_def_g"]).max())
+        assert np.abs(g - d[f"{mat}_{name}_g"]).max() <= TOL * gscale
         Eref = float(d[f"{mat}_{name}_E"])
-        assert abs(E - Eref) <= TOL * max(abs(Eref), 1e-300)
+        assert abs(E - Eref) <= TOL * max(abs(Eref), abs(float(d[f"{mat}_def_E"])))
         # reference pattern is the structural pattern minus exact cancellations
         Qs = Q.tocoo()
         present = set(zip(ref.tocoo().row.tolist(), ref.tocoo().col.tolist()))

@@ -102,7 +102,7 @@ def test_rest_measures_and_kinematics():
     J = sk.deformation_jacobian(Xj, T)
     Jo = fem.deformation_jacobian(D, T, X.shape[0])
     assert J.shape == Jo.shape and J.format == "csc"
-    assert rel_err((J - Jo).toarray(), Jo.toarray()) < 1e-12
+    assert rel_err(J.toarray(), Jo.toarray()) < 1e-12
     A = rng.standard_normal((3, 3))
     F = (J @ (Xj @ A.T).reshape(-1)).reshape(-1, 3, 3)
     assert np.abs(F - A[None]).max() < 1e-10

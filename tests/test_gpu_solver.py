@@ -38,7 +38,7 @@ def test_pcg_solve_matches_direct_solve():
     ref = spla.spsolve(H.tocsc(), b)
     assert rel_err(x, ref) < 1e-9 and info["iters"] > 0
     Q, bq, xs = _quadratic(7, 2)
-    assert rel_err(pcg_solve(Q, bq), xs.ravel()) < 1e-10          # dense input
+    assert rel_err(pcg_solve(Q, -bq), xs.ravel()) < 1e-10         # dense input; x* = -Q^-1 b
     with pytest.raises(np.linalg.LinAlgError):
         pcg_solve(np.diag([1.0, -2.0, 3.0]), np.ones(3))             # indefinite: CG breakdown
 
