@@ -53,16 +53,15 @@ __global__ void kinetic_prep_kernel(int64_t n, const double* x, const double* xc
 // grad = c (M d) (1/h^2); energy partials of d.(M d).
 __global__ __launch_bounds__(kBlock) void kinetic_finish_kernel(int64_t n, const double* d, const double* Md,
                                                                 double c, double inv_h2, double* grad,
-                                                                double* partials, unsigned* counter,
-                                                                double* result) {
+                                                                double* partials) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double s = 0.0;
   if (k < n) {
     s = d[k] * Md[k];
     if (grad) grad[k] = c * Md[k] * inv_h2;
   }
-  double v[1] = {s}, out[1];
-  if (grid_sum<1>(v, partials, counter, out) && threadIdx.x == 0) result[0] = out[0];
+  double v[1] = {s};
+  store_block_partials<1>(v, partials);
 }
 
 }  // namespace sk
@@ -104,7 +103,6 @@ int sk_kinetic(int64_t n, const double* x, const double* x_curr, const double* x
   const bool want_d = energy_host || grad;
   SK_REQUIRE(!want_d || (x && M_indptr && M_indices && M_values), "sk_kinetic: energy/gradient need x and M");
   DevBuf<double> d, Md, partials, result;
-  DevBuf<unsigned> counter;
   SK_TRY(d.alloc(n));
   if (x_curr || x) {
     kinetic_prep_kernel<<<grid_for(n), kBlock, 0, st>>>(n, x, x_curr, x_prev, h, d.p, x_tilde, velocity);
@@ -117,10 +115,9 @@ int sk_kinetic(int64_t n, const double* x, const double* x_curr, const double* x
   SK_REQUIRE(g <= Reducer::kMaxBlocks, "sk_kinetic: vector too long");
   SK_TRY(partials.alloc(g));
   SK_TRY(result.alloc(1));
-  SK_TRY(counter.alloc(1));
-  SK_HIP_CHECK(hipMemsetAsync(counter.p, 0, sizeof(unsigned), st));
   const double inv_h2 = 1.0 / (h * h);
-  kinetic_finish_kernel<<<g, kBlock, 0, st>>>(n, d.p, Md.p, c, inv_h2, grad, partials.p, counter.p, result.p);
+  kinetic_finish_kernel<<<g, kBlock, 0, st>>>(n, d.p, Md.p, c, inv_h2, grad, partials.p);
+  finalize_sum_kernel<1><<<1, kBlock, 0, st>>>(partials.p, g, result.p);
   SK_HIP_CHECK(hipGetLastError());
   double e = 0.0;
   SK_HIP_CHECK(hipMemcpyAsync(&e, result.p, sizeof(double), hipMemcpyDeviceToHost, st));

@@ -80,40 +80,56 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds /* NV * k
   }
 }
 
-// Grid-wide sum of NV values. Every block must call it (all threads). Returns
-// true in the last block to finish; there, thread 0 holds the totals in out[].
-// partials: NV * gridDim.x doubles; counter: one zero-initialised unsigned
-// that the last block resets to zero.
+// Grid reductions without fences or atomics: every block writes its block
+// sum to partials[k * gridDim.x + blockIdx.x] with plain stores; a later
+// kernel (ordered after this one on the stream) sums the partials in a fixed
+// order, either in a dedicated one-block finalize launch or redundantly in
+// every block of the consuming kernel. Results are bit-reproducible for a
+// fixed launch shape, and no block waits on another block.
 template <int NV>
-__device__ bool grid_sum(double (&v)[NV], double* partials, unsigned* counter, double (&out)[NV]) {
+__device__ __forceinline__ void store_block_partials(double (&v)[NV], double* partials) {
   __shared__ double lds[NV * kBlock / kWave];
-  __shared__ bool is_last;
   block_sum<NV>(v, lds);
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) partials[k * gridDim.x + blockIdx.x] = v[k];
-    __threadfence();
-    unsigned ticket = atomicAdd(counter, 1u);
-    is_last = (ticket == gridDim.x - 1);
   }
-  __syncthreads();
-  if (!is_last) return false;
-  __threadfence();
+}
+
+// Whole-block sum of n partials per value (value k at partials[k * stride + i]);
+// every thread of the block receives the totals.
+template <int NV>
+__device__ __forceinline__ void sum_partials(const double* __restrict__ partials, int n, int stride,
+                                             double (&out)[NV]) {
+  __shared__ double lds[NV * kBlock / kWave];
+  __shared__ double res[NV];
   double acc[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
     double s = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
-      s += partials[k * gridDim.x + b];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += partials[k * stride + i];
     acc[k] = s;
   }
   block_sum<NV>(acc, lds);
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < NV; ++k) out[k] = acc[k];
-    *counter = 0u;
+    for (int k = 0; k < NV; ++k) res[k] = acc[k];
   }
-  return true;
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) out[k] = res[k];
+}
+
+// One-block launch: result[k] = sum of the n partials of value k.
+template <int NV>
+__global__ __launch_bounds__(kBlock) void finalize_sum_kernel(const double* __restrict__ partials, int n,
+                                                              double* __restrict__ result) {
+  double out[NV];
+  sum_partials<NV>(partials, n, n, out);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) result[k] = out[k];
+  }
 }
 
 }  // namespace sk

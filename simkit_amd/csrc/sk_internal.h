@@ -46,17 +46,14 @@ __host__ __device__ constexpr int ublock(int i, int j) { return i * (7 - i) / 2 
 constexpr int kKst = 90;  // staged doubles per element: 10 upper 3x3 blocks of K_e
 constexpr int kFst = 12;  // staged doubles per element: f_e = vol P D
 
-// Grid-reduction scratch: partial sums, a ticket counter, result slots.
+// Grid-reduction scratch: per-block partial sums and result slots.
 struct Reducer {
   DevBuf<double> partials;  // up to 8 values x kMaxBlocks
-  DevBuf<unsigned> counter;
   DevBuf<double> result;    // 8 doubles
   static constexpr int kMaxBlocks = 65536;
   int init() {
     SK_TRY(partials.alloc(8 * (size_t)kMaxBlocks));
-    SK_TRY(counter.alloc(1));
     SK_TRY(result.alloc(8));
-    SK_HIP_CHECK(hipMemset(counter.p, 0, sizeof(unsigned)));
     SK_HIP_CHECK(hipMemset(result.p, 0, 8 * sizeof(double)));
     return SK_OK;
   }

@@ -78,8 +78,6 @@ struct EnergyArgs {
   const uint8_t* fixed;
   double inv_h2;
   double* partials;
-  unsigned* counter;
-  double* result;
 };
 
 template <class M>
@@ -119,8 +117,8 @@ __global__ __launch_bounds__(kBlock) void elem_energy_kernel(EnergyArgs a) {
       if (a.fext) acc -= a.fext[k] * (x1 - x0);
     }
   }
-  double v1[1] = {acc}, out[1];
-  if (grid_sum<1>(v1, a.partials, a.counter, out) && threadIdx.x == 0) a.result[0] = out[0];
+  double v1[1] = {acc};
+  store_block_partials<1>(v1, a.partials);
 }
 
 struct ElemArgs {
@@ -207,8 +205,6 @@ struct GatherGArgs {
   const uint8_t* fixed;
   double inv_h2;
   double* partials;
-  unsigned* counter;
-  double* result;
 };
 
 __global__ __launch_bounds__(kBlock) void gather_grad_kernel(GatherGArgs a) {
@@ -238,8 +234,8 @@ __global__ __launch_bounds__(kBlock) void gather_grad_kernel(GatherGArgs a) {
     }
   }
   if (a.mass) {
-    double v1[1] = {nrm}, out[1];
-    if (grid_sum<1>(v1, a.partials, a.counter, out) && threadIdx.x == 0) a.result[0] = out[0];
+    double v1[1] = {nrm};
+    store_block_partials<1>(v1, a.partials);
   }
 }
 
@@ -372,13 +368,14 @@ int launch_elem_energy(sk_mesh* m, int material, const double* x, const double* 
                        const double* xt, const double* fext, const uint8_t* fixed, double inv_h2,
                        hipStream_t st) {
   EnergyArgs a{view(m), m->nv, x, F0, dx, t, mu, lam, vol, elem_energy, mass, xt, fext, fixed, inv_h2,
-               m->red.partials.p, m->red.counter.p, m->red.result.p};
+               m->red.partials.p};
   const int64_t n = std::max<int64_t>(m->nt, mass ? m->nv : 0);
   const int g = grid_for(n);
   SK_REQUIRE(g <= Reducer::kMaxBlocks, "mesh too large for the energy reduction");
   if (material == SK_SNH) elem_energy_kernel<SNH<double>><<<g, kBlock, 0, st>>>(a);
   else if (material == SK_ARAP) elem_energy_kernel<ARAP<double>><<<g, kBlock, 0, st>>>(a);
   else SK_REQUIRE(false, "unknown material");
+  finalize_sum_kernel<1><<<1, kBlock, 0, st>>>(m->red.partials.p, g, m->red.result.p);
   SK_HIP_CHECK(hipGetLastError());
   return SK_OK;
 }
@@ -415,11 +412,11 @@ int launch_elem_hess(sk_mesh* m, int material, const double* x, const double* F0
 
 int launch_gather_grad(sk_mesh* m, double* g, const double* mass, const double* x, const double* xt,
                        const double* fext, const uint8_t* fixed, double inv_h2, hipStream_t st) {
-  GatherGArgs a{m->nv, m->vptr.p, m->vlist.p, m->fst.p, g, mass, x, xt, fext, fixed, inv_h2,
-                m->red.partials.p, m->red.counter.p, m->red.result.p};
+  GatherGArgs a{m->nv, m->vptr.p, m->vlist.p, m->fst.p, g, mass, x, xt, fext, fixed, inv_h2, m->red.partials.p};
   const int gr = grid_for(m->nv);
   SK_REQUIRE(gr <= Reducer::kMaxBlocks, "mesh too large for the gradient reduction");
   gather_grad_kernel<<<gr, kBlock, 0, st>>>(a);
+  if (mass) finalize_sum_kernel<1><<<1, kBlock, 0, st>>>(m->red.partials.p, gr, m->red.result.p);
   SK_HIP_CHECK(hipGetLastError());
   return SK_OK;
 }

@@ -1,12 +1,19 @@
 // Preconditioned conjugate gradients and the device backward-Euler driver.
 //
-// PCG iteration = 2 launches. The search-direction update p = z + beta p is
-// folded into the SpMV: the SpMV lanes read z and the previous p (ping-pong
-// buffers) and form p on the fly for every column they touch, the owning lane
-// writes the new p. The second launch updates x, r, z and reduces r.z and
-// r.r. All scalars (alpha, beta, r.z, counters, convergence flag) live in a
-// device struct; both launches return at once when the flag is set, so the
-// host can enqueue iterations in chunks and only reads the flag per chunk.
+// PCG iteration k = 2 launches, no fences, no atomics:
+//   spmv(k):   every block first sums the previous launch's per-block
+//              partials (r.z, r.r) in a fixed order, derives beta and the
+//              convergence test itself (all blocks get identical values),
+//              then forms p = z + beta p_prev on the fly for every column it
+//              touches (ping-pong p buffers), writes the new p and A p for its
+//              rows and one partial of p.Ap.
+//   update(k): every block sums the spmv partials (p.Ap), alpha = r.z / p.Ap,
+//              updates x, r, z = B^-1 r for its vertices and writes partials
+//              of the new r.z and r.r.
+// Block 0 of each launch records the scalars (r.z by iteration parity, r.r,
+// iteration count, stop flag) in a device struct that only later launches
+// read, so every launch returns at once after convergence and the host reads
+// the struct once per chunk of enqueued iterations.
 #include "sk_internal.h"
 
 #include <cmath>
@@ -14,11 +21,85 @@
 namespace sk {
 
 struct PcgState {
-  double rz, pAp, alpha, beta, rr, tol2, bb;
-  int iter, done;  // done: 0 running, 1 converged, 2 breakdown, 3 non-finite
+  double rz[2];  // r.z of iteration k in slot k & 1
+  double tol2, rr, bb, pAp;
+  int iter, done;  // done: 0 running, 1 converged, 2 breakdown (p.Ap <= 0), 3 non-finite
 };
 
-constexpr int kRowLanes = 8;  // lanes cooperating on one (block) row in SpMV
+constexpr int kRowLanes = 8;      // lanes cooperating on one (block) row in SpMV
+constexpr int kSpmvBlocks = 1024; // SpMV grid cap; blocks walk contiguous row chunks
+
+// Logical block index with consecutive logical blocks on the same XCD: the
+// hardware deals workgroups round-robin over the 8 XCDs, so XCD j receives
+// hardware blocks j, j+8, ... and is given the contiguous logical range
+// [j*G/8, (j+1)*G/8). Row chunks processed by one XCD then share its L2.
+__device__ __forceinline__ int xcd_block(int b, int G) {
+  if (G % 8) return b;
+  return (b % 8) * (G / 8) + b / 8;
+}
+
+// Shared prologue of the SpMV launches: reduce the update partials, decide
+// convergence, compute beta. Returns false if the iteration must not run.
+struct PcgCtl {
+  double beta;
+  bool run;
+};
+
+// Block 0 of a launch may set st->done while other blocks of the same launch
+// are starting, so the flag is read once per block (thread 0, through LDS):
+// every thread of a block takes the same branch around the barriers below.
+__device__ __forceinline__ bool block_reads_done(const PcgState* st) {
+  __shared__ int s_done;
+  if (threadIdx.x == 0) s_done = st->done;
+  __syncthreads();
+  return s_done != 0;
+}
+
+__device__ __forceinline__ PcgCtl spmv_prologue(PcgState* st, const double* upart, int gu, int k, double rtol2) {
+  PcgCtl c{0.0, false};
+  if (block_reads_done(st)) return c;
+  double red[2];
+  sum_partials<2>(upart, gu, gu, red);
+  const double rz = red[0], rr = red[1];
+  const double tol2 = (k == 0) ? rtol2 * rr : st->tol2;
+  const bool finite = isfinite(rr) && isfinite(rz);
+  const bool conv = finite && !(rr > tol2);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->rz[k & 1] = rz;
+    st->rr = rr;
+    if (k == 0) {
+      st->tol2 = tol2;
+      st->bb = rr;
+    }
+    if (!finite) st->done = 3;
+    else if (conv) st->done = 1;
+  }
+  if (!finite || conv) return c;
+  c.beta = (k == 0) ? 0.0 : rz / st->rz[(k - 1) & 1];
+  c.run = true;
+  return c;
+}
+
+// Shared prologue of the update launches: reduce p.Ap, detect breakdown.
+__device__ __forceinline__ bool update_prologue(PcgState* st, const double* spart, int gs, int k, double& alpha) {
+  if (block_reads_done(st)) return false;
+  double red[1];
+  sum_partials<1>(spart, gs, gs, red);
+  const double pAp = red[0];
+  if (!(pAp > 0.0)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->pAp = pAp;
+      st->done = isfinite(pAp) ? 2 : 3;
+    }
+    return false;
+  }
+  alpha = st->rz[k & 1] / pAp;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->pAp = pAp;
+    st->iter = k + 1;
+  }
+  return true;
+}
 
 // ---------------------------------------------------------------------------
 // FEM operator: sorted block-CSR pattern read from the scalar CSR values.
@@ -41,8 +122,11 @@ struct BlockPcgArgs {
   double* Ap;
   PcgState* st;
   double rtol2;
-  double* partials;
-  unsigned* counter;
+  double* upart;  // [2][gu] partials of r.z and r.r
+  int gu;
+  double* spart;  // [gs] partials of p.Ap
+  int gs;
+  int k;          // iteration index of this launch
 };
 
 __device__ __forceinline__ void apply_binv(const double* __restrict__ binv, int64_t v, const double (&r)[3],
@@ -74,67 +158,62 @@ __global__ __launch_bounds__(kBlock) void bpcg_init_kernel(BlockPcgArgs a) {
       rr += r[p] * r[p];
     }
   }
-  double vals[2] = {rz, rr}, out[2];
-  if (grid_sum<2>(vals, a.partials, a.counter, out) && threadIdx.x == 0) {
-    PcgState* s = a.st;
-    s->rz = out[0];
-    s->rr = out[1];
-    s->bb = out[1];
-    s->tol2 = a.rtol2 * out[1];
-    s->beta = 0.0;
-    s->alpha = 0.0;
-    s->pAp = 0.0;
-    s->iter = 0;
-    s->done = (out[1] == 0.0) ? 1 : (isfinite(out[1]) ? 0 : 3);
+  double vals[2] = {rz, rr};
+  store_block_partials<2>(vals, a.upart);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.st->done = 0;
+    a.st->iter = 0;
   }
 }
 
 __global__ __launch_bounds__(kBlock) void bpcg_spmv_kernel(BlockPcgArgs a) {
-  if (a.st->done) return;
-  const double beta = a.st->beta;
-  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t v = gid / kRowLanes;
-  const int lane = (int)(gid % kRowLanes);
-  double acc[3] = {0.0, 0.0, 0.0};
+  const PcgCtl c = spmv_prologue(a.st, a.upart, a.gu, a.k, a.rtol2);
+  if (!c.run) return;
+  const double beta = c.beta;
+  constexpr int rows_per_pass = kBlock / kRowLanes;
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t chunk = (a.nv + gridDim.x - 1) / gridDim.x;
+  const int64_t v_begin = lb * chunk;
+  const int64_t v_end = v_begin + chunk < a.nv ? v_begin + chunk : a.nv;
+  const int lane = threadIdx.x % kRowLanes;
   double dot = 0.0;
-  if (v < a.nv) {
-    const int32_t b0 = a.bptr[v];
-    const int32_t len = 3 * (a.bptr[v + 1] - b0);
-    const double* __restrict__ row = a.vals + 9 * (int64_t)b0;
-    for (int32_t m = lane; m < len; m += kRowLanes) {
-      const int32_t rb = m / 3;
-      const int64_t col = 3 * (int64_t)a.bcol[b0 + rb] + (m - 3 * rb);
-      const double pv = a.z[col] + beta * a.pin[col];
-      acc[0] += row[m] * pv;
-      acc[1] += row[len + m] * pv;
-      acc[2] += row[2 * len + m] * pv;
+  for (int64_t v0 = v_begin; v0 < v_end; v0 += rows_per_pass) {
+    const int64_t v = v0 + threadIdx.x / kRowLanes;
+    double acc[3] = {0.0, 0.0, 0.0};
+    if (v < v_end) {
+      const int32_t b0 = a.bptr[v];
+      const int32_t len = 3 * (a.bptr[v + 1] - b0);
+      const double* __restrict__ row = a.vals + 9 * (int64_t)b0;
+      for (int32_t m = lane; m < len; m += kRowLanes) {
+        const int32_t rb = m / 3;
+        const int64_t col = 3 * (int64_t)a.bcol[b0 + rb] + (m - 3 * rb);
+        const double pv = a.z[col] + beta * a.pin[col];
+        acc[0] += row[m] * pv;
+        acc[1] += row[len + m] * pv;
+        acc[2] += row[2 * len + m] * pv;
+      }
+    }
+#pragma unroll
+    for (int off = kRowLanes / 2; off > 0; off >>= 1) {
+#pragma unroll
+      for (int p = 0; p < 3; ++p) acc[p] += __shfl_xor(acc[p], off, kRowLanes);
+    }
+    if (v < v_end && lane < 3) {
+      const int64_t k = 3 * v + lane;
+      const double pn = a.z[k] + beta * a.pin[k];
+      const double ap = (a.fixed && a.fixed[k]) ? 0.0 : (lane == 0 ? acc[0] : (lane == 1 ? acc[1] : acc[2]));
+      a.pout[k] = pn;
+      a.Ap[k] = ap;
+      dot += pn * ap;
     }
   }
-#pragma unroll
-  for (int off = kRowLanes / 2; off > 0; off >>= 1) {
-#pragma unroll
-    for (int p = 0; p < 3; ++p) acc[p] += __shfl_xor(acc[p], off, kRowLanes);
-  }
-  if (v < a.nv && lane < 3) {
-    const int64_t k = 3 * v + lane;
-    const double pn = a.z[k] + beta * a.pin[k];
-    const double ap = (a.fixed && a.fixed[k]) ? 0.0 : acc[lane];
-    a.pout[k] = pn;
-    a.Ap[k] = ap;
-    dot = pn * ap;
-  }
-  double vals[1] = {dot}, out[1];
-  if (grid_sum<1>(vals, a.partials, a.counter, out) && threadIdx.x == 0) {
-    PcgState* s = a.st;
-    s->pAp = out[0];
-    if (!(out[0] > 0.0)) s->done = isfinite(out[0]) ? 2 : 3;
-    else s->alpha = s->rz / out[0];
-  }
+  double vals[1] = {dot};
+  store_block_partials<1>(vals, a.spart);
 }
 
 __global__ __launch_bounds__(kBlock) void bpcg_update_kernel(BlockPcgArgs a) {
-  if (a.st->done) return;
-  const double alpha = a.st->alpha;
+  double alpha;
+  if (!update_prologue(a.st, a.spart, a.gs, a.k, alpha)) return;
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double rz = 0.0, rr = 0.0;
   if (v < a.nv) {
@@ -155,16 +234,8 @@ __global__ __launch_bounds__(kBlock) void bpcg_update_kernel(BlockPcgArgs a) {
       rr += r[p] * r[p];
     }
   }
-  double vals[2] = {rz, rr}, out[2];
-  if (grid_sum<2>(vals, a.partials, a.counter, out) && threadIdx.x == 0) {
-    PcgState* s = a.st;
-    s->beta = out[0] / s->rz;
-    s->rz = out[0];
-    s->rr = out[1];
-    s->iter += 1;
-    if (!isfinite(out[1])) s->done = 3;
-    else if (out[1] <= s->tol2) s->done = 1;
-  }
+  double vals[2] = {rz, rr};
+  store_block_partials<2>(vals, a.upart);
 }
 
 // ---------------------------------------------------------------------------
@@ -185,8 +256,11 @@ struct CsrPcgArgs {
   double* Ap;
   PcgState* st;
   double rtol2;
-  double* partials;
-  unsigned* counter;
+  double* upart;
+  int gu;
+  double* spart;
+  int gs;
+  int k;
 };
 
 __global__ __launch_bounds__(kBlock) void cpcg_init_kernel(CsrPcgArgs a) {
@@ -207,54 +281,49 @@ __global__ __launch_bounds__(kBlock) void cpcg_init_kernel(CsrPcgArgs a) {
     rz = r * z;
     rr = r * r;
   }
-  double vals[2] = {rz, rr}, out[2];
-  if (grid_sum<2>(vals, a.partials, a.counter, out) && threadIdx.x == 0) {
-    PcgState* s = a.st;
-    s->rz = out[0];
-    s->rr = out[1];
-    s->bb = out[1];
-    s->tol2 = a.rtol2 * out[1];
-    s->beta = 0.0;
-    s->alpha = 0.0;
-    s->pAp = 0.0;
-    s->iter = 0;
-    s->done = (out[1] == 0.0) ? 1 : (isfinite(out[1]) ? 0 : 3);
+  double vals[2] = {rz, rr};
+  store_block_partials<2>(vals, a.upart);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.st->done = 0;
+    a.st->iter = 0;
   }
 }
 
 __global__ __launch_bounds__(kBlock) void cpcg_spmv_kernel(CsrPcgArgs a) {
-  if (a.st->done) return;
-  const double beta = a.st->beta;
-  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t i = gid / kRowLanes;
-  const int lane = (int)(gid % kRowLanes);
-  double acc = 0.0, dot = 0.0;
-  if (i < a.n) {
-    for (int32_t k = a.indptr[i] + lane; k < a.indptr[i + 1]; k += kRowLanes) {
-      const int64_t j = a.indices[k];
-      acc += a.vals[k] * (a.z[j] + beta * a.pin[j]);
+  const PcgCtl c = spmv_prologue(a.st, a.upart, a.gu, a.k, a.rtol2);
+  if (!c.run) return;
+  const double beta = c.beta;
+  constexpr int rows_per_pass = kBlock / kRowLanes;
+  const int64_t chunk = (a.n + gridDim.x - 1) / gridDim.x;
+  const int64_t i_begin = blockIdx.x * chunk;
+  const int64_t i_end = i_begin + chunk < a.n ? i_begin + chunk : a.n;
+  const int lane = threadIdx.x % kRowLanes;
+  double dot = 0.0;
+  for (int64_t i0 = i_begin; i0 < i_end; i0 += rows_per_pass) {
+    const int64_t i = i0 + threadIdx.x / kRowLanes;
+    double acc = 0.0;
+    if (i < i_end) {
+      for (int32_t k = a.indptr[i] + lane; k < a.indptr[i + 1]; k += kRowLanes) {
+        const int64_t j = a.indices[k];
+        acc += a.vals[k] * (a.z[j] + beta * a.pin[j]);
+      }
+    }
+#pragma unroll
+    for (int off = kRowLanes / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, kRowLanes);
+    if (i < i_end && lane == 0) {
+      const double pn = a.z[i] + beta * a.pin[i];
+      a.pout[i] = pn;
+      a.Ap[i] = acc;
+      dot += pn * acc;
     }
   }
-#pragma unroll
-  for (int off = kRowLanes / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, kRowLanes);
-  if (i < a.n && lane == 0) {
-    const double pn = a.z[i] + beta * a.pin[i];
-    a.pout[i] = pn;
-    a.Ap[i] = acc;
-    dot = pn * acc;
-  }
-  double vals[1] = {dot}, out[1];
-  if (grid_sum<1>(vals, a.partials, a.counter, out) && threadIdx.x == 0) {
-    PcgState* s = a.st;
-    s->pAp = out[0];
-    if (!(out[0] > 0.0)) s->done = isfinite(out[0]) ? 2 : 3;
-    else s->alpha = s->rz / out[0];
-  }
+  double vals[1] = {dot};
+  store_block_partials<1>(vals, a.spart);
 }
 
 __global__ __launch_bounds__(kBlock) void cpcg_update_kernel(CsrPcgArgs a) {
-  if (a.st->done) return;
-  const double alpha = a.st->alpha;
+  double alpha;
+  if (!update_prologue(a.st, a.spart, a.gs, a.k, alpha)) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double rz = 0.0, rr = 0.0;
   if (i < a.n) {
@@ -266,16 +335,8 @@ __global__ __launch_bounds__(kBlock) void cpcg_update_kernel(CsrPcgArgs a) {
     rz = r * z;
     rr = r * r;
   }
-  double vals[2] = {rz, rr}, out[2];
-  if (grid_sum<2>(vals, a.partials, a.counter, out) && threadIdx.x == 0) {
-    PcgState* s = a.st;
-    s->beta = out[0] / s->rz;
-    s->rz = out[0];
-    s->rr = out[1];
-    s->iter += 1;
-    if (!isfinite(out[1])) s->done = 3;
-    else if (out[1] <= s->tol2) s->done = 1;
-  }
+  double vals[2] = {rz, rr};
+  store_block_partials<2>(vals, a.upart);
 }
 
 // ---------------------------------------------------------------------------
@@ -298,11 +359,10 @@ __global__ void axpy_kernel(int64_t n, double t, const double* dx, double* x) {
   if (k < n) x[k] = x[k] + t * dx[k];
 }
 
-__global__ __launch_bounds__(kBlock) void dot_kernel(int64_t n, const double* a, const double* b, double* partials,
-                                                     unsigned* counter, double* result) {
+__global__ __launch_bounds__(kBlock) void dot_kernel(int64_t n, const double* a, const double* b, double* partials) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  double v[1] = {k < n ? a[k] * b[k] : 0.0}, out[1];
-  if (grid_sum<1>(v, partials, counter, out) && threadIdx.x == 0) result[0] = out[0];
+  double v[1] = {k < n ? a[k] * b[k] : 0.0};
+  store_block_partials<1>(v, partials);
 }
 
 }  // namespace sk
@@ -333,25 +393,36 @@ int copy_dev(DevBuf<double>& dst, const double* src, size_t n) {
 
 constexpr int kPcgChunk = 16;
 
+int spmv_grid(int64_t rows) {
+  const int64_t g = (rows + (kBlock / kRowLanes) - 1) / (kBlock / kRowLanes);
+  return (int)(g < kSpmvBlocks ? (g < 1 ? 1 : g) : kSpmvBlocks);
+}
+
+BlockPcgArgs block_pcg_args(sk_sim* S, const double* b, double rtol2) {
+  sk_mesh* m = S->mesh;
+  const int gu = grid_for(m->nv);
+  const int gs = spmv_grid(m->nv);
+  double* part = m->red.partials.p;
+  return BlockPcgArgs{m->nv, m->bptr.p, m->bcol.p, S->vals.p, S->binv.p, S->fixed.p, b, S->dx.p, S->r.p, S->z.p,
+                      S->p1.p, S->p0.p, S->Ap.p, S->st.p, rtol2, part, gu, part + 2 * gu, gs, 0};
+}
+
 // Runs block-Jacobi PCG on the sim's assembled system with right-hand side b.
 int run_block_pcg(sk_sim* S, const double* b, hipStream_t st, int64_t* iters, int* done) {
   sk_mesh* m = S->mesh;
-  BlockPcgArgs a{m->nv, m->bptr.p, m->bcol.p, S->vals.p, S->binv.p, S->fixed.p, b, S->dx.p, S->r.p, S->z.p,
-                 S->p0.p, S->p1.p, S->Ap.p, S->st.p, S->prm.pcg_rtol * S->prm.pcg_rtol,
-                 m->red.partials.p, m->red.counter.p};
-  const int gv = grid_for(m->nv);
-  const int gs = grid_for(m->nv * kRowLanes);
-  SK_REQUIRE(gs <= Reducer::kMaxBlocks, "mesh too large for the PCG reductions");
-  a.pout = S->p0.p;  // init zeroes p0, which is the first "previous p"
-  bpcg_init_kernel<<<gv, kBlock, 0, st>>>(a);
+  BlockPcgArgs a = block_pcg_args(S, b, S->prm.pcg_rtol * S->prm.pcg_rtol);
+  SK_REQUIRE(2 * a.gu + a.gs <= 8 * Reducer::kMaxBlocks, "mesh too large for the PCG reductions");
+  a.pout = S->p0.p;  // init zeroes p0, the first "previous p"
+  bpcg_init_kernel<<<a.gu, kBlock, 0, st>>>(a);
   SK_HIP_CHECK(hipGetLastError());
   int64_t launched = 0;
   for (;;) {
     for (int k = 0; k < kPcgChunk; ++k, ++launched) {
+      a.k = (int)launched;
       a.pin = (launched & 1) ? S->p1.p : S->p0.p;
       a.pout = (launched & 1) ? S->p0.p : S->p1.p;
-      bpcg_spmv_kernel<<<gs, kBlock, 0, st>>>(a);
-      bpcg_update_kernel<<<gv, kBlock, 0, st>>>(a);
+      bpcg_spmv_kernel<<<a.gs, kBlock, 0, st>>>(a);
+      bpcg_update_kernel<<<a.gu, kBlock, 0, st>>>(a);
     }
     SK_HIP_CHECK(hipGetLastError());
     SK_HIP_CHECK(hipMemcpyAsync(S->st_host, S->st.p, sizeof(PcgState), hipMemcpyDeviceToHost, st));
@@ -359,6 +430,7 @@ int run_block_pcg(sk_sim* S, const double* b, hipStream_t st, int64_t* iters, in
     if (S->st_host->done || S->st_host->iter >= S->prm.pcg_max_iter) break;
     if (launched > (int64_t)S->prm.pcg_max_iter + kPcgChunk) break;  // device state not advancing
   }
+  (void)m;
   *iters = S->st_host->iter;
   *done = S->st_host->done;
   return SK_OK;
@@ -384,7 +456,6 @@ int sk_pcg_csr(int64_t n, const int32_t* indptr, const int32_t* indices, const d
   SK_REQUIRE(indptr && indices && values && b && x, "sk_pcg_csr: null pointer");
   hipStream_t st = as_stream(s);
   DevBuf<double> dinv, r, z, p0, p1, Ap, partials;
-  DevBuf<unsigned> counter;
   DevBuf<PcgState> state;
   SK_TRY(dinv.alloc(n));
   SK_TRY(r.alloc(n));
@@ -392,25 +463,24 @@ int sk_pcg_csr(int64_t n, const int32_t* indptr, const int32_t* indices, const d
   SK_TRY(p0.alloc(n));
   SK_TRY(p1.alloc(n));
   SK_TRY(Ap.alloc(n));
-  const int gv = grid_for(n);
-  const int gs = grid_for(n * kRowLanes);
-  SK_REQUIRE(gs <= Reducer::kMaxBlocks, "sk_pcg_csr: system too large");
-  SK_TRY(partials.alloc(2 * (size_t)Reducer::kMaxBlocks));
-  SK_TRY(counter.alloc(1));
+  const int gu = grid_for(n);
+  const int gs = spmv_grid(n);
+  SK_REQUIRE(gu <= Reducer::kMaxBlocks, "sk_pcg_csr: system too large");
+  SK_TRY(partials.alloc(2 * (size_t)gu + gs));
   SK_TRY(state.alloc(1));
-  SK_HIP_CHECK(hipMemsetAsync(counter.p, 0, sizeof(unsigned), st));
   CsrPcgArgs a{n, indptr, indices, values, dinv.p, b, x, r.p, z.p, p1.p, p0.p, Ap.p, state.p, rtol * rtol,
-               partials.p, counter.p};
-  cpcg_init_kernel<<<gv, kBlock, 0, st>>>(a);
+               partials.p, gu, partials.p + 2 * gu, gs, 0};
+  cpcg_init_kernel<<<gu, kBlock, 0, st>>>(a);
   SK_HIP_CHECK(hipGetLastError());
   PcgState hs{};
   int64_t launched = 0;
   for (;;) {
     for (int k = 0; k < kPcgChunk; ++k, ++launched) {
+      a.k = (int)launched;
       a.pin = (launched & 1) ? p1.p : p0.p;
       a.pout = (launched & 1) ? p0.p : p1.p;
       cpcg_spmv_kernel<<<gs, kBlock, 0, st>>>(a);
-      cpcg_update_kernel<<<gv, kBlock, 0, st>>>(a);
+      cpcg_update_kernel<<<gu, kBlock, 0, st>>>(a);
     }
     SK_HIP_CHECK(hipGetLastError());
     SK_HIP_CHECK(hipMemcpyAsync(&hs, state.p, sizeof(PcgState), hipMemcpyDeviceToHost, st));
@@ -535,8 +605,8 @@ int sk_sim_step(sk_sim* S, const double* x_curr, const double* x_prev, const dou
     if (done == 3) inf.status = SK_ERR_NONFINITE;
     if (inf.status != SK_OK) break;
     if (P.timing) SK_HIP_CHECK(hipEventRecord(S->ev[2], st));
-    dot_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, S->g.p, S->dx.p, m->red.partials.p, m->red.counter.p,
-                                                 m->red.result.p);
+    dot_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, S->g.p, S->dx.p, m->red.partials.p);
+    finalize_sum_kernel<1><<<1, kBlock, 0, st>>>(m->red.partials.p, grid_for(n3), m->red.result.p);
     SK_TRY(read_scalars(m, m->red.result.p, 1, st));
     const double slope = m->pinned[0];
     double t = 1.0;
@@ -583,18 +653,15 @@ int sk_sim_step(sk_sim* S, const double* x_curr, const double* x_prev, const dou
 
 int sk_sim_spmv_reps(sk_sim* S, int reps, sk_stream_t s) {
   SK_REQUIRE(S && reps >= 0, "sk_sim_spmv_reps: invalid argument");
-  sk_mesh* m = S->mesh;
   hipStream_t st = as_stream(s);
-  // Per-step scratch vectors only (dx, r, z, p0, p1, Ap); RHS = the last gradient.
-  BlockPcgArgs a{m->nv, m->bptr.p, m->bcol.p, S->vals.p, S->binv.p, S->fixed.p, S->g.p, S->dx.p, S->r.p, S->z.p,
-                 S->p1.p, S->p0.p, S->Ap.p, S->st.p, 0.0, m->red.partials.p, m->red.counter.p};
-  bpcg_init_kernel<<<grid_for(m->nv), kBlock, 0, st>>>(a);
-  const int gs = grid_for(m->nv * kRowLanes);
-  for (int k = 0; k < reps; ++k) {
-    a.pin = (k & 1) ? S->p1.p : S->p0.p;
-    a.pout = (k & 1) ? S->p0.p : S->p1.p;
-    bpcg_spmv_kernel<<<gs, kBlock, 0, st>>>(a);
-  }
+  // Per-step scratch vectors only (dx, r, z, p0, p1, Ap); RHS = the last
+  // gradient; rtol 0 so every launch runs iteration 0 in full.
+  BlockPcgArgs a = block_pcg_args(S, S->g.p, 0.0);
+  a.pout = S->p0.p;
+  bpcg_init_kernel<<<a.gu, kBlock, 0, st>>>(a);
+  a.pin = S->p0.p;
+  a.pout = S->p1.p;
+  for (int k = 0; k < reps; ++k) bpcg_spmv_kernel<<<a.gs, kBlock, 0, st>>>(a);
   SK_HIP_CHECK(hipGetLastError());
   return SK_OK;
 }

@@ -40,7 +40,7 @@ def test_pcg_solve_matches_direct_solve():
     Q, bq, xs = _quadratic(7, 2)
     assert rel_err(pcg_solve(Q, -bq), xs.ravel()) < 1e-10         # dense input; x* = -Q^-1 b
     with pytest.raises(np.linalg.LinAlgError):
-        pcg_solve(np.diag([1.0, -2.0, 3.0]), np.ones(3))             # indefinite: CG breakdown
+        pcg_solve(np.array([[1.0, 2.0], [2.0, 1.0]]), np.array([1.0, -1.0]))  # p^T A p = -2: breakdown
 
 
 def test_newton_and_line_search_reference_cases():
