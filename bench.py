@@ -43,11 +43,11 @@ def build_workload(cells):
 
 
 def spmv_bytes(nv, nblk):
-    """Algorithmic HBM bytes of one block-CSR SpMV launch: FP64 values (9 per block),
-    int32 block columns and row pointers, the z and previous-p vectors read, the new p
+    """Algorithmic HBM bytes of one BSR SpMV launch: FP64 values (9 per block),
+    int32 block column and block row per block, row pointers, the z and previous-p vectors read, the new p
     and A p written, and the 1-byte fixed-DOF mask."""
     n3 = 3 * nv
-    return 9 * nblk * 8 + nblk * 4 + (nv + 1) * 4 + 4 * n3 * 8 + n3
+    return 9 * nblk * 8 + 2 * nblk * 4 + (nv + 1) * 4 + 4 * n3 * 8 + n3
 
 
 def cpu_baseline(X, T, bottom, top, mu, lam, gpu_infos, steps, pcg_sample):
@@ -96,6 +96,9 @@ def main():
     ap.add_argument("--spmv-reps", type=int, default=200)
     ap.add_argument("--cpu-pcg-sample", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--inner-rtol", type=float, default=1e-4)
+    ap.add_argument("--spmv-blocks", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -114,7 +117,8 @@ def main():
     t_setup = time.perf_counter()
     sim = BackwardEulerSim(X, T, mu, lam, rho=1000.0, h=1.0 / 60.0, material="snh",
                            fixed_vertices=np.concatenate([bottom, top]), pcg_rtol=1e-8, newton_rtol=1e-6,
-                           timing=True)
+                           timing=True, precision=args.precision, inner_rtol=args.inner_rtol,
+                           spmv_blocks=args.spmv_blocks)
     bc = top_face_sinusoid(X, top, 0.1, 1.0)
     torch.cuda.synchronize()
     log(f"bench: setup {time.perf_counter() - t_setup:.1f}s; warmup {args.warmup} steps")

@@ -43,6 +43,11 @@ enum sk_status {
 };
 
 enum sk_material { SK_SNH = 0, SK_ARAP = 1 };
+enum sk_precision {
+  SK_PREC_FP64 = 0,  /* every stage in FP64 */
+  SK_PREC_MIXED = 1  /* FP32 element Hessian, PSD projection, assembled values and inner PCG, wrapped in FP64
+                        iterative refinement; FP64 gradient, energies and near-inverted ARAP polar SVD */
+};
 enum sk_psd_method { SK_PSD_NONE = 0, SK_PSD_PROJ = 1, SK_PSD_ABS = 2 };
 
 const char* sk_last_error(void);
@@ -185,6 +190,10 @@ typedef struct sk_sim_params {
   int ls_max_iter;
   double ls_threshold;
   int timing;            /* 1: record per-phase times in sk_step_info */
+  int precision;         /* sk_precision */
+  double inner_rtol;     /* SK_PREC_MIXED: relative tolerance of each inner FP32 PCG solve */
+  int max_refine;        /* SK_PREC_MIXED: refinement steps per Newton solve at most */
+  int spmv_blocks;       /* SpMV launch width (workgroups); 0 = default */
 } sk_sim_params;
 
 typedef struct sk_step_info {
@@ -198,6 +207,7 @@ typedef struct sk_step_info {
   double ms_hessian;
   double ms_pcg;
   double ms_line_search;
+  int refine_steps;      /* SK_PREC_MIXED: refinement steps summed over the Newton iterations */
 } sk_step_info;
 
 /* mu, lam, vol: per element; mass: per DOF (3nv, M = kron(M_lumped, I3));

@@ -24,6 +24,9 @@ SK_ERR_UNSUPPORTED = -9
 
 SK_SNH = 0
 SK_ARAP = 1
+SK_PREC_FP64 = 0
+SK_PREC_MIXED = 1
+
 SK_PSD_NONE = 0
 SK_PSD_PROJ = 1
 SK_PSD_ABS = 2
@@ -53,6 +56,10 @@ class SkSimParams(ctypes.Structure):
         ("ls_max_iter", ctypes.c_int),
         ("ls_threshold", ctypes.c_double),
         ("timing", ctypes.c_int),
+        ("precision", ctypes.c_int),
+        ("inner_rtol", ctypes.c_double),
+        ("max_refine", ctypes.c_int),
+        ("spmv_blocks", ctypes.c_int),
     ]
 
 
@@ -68,6 +75,7 @@ class SkStepInfo(ctypes.Structure):
         ("ms_hessian", ctypes.c_double),
         ("ms_pcg", ctypes.c_double),
         ("ms_line_search", ctypes.c_double),
+        ("refine_steps", ctypes.c_int),
     ]
 
     def as_dict(self):

@@ -44,6 +44,8 @@ struct DevBuf {
 __host__ __device__ constexpr int ublock(int i, int j) { return i * (7 - i) / 2 + j; }
 
 constexpr int kKst = 90;  // staged doubles per element: 10 upper 3x3 blocks of K_e
+constexpr int kSellC = 16;       // rows per SELL chunk
+constexpr int kSellLanes = 4;    // lanes per row in the SELL SpMV (kSellC * kSellLanes = wave size)
 constexpr int kFst = 12;  // staged doubles per element: f_e = vol P D
 
 // Grid-reduction scratch: per-block partial sums and result slots.
@@ -73,6 +75,20 @@ struct sk_mesh {
   sk::DevBuf<int32_t> clist;  // [16 nt] (e << 4) | (4 i + j), e ascending per block
   sk::DevBuf<int32_t> vptr;   // [nv+1] element incidences per vertex
   sk::DevBuf<int32_t> vlist;  // [4 nt] (e << 2) | i, e ascending per vertex
+  // Sliced-ELL (SELL-C-sigma) layout of the simulator's matrix: rows sorted by
+  // block count (descending), cut into chunks of kSellC rows; chunk c has
+  // chunk_deg[c] slots per row and starts at slot chunk_off[c]. Slot
+  // q = chunk_off[c] + s*kSellC + i holds block s of the chunk's row i; its
+  // column vertex is scol[q] and its value m sits at
+  // values[(q - i) * 9 + m * kSellC + i] (component-major within a slot group),
+  // so each component load of a chunk is one contiguous run. Padding slots
+  // have value 0.
+  sk::DevBuf<int32_t> sell_row;   // [nchunk * kSellC] original row, -1 for padding
+  sk::DevBuf<int32_t> chunk_deg;  // [nchunk]
+  sk::DevBuf<int32_t> chunk_off;  // [nchunk + 1]
+  sk::DevBuf<int32_t> scol;       // [nslot]
+  sk::DevBuf<int32_t> sell_pos;   // [nblk] slot of each CSR block
+  int64_t nchunk = 0, nslot = 0;
   sk::DevBuf<double> Kst;     // [nt][90] staged element stiffness blocks
   sk::DevBuf<double> fst;     // [nt][12] staged element forces
   sk::Reducer red;
@@ -89,8 +105,10 @@ int launch_elem_energy(sk_mesh* m, int material, const double* x, const double* 
                        hipStream_t st);
 int launch_elem_grad(sk_mesh* m, int material, const double* x, const double* F0, const double* mu,
                      const double* lam, const double* vol, hipStream_t st);
+// fp32: element Hessian, PSD projection and staged blocks in FP32 (staging
+// buffer reinterpreted), otherwise FP64.
 int launch_elem_hess(sk_mesh* m, int material, const double* x, const double* F0, const double* mu,
-                     const double* lam, const double* vol, int psd, hipStream_t st);
+                     const double* lam, const double* vol, int psd, bool fp32, hipStream_t st);
 // Gathers staged element forces into g; with mass != nullptr adds
 // m (x - xt) * inv_h2 - fext, zeroes fixed DOFs and reduces ||g||^2 into red.result[0].
 int launch_gather_grad(sk_mesh* m, double* g, const double* mass, const double* x, const double* xt,
@@ -98,8 +116,11 @@ int launch_gather_grad(sk_mesh* m, double* g, const double* mass, const double* 
 // Gathers staged element blocks into CSR values; with mass != nullptr adds
 // mass * inv_h2 on the diagonal and writes the fixed-DOF-masked inverse of each
 // diagonal 3x3 block into binv (9 per vertex).
-int launch_gather_hess(sk_mesh* m, double* values, const double* mass, double inv_h2, const uint8_t* fixed,
-                       double* binv, hipStream_t st);
+// sell = true stores the values in the SELL layout (the simulator's private
+// matrix); false uses the scalar CSR layout. fp32: values / binv are float
+// arrays summed in FP32, otherwise double.
+int launch_gather_hess(sk_mesh* m, void* values, const double* mass, double inv_h2, const uint8_t* fixed,
+                       void* binv, bool sell, bool fp32, hipStream_t st);
 // Copies n doubles from device to the mesh's pinned buffer and waits.
 int read_scalars(sk_mesh* m, const double* dev, int n, hipStream_t st);
 

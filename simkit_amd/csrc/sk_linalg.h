@@ -29,7 +29,10 @@ __device__ __forceinline__ int jacobi_eigh(R (&a)[Sym<N>::size], R (&v)[N * N], 
 #pragma unroll
     for (int j = 0; j < N; ++j) v[i * N + j] = (i == j) ? R(1) : R(0);
 
-  const R tol = sizeof(R) == 8 ? R(1e-32) : R(1e-14);
+  // Off-diagonal mass relative to the diagonal mass at which sweeps stop:
+  // (1e-16)^2 in FP64; in FP32 roundoff keeps off-diagonals near 1e-7 relative,
+  // so the FP32 threshold is (3e-7)^2.
+  const R tol = sizeof(R) == 8 ? R(1e-32) : R(1e-13);
   int sweep = 0;
   for (; sweep < max_sweeps; ++sweep) {
     R off = 0, dia = 0;
@@ -51,7 +54,10 @@ __device__ __forceinline__ int jacobi_eigh(R (&a)[Sym<N>::size], R (&v)[N * N], 
         const R rt = sqrt(d * d + R(4) * apq * apq);
         R t = R(2) * apq / (fabs(d) + rt);
         t = (d < R(0)) ? -t : t;
-        t = (apq == R(0)) ? R(0) : t;
+        // No rotation when a_pq is zero or so small that d^2 + 4 a_pq^2
+        // underflows (rt == 0): a_pq is then negligible, and rotating would
+        // divide by zero.
+        t = (apq == R(0) || !(rt > R(0))) ? R(0) : t;
         const R c = R(1) / sqrt(R(1) + t * t);
         const R s = t * c;
         const R h = t * apq;

@@ -153,28 +153,66 @@ __global__ __launch_bounds__(kBlock) void elem_grad_kernel(ElemArgs a) {
       out[3 * k + i] = vol * (P[3 * i] * D[k] + P[3 * i + 1] * D[4 + k] + P[3 * i + 2] * D[8 + k]);
 }
 
-template <class M, int PSD>
+// Element Hessian in precision R (double, or float for the mixed-precision
+// path). F is formed in FP64 from the FP64 positions and rounded to R; in FP32
+// the ARAP twists (the polar SVD) of elements with det F < 1e-3 are still
+// computed in FP64, where the FP32 SVD of a near-singular F loses the
+// rotation.
+template <class M, typename R>
+struct ElemHessian {
+  static __device__ __forceinline__ void eval(const double (&F)[9], double mu, double lam, R (&H)[45]) {
+    R Fr[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Fr[k] = (R)F[k];
+    M::hessian(Fr, (R)mu, (R)lam, H);
+  }
+};
+
+template <>
+struct ElemHessian<ARAP<float>, float> {
+  static __device__ __forceinline__ void eval(const double (&F)[9], double mu, double lam, float (&H)[45]) {
+    const double det = F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) +
+                       F[2] * (F[3] * F[7] - F[4] * F[6]);
+    if (det < 1e-3) {
+      double Hd[45];
+      ARAP<double>::hessian(F, mu, lam, Hd);
+#pragma unroll
+      for (int k = 0; k < 45; ++k) H[k] = (float)Hd[k];
+    } else {
+      float Fr[9];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) Fr[k] = (float)F[k];
+      ARAP<float>::hessian(Fr, (float)mu, (float)lam, H);
+    }
+  }
+};
+
+template <class M, int PSD, typename R>
 __global__ __launch_bounds__(kBlock) void elem_hess_kernel(ElemArgs a) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= a.m.nt) return;
   int32_t v[4];
-  double D[12], xe[12], F[9], H[45];
-  load_elem(a.m, e, v, D);
+  double Dd[12], xe[12], F[9];
+  load_elem(a.m, e, v, Dd);
   gather_x(a.x, v, xe);
-  make_F(xe, D, F);
+  make_F(xe, Dd, F);
   add_F0(a.F0, e, F);
-  M::hessian(F, a.mu[e], a.lam ? a.lam[e] : 0.0, H);
-  const double vol = a.vol[e];
+  R H[45];
+  ElemHessian<M, R>::eval(F, a.mu[e], a.lam ? a.lam[e] : 0.0, H);
+  const R vol = (R)a.vol[e];
 #pragma unroll
   for (int k = 0; k < 45; ++k) H[k] *= vol;
-  if (PSD != SK_PSD_NONE) psd_project_sym<9, double>(H, PSD, 1e-6);
+  if (PSD != SK_PSD_NONE) psd_project_sym<9, R>(H, PSD, R(1e-6));
+  R D[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) D[k] = (R)Dd[k];
   // K_e[3i+p][3j+q] = sum_{c,c'} D[c][i] H[3p+c][3q+c'] D[c'][j], upper blocks i <= j.
-  double* out = a.Kst + (int64_t)kKst * e;
+  R* out = reinterpret_cast<R*>(a.Kst) + (int64_t)kKst * e;
 #pragma unroll
   for (int p = 0; p < 3; ++p)
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-      double G[3][4];
+      R G[3][4];
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -246,31 +284,38 @@ struct GatherHArgs {
   const int32_t* brow;
   const int32_t* cptr;
   const int32_t* clist;
-  const double* Kst;
-  double* values;
+  const void* Kst;
+  void* values;
   const double* mass;
   double inv_h2;
   const uint8_t* fixed;
-  double* binv;
+  void* binv;
+  const int32_t* sell_pos;  // non-null: SELL layout
 };
 
+// One lane per block: sums the staged contributions in list order (precision
+// R), adds the lumped mass / h^2 on diagonal blocks, writes the block in CSR or
+// BSR order and, for diagonal blocks, the inverse of the fixed-DOF-masked block
+// (inverse of diag(f) A diag(f) + diag(1 - f), masked to the free DOFs).
+template <typename R>
 __global__ __launch_bounds__(kBlock) void gather_hess_kernel(GatherHArgs a) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= a.nblk) return;
-  double acc[9];
+  const R* __restrict__ Kst = reinterpret_cast<const R*>(a.Kst);
+  R acc[9];
 #pragma unroll
-  for (int k = 0; k < 9; ++k) acc[k] = 0.0;
+  for (int k = 0; k < 9; ++k) acc[k] = R(0);
   const int32_t k0 = a.cptr[b], k1 = a.cptr[b + 1];
   for (int32_t k = k0; k < k1; ++k) {
     const int32_t pk = a.clist[k];
     const int i = (pk >> 2) & 3, j = pk & 3;
     const int64_t e = pk >> 4;
     if (i <= j) {
-      const double* s = a.Kst + kKst * e + 9 * ublock(i, j);
+      const R* s = Kst + kKst * e + 9 * ublock(i, j);
 #pragma unroll
       for (int m = 0; m < 9; ++m) acc[m] += s[m];
     } else {
-      const double* s = a.Kst + kKst * e + 9 * ublock(j, i);
+      const R* s = Kst + kKst * e + 9 * ublock(j, i);
 #pragma unroll
       for (int p = 0; p < 3; ++p)
 #pragma unroll
@@ -281,31 +326,40 @@ __global__ __launch_bounds__(kBlock) void gather_hess_kernel(GatherHArgs a) {
   const int32_t b0 = a.bptr[v], deg = a.bptr[v + 1] - b0, r = (int32_t)(b - b0);
   if (v == w && a.mass) {
 #pragma unroll
-    for (int p = 0; p < 3; ++p) acc[4 * p] += a.mass[3 * (int64_t)v + p] * a.inv_h2;
+    for (int p = 0; p < 3; ++p) acc[4 * p] += (R)(a.mass[3 * (int64_t)v + p] * a.inv_h2);
   }
-  double* out = a.values + 9 * (int64_t)b0 + 3 * r;
+  R* values = reinterpret_cast<R*>(a.values);
+  if (a.sell_pos) {
+    const int64_t q = a.sell_pos[b];
+    const int64_t i = q % kSellC;
+    R* out = values + (q - i) * 9 + i;
 #pragma unroll
-  for (int p = 0; p < 3; ++p)
+    for (int m = 0; m < 9; ++m) out[m * kSellC] = acc[m];
+  } else {
+    R* out = values + 9 * (int64_t)b0 + 3 * r;
 #pragma unroll
-    for (int q = 0; q < 3; ++q) out[3 * p * deg + q] = acc[3 * p + q];
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) out[3 * p * deg + q] = acc[3 * p + q];
+  }
   if (v == w && a.binv) {
-    // Inverse of diag(f) A diag(f) + diag(1 - f), then masked to the free DOFs.
-    double f[3];
+    R f[3];
 #pragma unroll
-    for (int p = 0; p < 3; ++p) f[p] = (a.fixed && a.fixed[3 * (int64_t)v + p]) ? 0.0 : 1.0;
-    double A[9];
+    for (int p = 0; p < 3; ++p) f[p] = (a.fixed && a.fixed[3 * (int64_t)v + p]) ? R(0) : R(1);
+    R A[9];
 #pragma unroll
     for (int p = 0; p < 3; ++p)
 #pragma unroll
-      for (int q = 0; q < 3; ++q) A[3 * p + q] = f[p] * f[q] * acc[3 * p + q] + (p == q ? 1.0 - f[p] : 0.0);
-    double c[9];
+      for (int q = 0; q < 3; ++q) A[3 * p + q] = f[p] * f[q] * acc[3 * p + q] + (p == q ? R(1) - f[p] : R(0));
+    R c[9];
     cofactor3(A, c);
-    const double det = A[0] * c[0] + A[1] * c[1] + A[2] * c[2];
-    const double inv = 1.0 / det;
+    const R det = A[0] * c[0] + A[1] * c[1] + A[2] * c[2];
+    const R inv = R(1) / det;
+    R* binv = reinterpret_cast<R*>(a.binv);
 #pragma unroll
     for (int p = 0; p < 3; ++p)
 #pragma unroll
-      for (int q = 0; q < 3; ++q) a.binv[9 * (int64_t)v + 3 * p + q] = f[p] * f[q] * c[3 * q + p] * inv;
+      for (int q = 0; q < 3; ++q) binv[9 * (int64_t)v + 3 * p + q] = f[p] * f[q] * c[3 * q + p] * inv;
   }
 }
 
@@ -391,21 +445,27 @@ int launch_elem_grad(sk_mesh* m, int material, const double* x, const double* F0
   return SK_OK;
 }
 
-template <class M>
+template <class M, typename R>
 static void hess_dispatch(int psd, int g, hipStream_t st, const ElemArgs& a) {
-  if (psd == SK_PSD_PROJ) elem_hess_kernel<M, SK_PSD_PROJ><<<g, kBlock, 0, st>>>(a);
-  else if (psd == SK_PSD_ABS) elem_hess_kernel<M, SK_PSD_ABS><<<g, kBlock, 0, st>>>(a);
-  else elem_hess_kernel<M, SK_PSD_NONE><<<g, kBlock, 0, st>>>(a);
+  if (psd == SK_PSD_PROJ) elem_hess_kernel<M, SK_PSD_PROJ, R><<<g, kBlock, 0, st>>>(a);
+  else if (psd == SK_PSD_ABS) elem_hess_kernel<M, SK_PSD_ABS, R><<<g, kBlock, 0, st>>>(a);
+  else elem_hess_kernel<M, SK_PSD_NONE, R><<<g, kBlock, 0, st>>>(a);
 }
 
 int launch_elem_hess(sk_mesh* m, int material, const double* x, const double* F0, const double* mu,
-                     const double* lam, const double* vol, int psd, hipStream_t st) {
+                     const double* lam, const double* vol, int psd, bool fp32, hipStream_t st) {
   SK_REQUIRE(psd == SK_PSD_NONE || psd == SK_PSD_PROJ || psd == SK_PSD_ABS, "invalid psd method");
   ElemArgs a{view(m), x, F0, mu, lam, vol, m->fst.p, m->Kst.p, psd};
   const int g = grid_for(m->nt);
-  if (material == SK_SNH) hess_dispatch<SNH<double>>(psd, g, st, a);
-  else if (material == SK_ARAP) hess_dispatch<ARAP<double>>(psd, g, st, a);
-  else SK_REQUIRE(false, "unknown material");
+  if (material == SK_SNH) {
+    if (fp32) hess_dispatch<SNH<float>, float>(psd, g, st, a);
+    else hess_dispatch<SNH<double>, double>(psd, g, st, a);
+  } else if (material == SK_ARAP) {
+    if (fp32) hess_dispatch<ARAP<float>, float>(psd, g, st, a);
+    else hess_dispatch<ARAP<double>, double>(psd, g, st, a);
+  } else {
+    SK_REQUIRE(false, "unknown material");
+  }
   SK_HIP_CHECK(hipGetLastError());
   return SK_OK;
 }
@@ -421,11 +481,12 @@ int launch_gather_grad(sk_mesh* m, double* g, const double* mass, const double* 
   return SK_OK;
 }
 
-int launch_gather_hess(sk_mesh* m, double* values, const double* mass, double inv_h2, const uint8_t* fixed,
-                       double* binv, hipStream_t st) {
+int launch_gather_hess(sk_mesh* m, void* values, const double* mass, double inv_h2, const uint8_t* fixed,
+                       void* binv, bool sell, bool fp32, hipStream_t st) {
   GatherHArgs a{m->nblk, m->bptr.p, m->bcol.p, m->brow.p, m->cptr.p, m->clist.p, m->Kst.p,
-                values, mass, inv_h2, fixed, binv};
-  gather_hess_kernel<<<grid_for(m->nblk), kBlock, 0, st>>>(a);
+                values, mass, inv_h2, fixed, binv, sell ? m->sell_pos.p : nullptr};
+  if (fp32) gather_hess_kernel<float><<<grid_for(m->nblk), kBlock, 0, st>>>(a);
+  else gather_hess_kernel<double><<<grid_for(m->nblk), kBlock, 0, st>>>(a);
   SK_HIP_CHECK(hipGetLastError());
   return SK_OK;
 }
@@ -494,6 +555,41 @@ static int build_mesh(const int32_t* T, int64_t nv, int64_t nt, sk_mesh* m) {
     for (int64_t e = 0; e < nt; ++e)
       for (int ij = 0; ij < 16; ++ij) clist[fill[emap[ij * nt + e]]++] = (int32_t)((e << 4) | ij);
   }
+  // SELL-C-sigma layout: rows sorted by block count (descending, stable, so
+  // rows of equal degree keep their spatial order), chunks of kSellC rows.
+  std::vector<int32_t> order(nv);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+    return bptr[a + 1] - bptr[a] > bptr[b + 1] - bptr[b];
+  });
+  const int64_t nchunk = (nv + kSellC - 1) / kSellC;
+  std::vector<int32_t> sell_row(nchunk * kSellC, -1), chunk_deg(nchunk), chunk_off(nchunk + 1, 0);
+  for (int64_t i = 0; i < nv; ++i) sell_row[i] = order[i];
+  for (int64_t c = 0; c < nchunk; ++c) {
+    int32_t d = 0;
+    for (int i = 0; i < kSellC; ++i) {
+      const int32_t r = sell_row[c * kSellC + i];
+      if (r >= 0) d = std::max(d, bptr[r + 1] - bptr[r]);
+    }
+    chunk_deg[c] = d;
+    SK_REQUIRE((int64_t)chunk_off[c] + (int64_t)kSellC * d < (int64_t(1) << 31) / 9, "sk_mesh_create: SELL too large");
+    chunk_off[c + 1] = chunk_off[c] + kSellC * d;
+  }
+  const int64_t nslot = chunk_off[nchunk];
+  std::vector<int32_t> scol(nslot, 0), sell_pos(nblk);
+  for (int64_t c = 0; c < nchunk; ++c)
+    for (int i = 0; i < kSellC; ++i) {
+      const int32_t r = sell_row[c * kSellC + i];
+      for (int32_t s = 0; s < chunk_deg[c]; ++s) {
+        const int32_t q = chunk_off[c] + s * kSellC + i;
+        if (r >= 0 && s < bptr[r + 1] - bptr[r]) {
+          scol[q] = bcol[bptr[r] + s];
+          sell_pos[bptr[r] + s] = q;
+        } else {
+          scol[q] = r >= 0 ? r : 0;  // padding slot: value 0, any valid column
+        }
+      }
+    }
   std::vector<int32_t> Tsoa(4 * nt);
   for (int64_t e = 0; e < nt; ++e)
     for (int k = 0; k < 4; ++k) Tsoa[k * nt + e] = T[4 * e + k];
@@ -510,6 +606,13 @@ static int build_mesh(const int32_t* T, int64_t nv, int64_t nt, sk_mesh* m) {
   SK_TRY(m->clist.upload(clist));
   SK_TRY(m->vptr.upload(vptr));
   SK_TRY(m->vlist.upload(vlist));
+  SK_TRY(m->sell_row.upload(sell_row));
+  SK_TRY(m->chunk_deg.upload(chunk_deg));
+  SK_TRY(m->chunk_off.upload(chunk_off));
+  SK_TRY(m->scol.upload(scol));
+  SK_TRY(m->sell_pos.upload(sell_pos));
+  m->nchunk = nchunk;
+  m->nslot = nslot;
   SK_TRY(m->D.alloc(12 * (size_t)nt));
   SK_TRY(m->Kst.alloc((size_t)kKst * nt));
   SK_TRY(m->fst.alloc((size_t)kFst * nt));
@@ -611,8 +714,8 @@ int sk_mesh_hessian(sk_mesh* m, int material, const double* x, const double* F0,
   SK_REQUIRE(m && x && mu && vol && values, "sk_mesh_hessian: null argument");
   SK_REQUIRE(material == SK_ARAP || lam, "sk_mesh_hessian: stable neo-Hookean needs lam");
   hipStream_t st = as_stream(s);
-  SK_TRY(launch_elem_hess(m, material, x, F0, mu, lam, vol, psd, st));
-  SK_TRY(launch_gather_hess(m, values, nullptr, 0.0, nullptr, nullptr, st));
+  SK_TRY(launch_elem_hess(m, material, x, F0, mu, lam, vol, psd, false, st));
+  SK_TRY(launch_gather_hess(m, values, nullptr, 0.0, nullptr, nullptr, false, false, st));
   return SK_OK;
 }
 

@@ -102,24 +102,34 @@ __device__ __forceinline__ bool update_prologue(PcgState* st, const double* spar
 }
 
 // ---------------------------------------------------------------------------
-// FEM operator: sorted block-CSR pattern read from the scalar CSR values.
-// Block row v occupies values[9*bptr[v] .. 9*bptr[v+1]) as three scalar rows
-// of 3*deg entries each; block r of the row holds columns 3*bcol[bptr[v]+r]+q.
+// FEM operator on the simulator's BSR values (block b's 3x3 at
+// vals[9 b .. 9 b + 9)), templated on the precision T of values and vectors
+// (double: plain path; float: inner solve of the mixed-precision path). Dot
+// products are always accumulated in FP64.
 // ---------------------------------------------------------------------------
+// SELL-C-sigma description of the matrix (see sk_mesh).
+struct SellView {
+  int64_t nchunk;
+  const int32_t* sell_row;
+  const int32_t* chunk_deg;
+  const int32_t* chunk_off;
+  const int32_t* scol;
+};
+
+template <typename T>
 struct BlockPcgArgs {
   int64_t nv;
-  const int32_t* bptr;
-  const int32_t* bcol;
-  const double* vals;
-  const double* binv;
+  SellView sv;
+  const T* vals;
+  const T* binv;
   const uint8_t* fixed;
-  const double* b;
-  double* x;
-  double* r;
-  double* z;
-  const double* pin;
-  double* pout;
-  double* Ap;
+  const T* b;
+  T* x;
+  T* r;
+  T* z;
+  const T* pin;
+  T* pout;
+  T* Ap;
   PcgState* st;
   double rtol2;
   double* upart;  // [2][gu] partials of r.z and r.r
@@ -129,22 +139,23 @@ struct BlockPcgArgs {
   int k;          // iteration index of this launch
 };
 
-__device__ __forceinline__ void apply_binv(const double* __restrict__ binv, int64_t v, const double (&r)[3],
-                                           double (&z)[3]) {
-  const double* B = binv + 9 * v;
+template <typename T>
+__device__ __forceinline__ void apply_binv(const T* __restrict__ binv, int64_t v, const T (&r)[3], T (&z)[3]) {
+  const T* B = binv + 9 * v;
 #pragma unroll
   for (int p = 0; p < 3; ++p) z[p] = B[3 * p] * r[0] + B[3 * p + 1] * r[1] + B[3 * p + 2] * r[2];
 }
 
-__global__ __launch_bounds__(kBlock) void bpcg_init_kernel(BlockPcgArgs a) {
+template <typename T>
+__global__ __launch_bounds__(kBlock) void bpcg_init_kernel(BlockPcgArgs<T> a) {
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double rz = 0.0, rr = 0.0;
   if (v < a.nv) {
-    double r[3], z[3];
+    T r[3], z[3];
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
       const int64_t k = 3 * v + p;
-      r[p] = (a.fixed && a.fixed[k]) ? 0.0 : a.b[k];
+      r[p] = (a.fixed && a.fixed[k]) ? T(0) : a.b[k];
     }
     apply_binv(a.binv, v, r, z);
 #pragma unroll
@@ -152,10 +163,10 @@ __global__ __launch_bounds__(kBlock) void bpcg_init_kernel(BlockPcgArgs a) {
       const int64_t k = 3 * v + p;
       a.r[k] = r[p];
       a.z[k] = z[p];
-      a.x[k] = 0.0;
-      a.pout[k] = 0.0;
-      rz += r[p] * z[p];
-      rr += r[p] * r[p];
+      a.x[k] = T(0);
+      a.pout[k] = T(0);
+      rz += (double)r[p] * (double)z[p];
+      rr += (double)r[p] * (double)r[p];
     }
   }
   double vals[2] = {rz, rr};
@@ -166,58 +177,86 @@ __global__ __launch_bounds__(kBlock) void bpcg_init_kernel(BlockPcgArgs a) {
   }
 }
 
-__global__ __launch_bounds__(kBlock) void bpcg_spmv_kernel(BlockPcgArgs a) {
+// SELL chunk of this wave: one chunk of kSellC rows per wave, kSellLanes
+// lanes per row (lane = j * kSellC + i: row i of the chunk, slots s = j,
+// j + kSellLanes, ...). Consecutive chunks go to the same XCD.
+__device__ __forceinline__ int64_t sell_chunk(int wid) {
+  constexpr int kWaves = kBlock / kWave;
+  return (int64_t)xcd_block(blockIdx.x, gridDim.x) * kWaves + wid;
+}
+
+// y[p] = sum over the row's slots of A_s[p][:] . xv(col_s), where xv(w) gives
+// the 3-vector multiplied at column vertex w; summed over the row's
+// kSellLanes lanes (fixed xor order) and returned to every lane of the row.
+template <typename T, typename V, typename XV>
+__device__ __forceinline__ void sell_row_product(const SellView& sv, const T* __restrict__ vals, int64_t c, int i,
+                                                 int j, XV xv, V (&y)[3]) {
+  const int32_t deg = sv.chunk_deg[c];
+  const int64_t off = sv.chunk_off[c];
+  for (int32_t s = j; s < deg; s += kSellLanes) {
+    const int64_t g = off + (int64_t)s * kSellC;  // slot group of the chunk's rows
+    const int64_t w = 3 * (int64_t)sv.scol[g + i];
+    V pv[3];
+    xv(w, pv);
+    const T* __restrict__ A = vals + g * 9 + i;
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+      y[p] += (V)A[(3 * p) * kSellC] * pv[0] + (V)A[(3 * p + 1) * kSellC] * pv[1] +
+              (V)A[(3 * p + 2) * kSellC] * pv[2];
+  }
+#pragma unroll
+  for (int off2 = kSellC; off2 < kWave; off2 <<= 1) {
+#pragma unroll
+    for (int p = 0; p < 3; ++p) y[p] += __shfl_xor(y[p], off2, kWave);
+  }
+}
+
+// SpMV of PCG iteration k on the SELL matrix: forms p = z + beta p_prev on
+// the fly for every column touched, writes the new p and A p of its rows and
+// a partial of p.Ap.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void bpcg_spmv_kernel(BlockPcgArgs<T> a) {
   const PcgCtl c = spmv_prologue(a.st, a.upart, a.gu, a.k, a.rtol2);
   if (!c.run) return;
-  const double beta = c.beta;
-  constexpr int rows_per_pass = kBlock / kRowLanes;
-  const int lb = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t chunk = (a.nv + gridDim.x - 1) / gridDim.x;
-  const int64_t v_begin = lb * chunk;
-  const int64_t v_end = v_begin + chunk < a.nv ? v_begin + chunk : a.nv;
-  const int lane = threadIdx.x % kRowLanes;
+  const T beta = (T)c.beta;
+  const int lane = threadIdx.x % kWave;
+  const int64_t ch = sell_chunk(threadIdx.x / kWave);
   double dot = 0.0;
-  for (int64_t v0 = v_begin; v0 < v_end; v0 += rows_per_pass) {
-    const int64_t v = v0 + threadIdx.x / kRowLanes;
-    double acc[3] = {0.0, 0.0, 0.0};
-    if (v < v_end) {
-      const int32_t b0 = a.bptr[v];
-      const int32_t len = 3 * (a.bptr[v + 1] - b0);
-      const double* __restrict__ row = a.vals + 9 * (int64_t)b0;
-      for (int32_t m = lane; m < len; m += kRowLanes) {
-        const int32_t rb = m / 3;
-        const int64_t col = 3 * (int64_t)a.bcol[b0 + rb] + (m - 3 * rb);
-        const double pv = a.z[col] + beta * a.pin[col];
-        acc[0] += row[m] * pv;
-        acc[1] += row[len + m] * pv;
-        acc[2] += row[2 * len + m] * pv;
+  if (ch < a.sv.nchunk) {
+    const int i = lane % kSellC, j = lane / kSellC;
+    T y[3] = {T(0), T(0), T(0)};
+    const T* __restrict__ z = a.z;
+    const T* __restrict__ pin = a.pin;
+    sell_row_product<T, T>(a.sv, a.vals, ch, i, j, [&](int64_t w, T (&pv)[3]) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) pv[q] = z[w + q] + beta * pin[w + q];
+    }, y);
+    const int32_t row = a.sv.sell_row[ch * kSellC + i];
+    if (row >= 0 && j == 0) {
+#pragma unroll
+      for (int p = 0; p < 3; ++p) {
+        const int64_t k = 3 * (int64_t)row + p;
+        const T pn = z[k] + beta * pin[k];
+        const T ap = (a.fixed && a.fixed[k]) ? T(0) : y[p];
+        a.pout[k] = pn;
+        a.Ap[k] = ap;
+        dot += (double)pn * (double)ap;
       }
-    }
-#pragma unroll
-    for (int off = kRowLanes / 2; off > 0; off >>= 1) {
-#pragma unroll
-      for (int p = 0; p < 3; ++p) acc[p] += __shfl_xor(acc[p], off, kRowLanes);
-    }
-    if (v < v_end && lane < 3) {
-      const int64_t k = 3 * v + lane;
-      const double pn = a.z[k] + beta * a.pin[k];
-      const double ap = (a.fixed && a.fixed[k]) ? 0.0 : (lane == 0 ? acc[0] : (lane == 1 ? acc[1] : acc[2]));
-      a.pout[k] = pn;
-      a.Ap[k] = ap;
-      dot += pn * ap;
     }
   }
   double vals[1] = {dot};
   store_block_partials<1>(vals, a.spart);
 }
 
-__global__ __launch_bounds__(kBlock) void bpcg_update_kernel(BlockPcgArgs a) {
-  double alpha;
-  if (!update_prologue(a.st, a.spart, a.gs, a.k, alpha)) return;
+template <typename T>
+__global__ __launch_bounds__(kBlock) void bpcg_update_kernel(BlockPcgArgs<T> a) {
+  double alpha_d;
+  if (!update_prologue(a.st, a.spart, a.gs, a.k, alpha_d)) return;
+  const T alpha = (T)alpha_d;
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double rz = 0.0, rr = 0.0;
   if (v < a.nv) {
-    double r[3], z[3];
+    T r[3], z[3];
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
       const int64_t k = 3 * v + p;
@@ -230,12 +269,64 @@ __global__ __launch_bounds__(kBlock) void bpcg_update_kernel(BlockPcgArgs a) {
       const int64_t k = 3 * v + p;
       a.r[k] = r[p];
       a.z[k] = z[p];
-      rz += r[p] * z[p];
-      rr += r[p] * r[p];
+      rz += (double)r[p] * (double)z[p];
+      rr += (double)r[p] * (double)r[p];
     }
   }
   double vals[2] = {rz, rr};
   store_block_partials<2>(vals, a.upart);
+}
+
+// ---------------------------------------------------------------------------
+// Iterative refinement of the mixed-precision path: FP64 residual of the
+// FP32-stored matrix, r = b - A x (masked to the free DOFs), its FP32 copy for
+// the next inner solve, and per-block partials of r.r.
+// ---------------------------------------------------------------------------
+struct ResidualArgs {
+  int64_t nv;
+  SellView sv;
+  const float* vals;
+  const uint8_t* fixed;
+  const double* b;
+  const double* x;  // nullptr: x = 0
+  double* r;
+  float* r32;
+  double* partials;
+};
+
+__global__ __launch_bounds__(kBlock) void ir_residual_kernel(ResidualArgs a) {
+  const int lane = threadIdx.x % kWave;
+  const int64_t ch = sell_chunk(threadIdx.x / kWave);
+  double rr = 0.0;
+  if (ch < a.sv.nchunk) {
+    const int i = lane % kSellC, j = lane / kSellC;
+    double y[3] = {0.0, 0.0, 0.0};
+    if (a.x) {
+      const double* __restrict__ x = a.x;
+      sell_row_product<float, double>(a.sv, a.vals, ch, i, j, [&](int64_t w, double (&pv)[3]) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) pv[q] = x[w + q];
+      }, y);
+    }
+    const int32_t row = a.sv.sell_row[ch * kSellC + i];
+    if (row >= 0 && j == 0) {
+#pragma unroll
+      for (int p = 0; p < 3; ++p) {
+        const int64_t k = 3 * (int64_t)row + p;
+        const double rk = (a.fixed && a.fixed[k]) ? 0.0 : a.b[k] - y[p];
+        a.r[k] = rk;
+        a.r32[k] = (float)rk;
+        rr += rk * rk;
+      }
+    }
+  }
+  double vals[1] = {rr};
+  store_block_partials<1>(vals, a.partials);
+}
+
+__global__ void ir_accumulate_kernel(int64_t n, const float* d, double* x, int first) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) x[k] = first ? (double)d[k] : x[k] + (double)d[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -372,13 +463,17 @@ using namespace sk;
 struct sk_sim {
   sk_mesh* mesh = nullptr;
   sk_sim_params prm{};
+  int spmv_blocks = 0;
   DevBuf<double> mu, lam, vol, mass, fext;
   DevBuf<uint8_t> fixed;
   DevBuf<double> x, xt, g, dx, r, z, p0, p1, Ap, vals, binv;
+  // mixed precision: FP32 matrix, preconditioner and inner-PCG vectors
+  DevBuf<float> vals32, binv32, x32, r32, z32, p0_32, p1_32, Ap32;
   DevBuf<PcgState> st;
   PcgState* st_host = nullptr;
   hipEvent_t ev[6] = {};
   bool have_events = false;
+  bool mixed() const { return prm.precision == SK_PREC_MIXED; }
 };
 
 namespace {
@@ -398,41 +493,119 @@ int spmv_grid(int64_t rows) {
   return (int)(g < kSpmvBlocks ? (g < 1 ? 1 : g) : kSpmvBlocks);
 }
 
-BlockPcgArgs block_pcg_args(sk_sim* S, const double* b, double rtol2) {
-  sk_mesh* m = S->mesh;
-  const int gu = grid_for(m->nv);
-  const int gs = spmv_grid(m->nv);
-  double* part = m->red.partials.p;
-  return BlockPcgArgs{m->nv, m->bptr.p, m->bcol.p, S->vals.p, S->binv.p, S->fixed.p, b, S->dx.p, S->r.p, S->z.p,
-                      S->p1.p, S->p0.p, S->Ap.p, S->st.p, rtol2, part, gu, part + 2 * gu, gs, 0};
+// One SELL chunk per wave; the grid is rounded up to a multiple of 8 so the
+// XCD-aware block order is a permutation (surplus waves do no work).
+int sell_grid(int64_t nchunk) {
+  int64_t g = (nchunk + (kBlock / kWave) - 1) / (kBlock / kWave);
+  g = (g + 7) / 8 * 8;
+  return (int)(g < 8 ? 8 : g);
 }
 
-// Runs block-Jacobi PCG on the sim's assembled system with right-hand side b.
-int run_block_pcg(sk_sim* S, const double* b, hipStream_t st, int64_t* iters, int* done) {
+SellView sell_view(sk_mesh* m) {
+  return SellView{m->nchunk, m->sell_row.p, m->chunk_deg.p, m->chunk_off.p, m->scol.p};
+}
+
+// Buffers of one precision's PCG.
+template <typename T>
+struct PcgBufs {
+  const T* vals;
+  const T* binv;
+  T *x, *r, *z, *p0, *p1, *Ap;
+};
+
+PcgBufs<double> bufs64(sk_sim* S) {
+  return {S->vals.p, S->binv.p, S->dx.p, S->r.p, S->z.p, S->p0.p, S->p1.p, S->Ap.p};
+}
+PcgBufs<float> bufs32(sk_sim* S) {
+  return {S->vals32.p, S->binv32.p, S->x32.p, S->r32.p, S->z32.p, S->p0_32.p, S->p1_32.p, S->Ap32.p};
+}
+
+template <typename T>
+BlockPcgArgs<T> block_pcg_args(sk_sim* S, const PcgBufs<T>& B, const T* b, double rtol2) {
   sk_mesh* m = S->mesh;
-  BlockPcgArgs a = block_pcg_args(S, b, S->prm.pcg_rtol * S->prm.pcg_rtol);
+  const int gu = grid_for(m->nv);
+  const int gs = sell_grid(m->nchunk);
+  double* part = m->red.partials.p;
+  return BlockPcgArgs<T>{m->nv, sell_view(m), B.vals, B.binv, S->fixed.p, b, B.x, B.r, B.z, B.p1, B.p0, B.Ap,
+                         S->st.p, rtol2, part, gu, part + 2 * gu, gs, 0};
+}
+
+// Block-Jacobi PCG on the assembled system with right-hand side b (x0 = 0).
+template <typename T>
+int run_block_pcg(sk_sim* S, const PcgBufs<T>& B, const T* b, double rtol, int64_t max_iter, hipStream_t st,
+                  int64_t* iters, int* done) {
+  BlockPcgArgs<T> a = block_pcg_args<T>(S, B, b, rtol * rtol);
   SK_REQUIRE(2 * a.gu + a.gs <= 8 * Reducer::kMaxBlocks, "mesh too large for the PCG reductions");
-  a.pout = S->p0.p;  // init zeroes p0, the first "previous p"
-  bpcg_init_kernel<<<a.gu, kBlock, 0, st>>>(a);
+  a.pout = B.p0;  // init zeroes p0, the first "previous p"
+  bpcg_init_kernel<T><<<a.gu, kBlock, 0, st>>>(a);
   SK_HIP_CHECK(hipGetLastError());
   int64_t launched = 0;
   for (;;) {
     for (int k = 0; k < kPcgChunk; ++k, ++launched) {
       a.k = (int)launched;
-      a.pin = (launched & 1) ? S->p1.p : S->p0.p;
-      a.pout = (launched & 1) ? S->p0.p : S->p1.p;
-      bpcg_spmv_kernel<<<a.gs, kBlock, 0, st>>>(a);
-      bpcg_update_kernel<<<a.gu, kBlock, 0, st>>>(a);
+      a.pin = (launched & 1) ? B.p1 : B.p0;
+      a.pout = (launched & 1) ? B.p0 : B.p1;
+      bpcg_spmv_kernel<T><<<a.gs, kBlock, 0, st>>>(a);
+      bpcg_update_kernel<T><<<a.gu, kBlock, 0, st>>>(a);
     }
     SK_HIP_CHECK(hipGetLastError());
     SK_HIP_CHECK(hipMemcpyAsync(S->st_host, S->st.p, sizeof(PcgState), hipMemcpyDeviceToHost, st));
     SK_HIP_CHECK(hipStreamSynchronize(st));
-    if (S->st_host->done || S->st_host->iter >= S->prm.pcg_max_iter) break;
-    if (launched > (int64_t)S->prm.pcg_max_iter + kPcgChunk) break;  // device state not advancing
+    if (S->st_host->done || S->st_host->iter >= max_iter) break;
+    if (launched > max_iter + kPcgChunk) break;  // device state not advancing
   }
-  (void)m;
   *iters = S->st_host->iter;
   *done = S->st_host->done;
+  return SK_OK;
+}
+
+// Mixed-precision solve of A dx = b (b FP64 in S->g, dx FP64 in S->dx): FP64
+// iterative refinement r = b - A32 dx around FP32 block-Jacobi PCG solves of
+// A32 d = r to inner_rtol, until ||r|| <= pcg_rtol ||b||.
+int run_mixed_solve(sk_sim* S, hipStream_t st, int64_t* iters, int* refines, int* done) {
+  sk_mesh* m = S->mesh;
+  const int64_t n3 = 3 * m->nv;
+  ResidualArgs ra{m->nv, sell_view(m), S->vals32.p, S->fixed.p, S->g.p, nullptr, S->r.p, S->r32.p,
+                  m->red.partials.p + 4 * (size_t)Reducer::kMaxBlocks};
+  const int gr = sell_grid(m->nchunk);
+  double* rr_dev = m->red.result.p + 1;
+  ir_residual_kernel<<<gr, kBlock, 0, st>>>(ra);
+  finalize_sum_kernel<1><<<1, kBlock, 0, st>>>(ra.partials, gr, rr_dev);
+  SK_TRY(read_scalars(m, rr_dev, 1, st));
+  const double bb = m->pinned[0];
+  const double target = S->prm.pcg_rtol * S->prm.pcg_rtol * bb;
+  double rr = bb;
+  *iters = 0;
+  *refines = 0;
+  *done = 1;
+  const PcgBufs<float> B = bufs32(S);
+  for (int k = 0; k < S->prm.max_refine && rr > target && bb > 0.0; ++k) {
+    // Inner solve A32 d = r32 into x32. The right-hand side may alias the
+    // inner residual r32: the init launch reads each entry before the same
+    // thread overwrites it, and nothing reads the right-hand side afterwards.
+    int64_t it = 0;
+    int d = 0;
+    SK_TRY(run_block_pcg<float>(S, B, B.r, S->prm.inner_rtol, S->prm.pcg_max_iter - *iters, st, &it, &d));
+    *iters += it;
+    ++*refines;
+    if (d == 2 || d == 3) {
+      *done = d;
+      return SK_OK;
+    }
+    ir_accumulate_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, B.x, S->dx.p, k == 0 ? 1 : 0);
+    ra.x = S->dx.p;
+    ir_residual_kernel<<<gr, kBlock, 0, st>>>(ra);
+    finalize_sum_kernel<1><<<1, kBlock, 0, st>>>(ra.partials, gr, rr_dev);
+    SK_TRY(read_scalars(m, rr_dev, 1, st));
+    rr = m->pinned[0];
+    if (!std::isfinite(rr)) {
+      *done = 3;
+      return SK_OK;
+    }
+    if (*iters >= S->prm.pcg_max_iter) break;
+  }
+  if (bb == 0.0) SK_HIP_CHECK(hipMemsetAsync(S->dx.p, 0, n3 * sizeof(double), st));
+  *done = (rr <= target) ? 1 : 0;
   return SK_OK;
 }
 
@@ -516,11 +689,15 @@ int sk_sim_create(sk_mesh* mesh, const sk_sim_params* p, const double* mu, const
   SK_REQUIRE(p->ls_alpha > 0.0 && p->ls_alpha <= 0.5, "sk_sim_create: ls_alpha must be in (0, 0.5]");
   SK_REQUIRE(p->ls_beta > 0.0 && p->ls_beta < 1.0, "sk_sim_create: ls_beta must be in (0, 1)");
   SK_REQUIRE(p->psd == SK_PSD_NONE || p->psd == SK_PSD_PROJ || p->psd == SK_PSD_ABS, "sk_sim_create: bad psd");
+  SK_REQUIRE(p->precision == SK_PREC_FP64 || p->precision == SK_PREC_MIXED, "sk_sim_create: bad precision");
+  SK_REQUIRE(p->precision == SK_PREC_FP64 || (p->inner_rtol > 0.0 && p->inner_rtol < 1.0 && p->max_refine > 0),
+             "sk_sim_create: mixed precision needs 0 < inner_rtol < 1 and max_refine > 0");
+  SK_REQUIRE(p->spmv_blocks >= 0, "sk_sim_create: spmv_blocks must be >= 0");
   sk_sim* S = new sk_sim();
   S->mesh = mesh;
   S->prm = *p;
-  const size_t nt = mesh->nt, n3 = 3 * mesh->nv, nnz = 9 * mesh->nblk;
-  int s = SK_OK;
+  S->spmv_blocks = p->spmv_blocks > 0 ? p->spmv_blocks : kSpmvBlocks;
+  const size_t nt = mesh->nt, n3 = 3 * mesh->nv;
   auto run = [&]() -> int {
     SK_TRY(copy_dev(S->mu, mu, nt));
     if (lam) SK_TRY(copy_dev(S->lam, lam, nt));
@@ -531,17 +708,28 @@ int sk_sim_create(sk_mesh* mesh, const sk_sim_params* p, const double* mu, const
       SK_TRY(S->fixed.alloc(n3));
       SK_HIP_CHECK(hipMemcpy(S->fixed.p, fixed, n3, hipMemcpyDeviceToDevice));
     }
-    for (DevBuf<double>* b : {&S->x, &S->xt, &S->g, &S->dx, &S->r, &S->z, &S->p0, &S->p1, &S->Ap})
+    for (DevBuf<double>* b : {&S->x, &S->xt, &S->g, &S->dx, &S->r})
       SK_TRY(b->alloc(n3));
-    SK_TRY(S->vals.alloc(nnz));
-    SK_TRY(S->binv.alloc(9 * (size_t)mesh->nv));
+    if (S->mixed()) {
+      for (DevBuf<float>* b : {&S->x32, &S->r32, &S->z32, &S->p0_32, &S->p1_32, &S->Ap32})
+        SK_TRY(b->alloc(n3));
+      SK_TRY(S->vals32.alloc(9 * (size_t)mesh->nslot));
+      SK_HIP_CHECK(hipMemset(S->vals32.p, 0, 9 * (size_t)mesh->nslot * sizeof(float)));
+      SK_TRY(S->binv32.alloc(9 * (size_t)mesh->nv));
+    } else {
+      for (DevBuf<double>* b : {&S->z, &S->p0, &S->p1, &S->Ap})
+        SK_TRY(b->alloc(n3));
+      SK_TRY(S->vals.alloc(9 * (size_t)mesh->nslot));
+      SK_HIP_CHECK(hipMemset(S->vals.p, 0, 9 * (size_t)mesh->nslot * sizeof(double)));
+      SK_TRY(S->binv.alloc(9 * (size_t)mesh->nv));
+    }
     SK_TRY(S->st.alloc(1));
     SK_HIP_CHECK(hipHostMalloc((void**)&S->st_host, sizeof(PcgState), hipHostMallocDefault));
     for (auto& e : S->ev) SK_HIP_CHECK(hipEventCreate(&e));
     S->have_events = true;
     return SK_OK;
   };
-  s = run();
+  int s = run();
   if (s != SK_OK) {
     sk_sim_destroy(S);
     return s;
@@ -565,6 +753,7 @@ int sk_sim_step(sk_sim* S, const double* x_curr, const double* x_prev, const dou
   sk_mesh* m = S->mesh;
   hipStream_t st = as_stream(s);
   const sk_sim_params& P = S->prm;
+  const bool mixed = S->mixed();
   const int64_t n3 = 3 * m->nv;
   const double inv_h2 = 1.0 / (P.h * P.h);
   const double* lam = S->lam.p;
@@ -591,16 +780,21 @@ int sk_sim_step(sk_sim* S, const double* x_curr, const double* x_prev, const dou
     }
     if (inf.newton_iters >= P.newton_max_iter) break;
     if (P.timing) SK_HIP_CHECK(hipEventRecord(S->ev[0], st));
-    SK_TRY(launch_elem_hess(m, P.material, S->x.p, nullptr, S->mu.p, lam, S->vol.p, P.psd, st));
-    SK_TRY(launch_gather_hess(m, S->vals.p, S->mass.p, inv_h2, fixed, S->binv.p, st));
+    SK_TRY(launch_elem_hess(m, P.material, S->x.p, nullptr, S->mu.p, lam, S->vol.p, P.psd, mixed, st));
+    if (mixed)
+      SK_TRY(launch_gather_hess(m, S->vals32.p, S->mass.p, inv_h2, fixed, S->binv32.p, true, true, st));
+    else
+      SK_TRY(launch_gather_hess(m, S->vals.p, S->mass.p, inv_h2, fixed, S->binv.p, true, false, st));
     if (P.timing) SK_HIP_CHECK(hipEventRecord(S->ev[1], st));
     // Newton direction: H dx = -g on the free DOFs. The RHS is negated in place.
     axpy_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, -2.0, S->g.p, S->g.p);
     int64_t it = 0;
-    int done = 0;
-    SK_TRY(run_block_pcg(S, S->g.p, st, &it, &done));
+    int done = 0, refines = 0;
+    if (mixed) SK_TRY(run_mixed_solve(S, st, &it, &refines, &done));
+    else SK_TRY(run_block_pcg<double>(S, bufs64(S), S->g.p, P.pcg_rtol, P.pcg_max_iter, st, &it, &done));
     axpy_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, -2.0, S->g.p, S->g.p);
     inf.pcg_iters += it;
+    inf.refine_steps += refines;
     if (done == 2) inf.status = SK_ERR_PCG_BREAKDOWN;
     if (done == 3) inf.status = SK_ERR_NONFINITE;
     if (inf.status != SK_OK) break;
@@ -612,8 +806,8 @@ int sk_sim_step(sk_sim* S, const double* x_curr, const double* x_prev, const dou
     double t = 1.0;
     bool accepted = false;
     for (int ls = 0; ls < P.ls_max_iter; ++ls) {
-      SK_TRY(launch_elem_energy(m, P.material, S->x.p, nullptr, S->dx.p, t, S->mu.p, lam, S->vol.p, nullptr, S->mass.p,
-                                S->xt.p, fext, fixed, inv_h2, st));
+      SK_TRY(launch_elem_energy(m, P.material, S->x.p, nullptr, S->dx.p, t, S->mu.p, lam, S->vol.p, nullptr,
+                                S->mass.p, S->xt.p, fext, fixed, inv_h2, st));
       SK_TRY(read_scalars(m, m->red.result.p, 1, st));
       inf.ls_trials++;
       const double dE = m->pinned[0];
@@ -650,18 +844,28 @@ int sk_sim_step(sk_sim* S, const double* x_curr, const double* x_prev, const dou
   return SK_OK;
 }
 
-
 int sk_sim_spmv_reps(sk_sim* S, int reps, sk_stream_t s) {
   SK_REQUIRE(S && reps >= 0, "sk_sim_spmv_reps: invalid argument");
   hipStream_t st = as_stream(s);
-  // Per-step scratch vectors only (dx, r, z, p0, p1, Ap); RHS = the last
-  // gradient; rtol 0 so every launch runs iteration 0 in full.
-  BlockPcgArgs a = block_pcg_args(S, S->g.p, 0.0);
-  a.pout = S->p0.p;
-  bpcg_init_kernel<<<a.gu, kBlock, 0, st>>>(a);
-  a.pin = S->p0.p;
-  a.pout = S->p1.p;
-  for (int k = 0; k < reps; ++k) bpcg_spmv_kernel<<<a.gs, kBlock, 0, st>>>(a);
+  // Per-step scratch vectors only; rtol 0 so every launch runs iteration 0 in
+  // full. Right-hand side: the last gradient (FP64) or the last FP32 residual.
+  if (S->mixed()) {
+    const PcgBufs<float> B = bufs32(S);
+    BlockPcgArgs<float> a = block_pcg_args<float>(S, B, B.r, 0.0);
+    a.pout = B.p0;
+    bpcg_init_kernel<float><<<a.gu, kBlock, 0, st>>>(a);
+    a.pin = B.p0;
+    a.pout = B.p1;
+    for (int k = 0; k < reps; ++k) bpcg_spmv_kernel<float><<<a.gs, kBlock, 0, st>>>(a);
+  } else {
+    const PcgBufs<double> B = bufs64(S);
+    BlockPcgArgs<double> a = block_pcg_args<double>(S, B, S->g.p, 0.0);
+    a.pout = B.p0;
+    bpcg_init_kernel<double><<<a.gu, kBlock, 0, st>>>(a);
+    a.pin = B.p0;
+    a.pout = B.p1;
+    for (int k = 0; k < reps; ++k) bpcg_spmv_kernel<double><<<a.gs, kBlock, 0, st>>>(a);
+  }
   SK_HIP_CHECK(hipGetLastError());
   return SK_OK;
 }

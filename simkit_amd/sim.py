@@ -30,7 +30,11 @@ class BackwardEulerSim:
     def __init__(self, X, T, mu, lam=None, rho=1000.0, h=1.0 / 60.0, material="snh", fixed_vertices=(),
                  gravity=0.0, psd=True, newton_rtol=1e-6, newton_max_iter=100, pcg_rtol=1e-8,
                  pcg_max_iter=100000, ls_alpha=0.01, ls_beta=0.5, ls_max_iter=100, ls_threshold=1e-12,
-                 timing=False):
+                 timing=False, precision="fp64", inner_rtol=1e-4, max_refine=20, spmv_blocks=0):
+        """precision: "fp64" (every stage FP64) or "mixed" (FP32 element Hessian, PSD projection,
+        assembled values and inner PCG to inner_rtol, wrapped in FP64 iterative refinement to
+        pcg_rtol; gradient, line-search energies and the polar SVD of ARAP elements with
+        det F < 1e-3 stay FP64)."""
         X = np.asarray(X, dtype=np.float64)
         T = np.asarray(T)
         self.X = X
@@ -69,7 +73,12 @@ class BackwardEulerSim:
         p.ls_max_iter = ls_max_iter
         p.ls_threshold = ls_threshold
         p.timing = 1 if timing else 0
+        p.precision = {"fp64": _lib.SK_PREC_FP64, "mixed": _lib.SK_PREC_MIXED}[precision]
+        p.inner_rtol = inner_rtol
+        p.max_refine = max_refine
+        p.spmv_blocks = spmv_blocks
         self.params = p
+        self.precision = precision
         h_ = ctypes.c_void_p()
         _lib.call("sk_sim_create", self.mesh.handle, ctypes.byref(p), ptr(self.mu), ptr(self.lam), ptr(self.vol),
                   ptr(self.mass), ptr(self.f_ext), ptr(self.fixed), ctypes.byref(h_))
