@@ -42,12 +42,13 @@ def build_workload(cells):
     return X, T, bottom, top
 
 
-def spmv_bytes(nv, nblk):
-    """Algorithmic HBM bytes of one BSR SpMV launch: FP64 values (9 per block),
-    int32 block column and block row per block, row pointers, the z and previous-p vectors read, the new p
-    and A p written, and the 1-byte fixed-DOF mask."""
+def spmv_bytes(nv, nblk, value_bytes):
+    """Algorithmic HBM bytes of one SpMV launch of the PCG: the matrix values
+    (9 per 3x3 block), one int32 column index per block, the z and previous-p
+    vectors read, the new p and A p written (vector entries of the same width as
+    the values), and the 1-byte fixed-DOF mask."""
     n3 = 3 * nv
-    return 9 * nblk * 8 + 2 * nblk * 4 + (nv + 1) * 4 + 4 * n3 * 8 + n3
+    return 9 * nblk * value_bytes + 4 * nblk + 4 * n3 * value_bytes + n3
 
 
 def cpu_baseline(X, T, bottom, top, mu, lam, gpu_infos, steps, pcg_sample):
@@ -87,41 +88,18 @@ def cpu_baseline(X, T, bottom, top, mu, lam, gpu_infos, steps, pcg_sample):
     }
 
 
-def main():
-    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=60)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--cells", type=int, default=48)
-    ap.add_argument("--spmv-reps", type=int, default=200)
-    ap.add_argument("--cpu-pcg-sample", type=int, default=20)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
-    ap.add_argument("--inner-rtol", type=float, default=1e-4)
-    ap.add_argument("--spmv-blocks", type=int, default=0)
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "multi-GPU sharding is a design note, not an implementation", file=sys.stderr)
-        sys.exit(2)
-
+def run_path(args, X, T, bottom, top, mu, lam, precision):
+    """Warm up, then time the first --steps steps of one solver path; returns a result dict."""
     import torch
     from simkit_amd import _lib
     from simkit_amd.sim import BackwardEulerSim, top_face_sinusoid
-    from simkit_amd.ympr_to_lame import ympr_to_lame
-
-    assert torch.cuda.is_available(), "bench.py needs a GPU"
-    X, T, bottom, top = build_workload(args.cells)
-    mu, lam = ympr_to_lame(1e6, 0.45)
-    log(f"bench: {args.cells}^3 cube, {X.shape[0]} vertices, {T.shape[0]} tets; building mesh")
     t_setup = time.perf_counter()
     sim = BackwardEulerSim(X, T, mu, lam, rho=1000.0, h=1.0 / 60.0, material="snh",
                            fixed_vertices=np.concatenate([bottom, top]), pcg_rtol=1e-8, newton_rtol=1e-6,
-                           timing=True, precision=args.precision, inner_rtol=args.inner_rtol,
-                           spmv_blocks=args.spmv_blocks)
+                           timing=True, precision=precision, inner_rtol=args.inner_rtol)
     bc = top_face_sinusoid(X, top, 0.1, 1.0)
     torch.cuda.synchronize()
-    log(f"bench: setup {time.perf_counter() - t_setup:.1f}s; warmup {args.warmup} steps")
+    log(f"bench[{precision}]: setup {time.perf_counter() - t_setup:.1f}s; warmup {args.warmup} steps")
 
     def run(nsteps):
         x_prev = sim.X_dev.clone()
@@ -131,23 +109,17 @@ def main():
             x_next, info = sim.step(x_curr, x_prev, bc(n, (n + 1) * sim.h))
             infos.append(info)
             if info["status"] != 0:
-                raise RuntimeError(f"step {n}: status {info['status']} ({info})")
+                raise RuntimeError(f"{precision} step {n}: status {info['status']} ({info})")
             x_prev, x_curr = x_curr, x_next
         return x_curr, infos
 
     run(args.warmup)
     torch.cuda.synchronize()
-    log(f"bench: timing {args.steps} steps")
     t0 = time.perf_counter()
     x_final, infos = run(args.steps)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    steps_per_s = args.steps / elapsed
-    n_newton = sum(i["newton_iters"] for i in infos)
-    n_pcg = sum(i["pcg_iters"] for i in infos)
-    log(f"bench: {elapsed:.2f}s, {steps_per_s:.3f} steps/s, newton {n_newton}, pcg {n_pcg}")
-
-    # Dominant kernel: block-CSR SpMV, average launch duration over --spmv-reps launches.
+    # dominant kernel: the PCG SpMV, average over back-to-back launches on this stream
     stream = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(stream)
@@ -155,36 +127,93 @@ def main():
     e1.record(stream)
     torch.cuda.synchronize()
     spmv_ms = e0.elapsed_time(e1) / args.spmv_reps
-    traffic = spmv_bytes(sim.nv, sim.mesh.nblocks)
-    achieved = traffic / (spmv_ms * 1e-3) / 1e9
-    phases = {k: round(sum(i[k] for i in infos), 2) for k in ("ms_gradient", "ms_hessian", "ms_pcg",
-                                                               "ms_line_search")}
+    traffic = spmv_bytes(sim.nv, sim.mesh.nblocks, 8 if precision == "fp64" else 4)
+    res = {
+        "steps_per_s": args.steps / elapsed, "ms_per_step": 1e3 * elapsed / args.steps,
+        "newton_iters": sum(i["newton_iters"] for i in infos), "pcg_iters": sum(i["pcg_iters"] for i in infos),
+        "ls_trials": sum(i["ls_trials"] for i in infos), "refine_steps": sum(i["refine_steps"] for i in infos),
+        "phase_ms_total": {k: round(sum(i[k] for i in infos), 2)
+                           for k in ("ms_gradient", "ms_hessian", "ms_pcg", "ms_line_search")},
+        "spmv_ms": spmv_ms, "spmv_bytes": traffic, "infos": infos, "x_final": x_final.cpu().numpy(),
+    }
+    log(f"bench[{precision}]: {elapsed:.2f}s, {res['steps_per_s']:.3f} steps/s, newton {res['newton_iters']}, "
+        f"pcg {res['pcg_iters']}, spmv {spmv_ms * 1e3:.1f} us")
+    del sim
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cells", type=int, default=48)
+    ap.add_argument("--spmv-reps", type=int, default=200)
+    ap.add_argument("--cpu-pcg-sample", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--inner-rtol", type=float, default=1e-2)
+    ap.add_argument("--paths", default="fp64,mixed", help="comma list of fp64, mixed")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU sharding is a design note, not an implementation", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from simkit_amd.ympr_to_lame import ympr_to_lame
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    X, T, bottom, top = build_workload(args.cells)
+    mu, lam = ympr_to_lame(1e6, 0.45)
+    log(f"bench: {args.cells}^3 cube, {X.shape[0]} vertices, {T.shape[0]} tets")
+    paths = [p for p in args.paths.split(",") if p]
+    results = {p: run_path(args, X, T, bottom, top, mu, lam, p) for p in paths}
+    head = results.get("fp64") or results[paths[0]]
+    head_name = "fp64" if "fp64" in results else paths[0]
+    achieved = head["spmv_bytes"] / (head["spmv_ms"] * 1e-3) / 1e9
     out = {
         "metric": METRIC,
-        "value": round(steps_per_s, 4),
+        "value": round(head["steps_per_s"], 4),
         "unit": "steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+        "ms_per_step": round(head["ms_per_step"], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if head_name == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {
-            "workload": f"prescribed-motion cube {args.cells}^3 cells, stable neo-Hookean, plain FP64 path",
+            "workload": f"prescribed-motion cube {args.cells}^3 cells, stable neo-Hookean, "
+                        f"{'plain FP64' if head_name == 'fp64' else head_name} path",
             "vertices": int(X.shape[0]), "tets": int(T.shape[0]), "dof": int(3 * X.shape[0]),
-            "newton_iters": n_newton, "pcg_iters": n_pcg, "ls_trials": sum(i["ls_trials"] for i in infos),
-            "phase_ms_total": phases,
+            "newton_iters": head["newton_iters"], "pcg_iters": head["pcg_iters"], "ls_trials": head["ls_trials"],
+            "phase_ms_total": head["phase_ms_total"],
         },
-        "roofline": {"bound": "hbm", "kernel": "bpcg_spmv_kernel", "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "bytes_per_launch": int(traffic), "avg_launch_ms": round(spmv_ms, 5)},
+        "roofline": {"bound": "hbm", "kernel": "bpcg_spmv_kernel<double>" if head_name == "fp64"
+                     else "bpcg_spmv_kernel<float>", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_launch": int(head["spmv_bytes"]), "avg_launch_ms": round(head["spmv_ms"], 5)},
     }
+    if "mixed" in results and head_name == "fp64":
+        mx = results["mixed"]
+        dev = float(np.abs(mx["x_final"] - head["x_final"]).max() / np.abs(head["x_final"]).max())
+        ach = mx["spmv_bytes"] / (mx["spmv_ms"] * 1e-3) / 1e9
+        out["fp32_refined"] = {
+            "value": round(mx["steps_per_s"], 4), "ms_per_step": round(mx["ms_per_step"], 3),
+            "speedup_vs_fp64": round(mx["steps_per_s"] / head["steps_per_s"], 3),
+            "final_position_rel_dev_vs_fp64": dev, "inner_rtol": args.inner_rtol,
+            "newton_iters": mx["newton_iters"], "pcg_iters": mx["pcg_iters"], "refine_steps": mx["refine_steps"],
+            "phase_ms_total": mx["phase_ms_total"],
+            "spmv": {"kernel": "bpcg_spmv_kernel<float>", "avg_launch_ms": round(mx["spmv_ms"], 5),
+                     "bytes_per_launch": int(mx["spmv_bytes"]), "achieved_GBs": round(ach, 1),
+                     "frac": round(ach / HBM_PEAK_GBS, 4)},
+        }
     if not args.no_cpu_baseline:
         log("bench: CPU baseline sample")
-        out["cpu_baseline"] = cpu_baseline(X, T, bottom, top, mu, lam, infos, args.steps, args.cpu_pcg_sample)
+        out["cpu_baseline"] = cpu_baseline(X, T, bottom, top, mu, lam, head["infos"], args.steps,
+                                           args.cpu_pcg_sample)
     print(json.dumps(out), flush=True)
 
 

@@ -191,8 +191,10 @@ typedef struct sk_sim_params {
   double ls_threshold;
   int timing;            /* 1: record per-phase times in sk_step_info */
   int precision;         /* sk_precision */
-  double inner_rtol;     /* SK_PREC_MIXED: relative tolerance of each inner FP32 PCG solve */
-  int max_refine;        /* SK_PREC_MIXED: refinement steps per Newton solve at most */
+  double inner_rtol;     /* SK_PREC_MIXED: each time the FP32 PCG residual has dropped by this factor, the
+                            correction is accumulated in FP64 and the FP64 residual b - A x replaces the
+                            FP32 one (CG continues with its search direction) */
+  int max_refine;        /* SK_PREC_MIXED: residual replacements per Newton solve at most */
   int spmv_blocks;       /* SpMV launch width (workgroups); 0 = default */
 } sk_sim_params;
 
@@ -207,7 +209,7 @@ typedef struct sk_step_info {
   double ms_hessian;
   double ms_pcg;
   double ms_line_search;
-  int refine_steps;      /* SK_PREC_MIXED: refinement steps summed over the Newton iterations */
+  int refine_steps;      /* SK_PREC_MIXED: FP64 residual replacements summed over the Newton iterations */
 } sk_step_info;
 
 /* mu, lam, vol: per element; mass: per DOF (3nv, M = kron(M_lumped, I3));

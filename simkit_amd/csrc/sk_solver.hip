@@ -55,22 +55,24 @@ __device__ __forceinline__ bool block_reads_done(const PcgState* st) {
   return s_done != 0;
 }
 
-__device__ __forceinline__ PcgCtl spmv_prologue(PcgState* st, const double* upart, int gu, int k, double rtol2) {
+// restart: first launch after a residual replacement (the stopping threshold
+// restarts from the replaced residual; beta continues from r.z of k - 1).
+__device__ __forceinline__ PcgCtl spmv_prologue(PcgState* st, const double* upart, int gu, int k, double rtol2,
+                                                int restart = 0) {
   PcgCtl c{0.0, false};
   if (block_reads_done(st)) return c;
   double red[2];
   sum_partials<2>(upart, gu, gu, red);
   const double rz = red[0], rr = red[1];
-  const double tol2 = (k == 0) ? rtol2 * rr : st->tol2;
+  const bool fresh = (k == 0) || restart;
+  const double tol2 = fresh ? rtol2 * rr : st->tol2;
   const bool finite = isfinite(rr) && isfinite(rz);
   const bool conv = finite && !(rr > tol2);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->rz[k & 1] = rz;
     st->rr = rr;
-    if (k == 0) {
-      st->tol2 = tol2;
-      st->bb = rr;
-    }
+    if (fresh) st->tol2 = tol2;
+    if (k == 0) st->bb = rr;
     if (!finite) st->done = 3;
     else if (conv) st->done = 1;
   }
@@ -137,6 +139,7 @@ struct BlockPcgArgs {
   double* spart;  // [gs] partials of p.Ap
   int gs;
   int k;          // iteration index of this launch
+  int restart;    // 1 on the first launch after a residual replacement
 };
 
 template <typename T>
@@ -175,6 +178,31 @@ __global__ __launch_bounds__(kBlock) void bpcg_init_kernel(BlockPcgArgs<T> a) {
     a.st->done = 0;
     a.st->iter = 0;
   }
+}
+
+// Residual replacement: r already holds the replaced residual; z = B^-1 r,
+// the correction accumulator x restarts from 0, the search direction is kept.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void bpcg_restart_kernel(BlockPcgArgs<T> a) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double rz = 0.0, rr = 0.0;
+  if (v < a.nv) {
+    T r[3], z[3];
+#pragma unroll
+    for (int p = 0; p < 3; ++p) r[p] = a.r[3 * v + p];
+    apply_binv(a.binv, v, r, z);
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+      const int64_t k = 3 * v + p;
+      a.z[k] = z[p];
+      a.x[k] = T(0);
+      rz += (double)r[p] * (double)z[p];
+      rr += (double)r[p] * (double)r[p];
+    }
+  }
+  double vals[2] = {rz, rr};
+  store_block_partials<2>(vals, a.upart);
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.st->done = 0;
 }
 
 // SELL chunk of this wave: one chunk of kSellC rows per wave, kSellLanes
@@ -216,7 +244,7 @@ __device__ __forceinline__ void sell_row_product(const SellView& sv, const T* __
 // a partial of p.Ap.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void bpcg_spmv_kernel(BlockPcgArgs<T> a) {
-  const PcgCtl c = spmv_prologue(a.st, a.upart, a.gu, a.k, a.rtol2);
+  const PcgCtl c = spmv_prologue(a.st, a.upart, a.gu, a.k, a.rtol2, a.restart);
   if (!c.run) return;
   const T beta = (T)c.beta;
   const int lane = threadIdx.x % kWave;
@@ -527,22 +555,32 @@ BlockPcgArgs<T> block_pcg_args(sk_sim* S, const PcgBufs<T>& B, const T* b, doubl
   const int gs = sell_grid(m->nchunk);
   double* part = m->red.partials.p;
   return BlockPcgArgs<T>{m->nv, sell_view(m), B.vals, B.binv, S->fixed.p, b, B.x, B.r, B.z, B.p1, B.p0, B.Ap,
-                         S->st.p, rtol2, part, gu, part + 2 * gu, gs, 0};
+                         S->st.p, rtol2, part, gu, part + 2 * gu, gs, 0, 0};
 }
 
 // Block-Jacobi PCG on the assembled system with right-hand side b (x0 = 0).
+// With k0 > 0 it instead continues an earlier run after a residual
+// replacement: B.r holds the replaced residual, iterations continue from
+// index k0 with the kept search direction, and x accumulates the correction
+// from 0. Stops at ||r|| <= rtol ||r(start of this run)||; *iters is the
+// total iteration index reached.
 template <typename T>
 int run_block_pcg(sk_sim* S, const PcgBufs<T>& B, const T* b, double rtol, int64_t max_iter, hipStream_t st,
-                  int64_t* iters, int* done) {
+                  int64_t* iters, int* done, int64_t k0 = 0) {
   BlockPcgArgs<T> a = block_pcg_args<T>(S, B, b, rtol * rtol);
   SK_REQUIRE(2 * a.gu + a.gs <= 8 * Reducer::kMaxBlocks, "mesh too large for the PCG reductions");
-  a.pout = B.p0;  // init zeroes p0, the first "previous p"
-  bpcg_init_kernel<T><<<a.gu, kBlock, 0, st>>>(a);
+  if (k0 == 0) {
+    a.pout = B.p0;  // init zeroes p0, the first "previous p"
+    bpcg_init_kernel<T><<<a.gu, kBlock, 0, st>>>(a);
+  } else {
+    bpcg_restart_kernel<T><<<a.gu, kBlock, 0, st>>>(a);
+  }
   SK_HIP_CHECK(hipGetLastError());
-  int64_t launched = 0;
+  int64_t launched = k0;
   for (;;) {
     for (int k = 0; k < kPcgChunk; ++k, ++launched) {
       a.k = (int)launched;
+      a.restart = (k0 > 0 && launched == k0) ? 1 : 0;
       a.pin = (launched & 1) ? B.p1 : B.p0;
       a.pout = (launched & 1) ? B.p0 : B.p1;
       bpcg_spmv_kernel<T><<<a.gs, kBlock, 0, st>>>(a);
@@ -553,15 +591,20 @@ int run_block_pcg(sk_sim* S, const PcgBufs<T>& B, const T* b, double rtol, int64
     SK_HIP_CHECK(hipStreamSynchronize(st));
     if (S->st_host->done || S->st_host->iter >= max_iter) break;
     if (launched > max_iter + kPcgChunk) break;  // device state not advancing
+    if (k0 > 0 && S->st_host->iter < k0) break;   // continuation did not start
   }
   *iters = S->st_host->iter;
   *done = S->st_host->done;
   return SK_OK;
 }
 
-// Mixed-precision solve of A dx = b (b FP64 in S->g, dx FP64 in S->dx): FP64
-// iterative refinement r = b - A32 dx around FP32 block-Jacobi PCG solves of
-// A32 d = r to inner_rtol, until ||r|| <= pcg_rtol ||b||.
+// Mixed-precision solve of A dx = b (b FP64 in S->g, dx FP64 in S->dx). One
+// FP32 block-Jacobi PCG run on the FP32 matrix, refined in FP64: whenever the
+// FP32 recurrence residual has dropped by inner_rtol, the FP32 correction is
+// accumulated into the FP64 dx, the true residual r = b - A32 dx is recomputed
+// in FP64 and replaces the FP32 one, and CG continues with its current search
+// direction (residual replacement; the Krylov space is kept). Stops when the
+// FP64 residual satisfies ||r|| <= pcg_rtol ||b||.
 int run_mixed_solve(sk_sim* S, hipStream_t st, int64_t* iters, int* refines, int* done) {
   sk_mesh* m = S->mesh;
   const int64_t n3 = 3 * m->nv;
@@ -579,31 +622,32 @@ int run_mixed_solve(sk_sim* S, hipStream_t st, int64_t* iters, int* refines, int
   *refines = 0;
   *done = 1;
   const PcgBufs<float> B = bufs32(S);
-  for (int k = 0; k < S->prm.max_refine && rr > target && bb > 0.0; ++k) {
-    // Inner solve A32 d = r32 into x32. The right-hand side may alias the
-    // inner residual r32: the init launch reads each entry before the same
-    // thread overwrites it, and nothing reads the right-hand side afterwards.
-    int64_t it = 0;
+  int64_t k = 0;
+  for (int seg = 0; seg < S->prm.max_refine && rr > target && bb > 0.0; ++seg) {
+    // The right-hand side may alias the residual r32: the init launch reads
+    // each entry before the same thread overwrites it.
     int d = 0;
-    SK_TRY(run_block_pcg<float>(S, B, B.r, S->prm.inner_rtol, S->prm.pcg_max_iter - *iters, st, &it, &d));
-    *iters += it;
+    SK_TRY(run_block_pcg<float>(S, B, B.r, S->prm.inner_rtol, S->prm.pcg_max_iter, st, &k, &d, seg == 0 ? 0 : k));
     ++*refines;
     if (d == 2 || d == 3) {
+      *iters = k;
       *done = d;
       return SK_OK;
     }
-    ir_accumulate_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, B.x, S->dx.p, k == 0 ? 1 : 0);
+    ir_accumulate_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, B.x, S->dx.p, seg == 0 ? 1 : 0);
     ra.x = S->dx.p;
     ir_residual_kernel<<<gr, kBlock, 0, st>>>(ra);
     finalize_sum_kernel<1><<<1, kBlock, 0, st>>>(ra.partials, gr, rr_dev);
     SK_TRY(read_scalars(m, rr_dev, 1, st));
     rr = m->pinned[0];
     if (!std::isfinite(rr)) {
+      *iters = k;
       *done = 3;
       return SK_OK;
     }
-    if (*iters >= S->prm.pcg_max_iter) break;
+    if (k >= S->prm.pcg_max_iter || d != 1) break;
   }
+  *iters = k;
   if (bb == 0.0) SK_HIP_CHECK(hipMemsetAsync(S->dx.p, 0, n3 * sizeof(double), st));
   *done = (rr <= target) ? 1 : 0;
   return SK_OK;
