@@ -42,7 +42,10 @@ enum sk_status {
   SK_ERR_UNSUPPORTED = -9
 };
 
-enum sk_material { SK_SNH = 0, SK_ARAP = 1 };
+/* Element densities (energies/ in the reference): stable_neo_hookean.py,
+ * arap.py, linear_elasticity.py, macklin_mueller_neo_hookean.py, fcr.py.
+ * Every density but ARAP reads lam. */
+enum sk_material { SK_SNH = 0, SK_ARAP = 1, SK_LINEAR = 2, SK_MMNH = 3, SK_FCR = 4 };
 enum sk_precision {
   SK_PREC_FP64 = 0,  /* every stage in FP64 */
   SK_PREC_MIXED = 1  /* FP32 element Hessian, PSD projection, assembled values and inner PCG, wrapped in FP64
@@ -162,6 +165,18 @@ int sk_kinetic(int64_t n, const double* x, const double* x_curr, const double* x
                const int32_t* M_indptr, const int32_t* M_indices, const double* M_values, double* energy_host,
                double* grad, double* x_tilde, double* velocity, sk_stream_t stream);
 
+/* BDF2 kinetic terms (kinetic.py bdf2_target, velocity_bdf2,
+ * kinetic_energy_bdf2 / _gradient_bdf2, c = 9/4):
+ * x~ = (4/3) x_c - (1/3) x_p + (8h/9) v_c - (2h/9) v_p with
+ * v_c = (3 x_c - 4 x_p + x_p2)/(2h), v_p = (3 x_p - 4 x_p2 + x_p3)/(2h);
+ * d = x - x~, energy = 0.5 (9/4) d.(M d) / h^2 (host), grad = (9/4) (M d) / h^2;
+ * velocity receives v_c. Every output may be NULL; the four history levels
+ * are required. */
+int sk_kinetic_bdf2(int64_t n, const double* x, const double* x_curr, const double* x_prev, const double* x_prev2,
+                    const double* x_prev3, double h, const int32_t* M_indptr, const int32_t* M_indices,
+                    const double* M_values, double* energy_host, double* grad, double* x_tilde, double* velocity,
+                    sk_stream_t stream);
+
 /* ------------------------------------------------------------------------ */
 /* Linear solve                                                              */
 /* ------------------------------------------------------------------------ */
@@ -178,7 +193,7 @@ int sk_pcg_csr(int64_t n, const int32_t* indptr, const int32_t* indices, const d
 typedef struct sk_sim sk_sim;
 
 typedef struct sk_sim_params {
-  int material;          /* SK_SNH / SK_ARAP */
+  int material;          /* sk_material */
   int psd;               /* SK_PSD_PROJ (reference default psd=True) or SK_PSD_NONE */
   double h;              /* time step */
   double newton_rtol;    /* stop when ||g_free|| <= newton_rtol * ||g_free(x0)|| */
@@ -210,6 +225,8 @@ typedef struct sk_step_info {
   double ms_pcg;
   double ms_line_search;
   int refine_steps;      /* SK_PREC_MIXED: FP64 residual replacements summed over the Newton iterations */
+  int guess_used;        /* sk_sim_step_guess: 1 if Newton started from the supplied guess */
+  double guess_dE;       /* sk_sim_step_guess: E(x_guess) - E(x~), FP64 */
 } sk_step_info;
 
 /* mu, lam, vol: per element; mass: per DOF (3nv, M = kron(M_lumped, I3));
@@ -226,6 +243,25 @@ int sk_sim_destroy(sk_sim* sim);
  * Replaces backward_euler.py:27-91 -> newton.py:7-75 -> backtracking_line_search.py:8-67. */
 int sk_sim_step(sk_sim* sim, const double* x_curr, const double* x_prev, const double* x_bc, double* x_next,
                 sk_step_info* info, sk_stream_t stream);
+
+/* sk_sim_step with a candidate start point x_guess (3nv, device). Newton
+ * starts from x_guess (fixed DOFs replaced by their prescribed values) when
+ * its incremental potential is lower than at x~, else from x~ as
+ * sk_sim_step does; the stopping test stays ||g_free|| <= newton_rtol *
+ * ||g_free(x~)||, so both start points meet the same absolute tolerance.
+ * No counterpart in the reference (backward_euler.py:85 always starts from
+ * x~); it is the hook for a learned warm start. */
+int sk_sim_step_guess(sk_sim* sim, const double* x_curr, const double* x_prev, const double* x_bc,
+                      const double* x_guess, double* x_next, sk_step_info* info, sk_stream_t stream);
+
+/* One BDF2 step (bdf2.py:31-101 with kinetic.py bdf2_target / velocity_bdf2,
+ * c = 9/4): Newton on V(x) + c/(2h^2) |x - x~|_M^2 - f_ext.x from
+ * x0 = x~ = (4/3) x_c - (1/3) x_p + (8h/9) v_c - (2h/9) v_p, with
+ * v_c = (3 x_c - 4 x_p + x_p2)/(2h) and v_p = (3 x_p - 4 x_p2 + x_p3)/(2h).
+ * Same simulator, parameters and status codes as sk_sim_step. */
+int sk_sim_step_bdf2(sk_sim* sim, const double* x_curr, const double* x_prev, const double* x_prev2,
+                     const double* x_prev3, const double* x_bc, double* x_next, sk_step_info* info,
+                     sk_stream_t stream);
 
 /* Measurement hook, no counterpart in the reference: re-initialises PCG
  * scratch vectors on the system assembled by the last sk_sim_step (RHS = the

@@ -1,7 +1,8 @@
 """Oracle FEM element math, PSD projection and assembly (test infrastructure).
 
 FP64 numpy restatement of the reference's per-tet kinematics, stable
-neo-Hookean (SNH) and ARAP element tiers, ``psd_project``, and the
+neo-Hookean (SNH), ARAP, linear-elasticity, Macklin-Mueller neo-Hookean and
+fixed-corotational element tiers, ``psd_project``, and the
 ``J^T blockdiag(H) J`` Hessian assembly. Layout conventions follow the
 reference: ``F`` is row-major (``F[e, i, c]``, i spatial, c material) and
 9-vectors / 9x9 blocks index ``vec(F)`` row-major (``3*i + c``).
@@ -207,6 +208,91 @@ def arap_hessian_element_F(F, mu):
 
 
 # --------------------------------------------------------------------------- #
+# Linear elasticity, Macklin-Mueller neo-Hookean, fixed corotational          #
+# --------------------------------------------------------------------------- #
+_TRANSPOSE9 = np.zeros((9, 9))
+for _i in range(3):
+    for _j in range(3):
+        _TRANSPOSE9[3 * _i + _j, 3 * _j + _i] = 1.0
+_VECI = np.eye(3).reshape(9)
+
+
+def linear_energy_element_F(F, mu, lam):
+    """psi = mu ||eps||^2 + lam/2 tr(eps)^2, eps = sym(F) - I (linear_elasticity.py, energy element tier)."""
+    mu = np.asarray(mu, float).reshape(-1)
+    lam = np.asarray(lam, float).reshape(-1)
+    eps = 0.5 * (F + F.transpose(0, 2, 1)) - np.eye(3)[None]
+    return (mu * (eps ** 2).sum(axis=(1, 2)) + 0.5 * lam * np.trace(eps, axis1=1, axis2=2) ** 2)[:, None]
+
+
+def linear_gradient_element_F(F, mu, lam):
+    """P = mu (F + F^T - 2I) + lam tr(F - I) I (linear_elasticity.py, gradient element tier)."""
+    mu = np.asarray(mu, float).reshape(-1, 1, 1)
+    lam = np.asarray(lam, float).reshape(-1, 1, 1)
+    I = np.eye(3)[None]
+    tr = np.trace(F - I, axis1=1, axis2=2)[:, None, None]
+    return mu * (F + F.transpose(0, 2, 1) - 2.0 * I) + lam * tr * I
+
+
+def linear_hessian_element_F(F, mu, lam):
+    """H = mu (I + T) + lam vec(I) vec(I)^T, constant in F (linear_elasticity.py, hessian element tier)."""
+    mu = np.asarray(mu, float).reshape(-1, 1, 1)
+    lam = np.asarray(lam, float).reshape(-1, 1, 1)
+    H1 = (np.eye(9) + _TRANSPOSE9)[None]
+    H2 = np.outer(_VECI, _VECI)[None]
+    return np.broadcast_to(mu * H1 + lam * H2, (F.shape[0], 9, 9)).copy()
+
+
+def mmnh_energy_element_F(F, mu, lam):
+    """psi = mu (1 - J) + lam/2 (1 - J)^2 + mu/2 (I_C - 3) (macklin_mueller_neo_hookean.py, 3D energy)."""
+    mu = np.asarray(mu, float).reshape(-1)
+    lam = np.asarray(lam, float).reshape(-1)
+    c = 1.0 - np.linalg.det(F)
+    IC = np.einsum("eij,eij->e", F, F)
+    return (mu * c + 0.5 * lam * c ** 2 + 0.5 * mu * (IC - 3.0))[:, None]
+
+
+def mmnh_gradient_element_F(F, mu, lam):
+    """P = mu F - (mu + lam (1 - J)) cof(F)."""
+    mu = np.asarray(mu, float).reshape(-1)
+    lam = np.asarray(lam, float).reshape(-1)
+    d = lam * (np.linalg.det(F) - 1.0) - mu
+    return mu[:, None, None] * F + d[:, None, None] * _cof(F)
+
+
+def mmnh_hessian_element_F(F, mu, lam):
+    """H = mu I + lam vec(cof) vec(cof)^T + (lam (J - 1) - mu) dcof/dF."""
+    mu = np.asarray(mu, float).reshape(-1)
+    lam = np.asarray(lam, float).reshape(-1)
+    d = lam * (np.linalg.det(F) - 1.0) - mu
+    c = _cof(F).reshape(-1, 9)
+    return (mu[:, None, None] * np.eye(9)[None] + lam[:, None, None] * c[:, :, None] * c[:, None, :]
+            + d[:, None, None] * _dcof(F))
+
+
+def fcr_energy_element_F(F, mu, lam):
+    """psi = 2 psi_ARAP + lam/2 (J - 1)^2 (fcr.py, energy element tier)."""
+    lam = np.asarray(lam, float).reshape(-1)
+    return 2.0 * arap_energy_element_F(F, mu) + (0.5 * lam * (np.linalg.det(F) - 1.0) ** 2)[:, None]
+
+
+def fcr_gradient_element_F(F, mu, lam):
+    """P = 2 mu (F - R) + lam (J - 1) cof(F) (fcr.py, gradient element tier)."""
+    lam = np.asarray(lam, float).reshape(-1)
+    d = lam * (np.linalg.det(F) - 1.0)
+    return 2.0 * arap_gradient_element_F(F, mu) + d[:, None, None] * _cof(F)
+
+
+def fcr_hessian_element_F(F, mu, lam):
+    """H = 2 mu (I - dR/dF) + lam vec(cof) vec(cof)^T + lam (J - 1) dcof/dF (fcr.py, hessian element tier)."""
+    lam = np.asarray(lam, float).reshape(-1)
+    d = lam * (np.linalg.det(F) - 1.0)
+    c = _cof(F).reshape(-1, 9)
+    Hv = lam[:, None, None] * c[:, :, None] * c[:, None, :] + d[:, None, None] * _dcof(F)
+    return 2.0 * arap_hessian_element_F(F, mu) + Hv
+
+
+# --------------------------------------------------------------------------- #
 # PSD projection and assembly                                                 #
 # --------------------------------------------------------------------------- #
 def psd_project(H, method="proj"):
@@ -256,6 +342,9 @@ _ELEMENT = {
     "arap": (lambda F, mu, lam: arap_energy_element_F(F, mu),
              lambda F, mu, lam: arap_gradient_element_F(F, mu),
              lambda F, mu, lam: arap_hessian_element_F(F, mu)),
+    "linear": (linear_energy_element_F, linear_gradient_element_F, linear_hessian_element_F),
+    "mmnh": (mmnh_energy_element_F, mmnh_gradient_element_F, mmnh_hessian_element_F),
+    "fcr": (fcr_energy_element_F, fcr_gradient_element_F, fcr_hessian_element_F),
 }
 
 

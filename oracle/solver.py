@@ -71,6 +71,14 @@ def be_target(x_curr, x_prev, h):
     return x_curr + h * ((x_curr - x_prev) / h)
 
 
+def bdf2_target(x_curr, x_prev, x_prev2, x_prev3, h):
+    """BDF2 target (kinetic.py bdf2_target / velocity_bdf2), same order of operations:
+    (4/3) x_c - (1/3) x_p + (8h/9) v_c - (2h/9) v_p, v = (3 x_k - 4 x_k-1 + x_k-2) / (2h)."""
+    v_curr = (3.0 * x_curr - 4.0 * x_prev + x_prev2) / (2.0 * h)
+    v_prev = (3.0 * x_prev - 4.0 * x_prev2 + x_prev3) / (2.0 * h)
+    return (4.0 / 3.0) * x_curr - (1.0 / 3.0) * x_prev + (8.0 * h / 9.0) * v_curr - (2.0 * h / 9.0) * v_prev
+
+
 def block_jacobi_pcg(A, b, free, rtol=1e-8, maxit=10000):
     """CPU block-Jacobi PCG on the free DOFs (same algorithm the GPU runs; for baselines)."""
     n = A.shape[0]
@@ -110,14 +118,15 @@ def block_jacobi_pcg(A, b, free, rtol=1e-8, maxit=10000):
 class BEProblem:
     """Backward-Euler harness on a tet mesh with eliminated (pinned/prescribed) DOFs.
 
-    E(x) = V(x) + 1/(2h^2) (x - x~)^T M (x - x~) - f_ext^T x, minimised over the
+    E(x) = V(x) + c/(2h^2) (x - x~)^T M (x - x~) - f_ext^T x, minimised over the
     free DOFs of x (backward_euler.py:76-91 with the kinetic terms of
     kinetic.py:107-194 and the gravity load of gravity_force.py:37-41).
+    integrator="bdf2" uses c = 9/4 and the BDF2 target (bdf2.py:86-101).
     """
 
     def __init__(self, X, T, mu, lam, rho, h, fixed_vertices, material="snh",
                  gravity=0.0, prescribe=None, linsolve="spsolve", newton_rtol=1e-6,
-                 newton_max_iter=100, pcg_rtol=1e-8):
+                 newton_max_iter=100, pcg_rtol=1e-8, integrator="be"):
         self.X = np.asarray(X, float)
         self.T = np.asarray(T)
         self.nv = X.shape[0]
@@ -141,15 +150,17 @@ class BEProblem:
         self.newton_rtol = newton_rtol
         self.newton_max_iter = newton_max_iter
         self.pcg_rtol = pcg_rtol
+        self.integrator = integrator
+        self.c = {"be": 1.0, "bdf2": 9.0 / 4.0}[integrator]
 
     # -- incremental potential pieces -------------------------------------
     def potential_gradient(self, x, xt):
         g = fem.gradient_x(self.material, x, self.J, self.mu, self.lam, self.vol).reshape(-1)
-        return g + self.m * (x - xt) / self.h ** 2 - self.f_ext
+        return g + self.c * self.m * (x - xt) / self.h ** 2 - self.f_ext
 
     def potential_hessian(self, x):
         Q = fem.hessian_x(self.material, x, self.J, self.mu, self.lam, self.vol, psd=True)
-        return (Q + sps.diags(self.m / self.h ** 2)).tocsr()
+        return (Q + sps.diags(self.c * self.m / self.h ** 2)).tocsr()
 
     def energy_delta(self, x0, x1, xt):
         """E(x1) - E(x0) summed as per-element differences (see module docstring)."""
@@ -157,12 +168,15 @@ class BEProblem:
         e0 = fem.element_energies_x(self.material, x0, self.J, self.mu, self.lam, self.vol)
         d0 = x0 - xt
         d1 = x1 - xt
-        kin = 0.5 / self.h ** 2 * (self.m * (d1 - d0) * (d1 + d0)).sum()
+        kin = 0.5 * self.c / self.h ** 2 * (self.m * (d1 - d0) * (d1 + d0)).sum()
         return (e1 - e0).sum() + kin - self.f_ext @ (x1 - x0)
 
-    def step(self, x_curr, x_prev, t_next, return_info=False):
-        """One BE step; returns x_next (and a dict of per-step counters)."""
-        xt = be_target(x_curr, x_prev, self.h)
+    def step(self, x_curr, x_prev, t_next, return_info=False, x_prev2=None, x_prev3=None):
+        """One BE (or BDF2) step; returns x_next (and a dict of per-step counters)."""
+        if self.integrator == "bdf2":
+            xt = bdf2_target(x_curr, x_prev, x_prev2, x_prev3, self.h)
+        else:
+            xt = be_target(x_curr, x_prev, self.h)
         x = xt.copy()
         if self.prescribe is not None:
             x[self.fixed] = self.prescribe(t_next).reshape(-1)[self.fixed]
@@ -205,12 +219,13 @@ class BEProblem:
 
     def rollout(self, steps, x0=None, return_info=False):
         """Run `steps` BE steps from rest (x_prev = x_curr = X); returns the list of states."""
-        x_prev = self.X.reshape(-1).copy() if x0 is None else x0.copy()
-        x_curr = x_prev.copy()
+        start = self.X.reshape(-1).copy() if x0 is None else x0.copy()
+        hist = [start, start.copy(), start.copy(), start.copy()]
         states, infos = [], []
         for n in range(steps):
-            x_next, info = self.step(x_curr, x_prev, (n + 1) * self.h, return_info=True)
+            x_next, info = self.step(hist[0], hist[1], (n + 1) * self.h, return_info=True, x_prev2=hist[2],
+                                     x_prev3=hist[3])
             states.append(x_next)
             infos.append(info)
-            x_prev, x_curr = x_curr, x_next
+            hist = [x_next] + hist[:3]
         return (states, infos) if return_info else states

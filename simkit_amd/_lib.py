@@ -24,6 +24,9 @@ SK_ERR_UNSUPPORTED = -9
 
 SK_SNH = 0
 SK_ARAP = 1
+SK_LINEAR = 2
+SK_MMNH = 3
+SK_FCR = 4
 SK_PREC_FP64 = 0
 SK_PREC_MIXED = 1
 
@@ -31,7 +34,13 @@ SK_PSD_NONE = 0
 SK_PSD_PROJ = 1
 SK_PSD_ABS = 2
 
-MATERIALS = {"snh": SK_SNH, "stable_neo_hookean": SK_SNH, "arap": SK_ARAP}
+MATERIALS = {
+    "snh": SK_SNH, "stable_neo_hookean": SK_SNH, "stable-neo-hookean": SK_SNH,
+    "arap": SK_ARAP,
+    "linear": SK_LINEAR, "linear-elasticity": SK_LINEAR,
+    "mmnh": SK_MMNH, "macklin-mueller-neo-hookean": SK_MMNH,
+    "fcr": SK_FCR,
+}
 
 
 class SimkitAmdError(RuntimeError):
@@ -76,6 +85,8 @@ class SkStepInfo(ctypes.Structure):
         ("ms_pcg", ctypes.c_double),
         ("ms_line_search", ctypes.c_double),
         ("refine_steps", ctypes.c_int),
+        ("guess_used", ctypes.c_int),
+        ("guess_dE", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -109,10 +120,13 @@ SIGNATURES = {
     "sk_mesh_hessian": [_P, _INT, _P, _P, _P, _P, _P, _INT, _P, _P],
     "sk_csr_spmv": [_I64, _P, _P, _P, _P, _P, _P],
     "sk_kinetic": [_I64, _P, _P, _P, _D, _D, _P, _P, _P, ctypes.POINTER(_D), _P, _P, _P, _P],
+    "sk_kinetic_bdf2": [_I64, _P, _P, _P, _P, _P, _D, _P, _P, _P, ctypes.POINTER(_D), _P, _P, _P, _P],
     "sk_pcg_csr": [_I64, _P, _P, _P, _P, _P, _D, _I64, ctypes.POINTER(_I64), ctypes.POINTER(_D), _P],
     "sk_sim_create": [_P, ctypes.POINTER(SkSimParams), _P, _P, _P, _P, _P, _P, ctypes.POINTER(_P)],
     "sk_sim_destroy": [_P],
     "sk_sim_step": [_P, _P, _P, _P, _P, ctypes.POINTER(SkStepInfo), _P],
+    "sk_sim_step_guess": [_P, _P, _P, _P, _P, _P, ctypes.POINTER(SkStepInfo), _P],
+    "sk_sim_step_bdf2": [_P, _P, _P, _P, _P, _P, _P, ctypes.POINTER(SkStepInfo), _P],
     "sk_sim_spmv_reps": [_P, _INT, _P],
 }
 _RESTYPES = {"sk_last_error": ctypes.c_char_p, "sk_status_string": ctypes.c_char_p}

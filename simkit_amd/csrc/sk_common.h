@@ -10,6 +10,9 @@
 
 namespace sk {
 
+// Whether a material reads lam (every density except ARAP).
+constexpr bool material_needs_lam(int material) { return material != SK_ARAP; }
+
 // Last error message of the calling host thread (returned by sk_last_error).
 void set_error(const std::string& msg);
 

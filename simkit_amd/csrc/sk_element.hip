@@ -212,14 +212,12 @@ int sk_element_F(int material, const double* F, const double* mu, const double* 
   if (nt == 0) return SK_OK;
   SK_REQUIRE(F && mu, "sk_element_F: null F or mu");
   const int g = grid_for(nt);
-  if (material == SK_SNH) {
-    SK_REQUIRE(lam, "sk_element_F: stable neo-Hookean needs lam");
-    element_F_kernel<SNH<double>><<<g, kBlock, 0, as_stream(s)>>>(F, mu, lam, nt, psi, P, H);
-  } else if (material == SK_ARAP) {
-    element_F_kernel<ARAP<double>><<<g, kBlock, 0, as_stream(s)>>>(F, mu, lam, nt, psi, P, H);
-  } else {
-    SK_REQUIRE(false, "sk_element_F: unknown material");
-  }
+  SK_REQUIRE(!material_needs_lam(material) || lam, "sk_element_F: this material needs lam");
+  SK_REQUIRE(dispatch_material(material, [&](auto tag) {
+               element_F_kernel<typename decltype(tag)::template type<double>>
+                   <<<g, kBlock, 0, as_stream(s)>>>(F, mu, lam, nt, psi, P, H);
+             }),
+             "sk_element_F: unknown material");
   SK_HIP_CHECK(hipGetLastError());
   return SK_OK;
 }

@@ -8,7 +8,6 @@
 
 namespace sk {
 
-enum Material : int { kSNH = 0, kARAP = 1 };
 
 // Levi-Civita sign of (i, k, n) for distinct indices in {0,1,2}.
 constexpr __host__ __device__ int levi(int i, int k) { return ((k - i + 3) % 3 == 1) ? 1 : -1; }
@@ -26,6 +25,29 @@ __device__ __forceinline__ void cofactor3(const R (&F)[9], R (&c)[9]) {
   c[8] = F[0] * F[4] - F[1] * F[3];
 }
 
+// H = a I + b vecF vecF^T + c vec(cof) vec(cof)^T + d dvec(cof)/dvecF, the
+// shape shared by every density built from I_C = ||F||^2 and J = det F, with
+// d cof_ij / d F_kl = eps_ikn eps_jlq F_nq.
+template <typename R>
+__device__ __forceinline__ void cofactor_form_hessian(const R (&F)[9], const R (&cof)[9], R a, R b, R c, R d,
+                                                      R (&H)[45]) {
+#pragma unroll
+  for (int p = 0; p < 9; ++p) {
+#pragma unroll
+    for (int q = p; q < 9; ++q) {
+      R h = b * F[p] * F[q] + c * cof[p] * cof[q];
+      if (p == q) h += a;
+      const int i = p / 3, j = p % 3, k = q / 3, l = q % 3;
+      if (i != k && j != l) {
+        const int n = 3 - i - k, m = 3 - j - l;
+        const int sg = levi(i, k) * levi(j, l);
+        h += (sg > 0 ? d : -d) * F[3 * n + m];
+      }
+      H[Sym<9>::idx(p, q)] = h;
+    }
+  }
+}
+
 // Stable neo-Hookean (Smith, de Goes, Kim 2018), d = 3:
 //   psi = mu/2 (I_C - 3) - mu/2 log(I_C + 1) + lam/2 (J - alpha)^2,
 //   alpha = 1 + 3 mu / (4 lam)                 (stable_neo_hookean.py:84-126)
@@ -35,6 +57,7 @@ __device__ __forceinline__ void cofactor3(const R (&F)[9], R (&c)[9]) {
 // with d cof_ij / d F_kl = eps_ikn eps_jlq F_nq.
 template <typename R>
 struct SNH {
+  static constexpr bool kPolar = false;  // no polar decomposition in the density
   struct Scal {
     R IC, J, alpha, cof[9];
   };
@@ -64,24 +87,7 @@ struct SNH {
   static __device__ __forceinline__ void hessian(const R (&F)[9], R mu, R lam, R (&H)[45]) {
     const Scal s = scalars(F, mu, lam);
     const R den = s.IC + R(1);
-    const R A = mu * s.IC / den;
-    const R B = R(2) * mu / (den * den);
-    const R D = lam * (s.J - s.alpha);
-#pragma unroll
-    for (int a = 0; a < 9; ++a) {
-#pragma unroll
-      for (int b = a; b < 9; ++b) {
-        R h = B * F[a] * F[b] + lam * s.cof[a] * s.cof[b];
-        if (a == b) h += A;
-        const int i = a / 3, j = a % 3, k = b / 3, l = b % 3;
-        if (i != k && j != l) {
-          const int n = 3 - i - k, q = 3 - j - l;
-          const int sg = levi(i, k) * levi(j, l);
-          h += (sg > 0 ? D : -D) * F[3 * n + q];
-        }
-        H[Sym<9>::idx(a, b)] = h;
-      }
-    }
+    cofactor_form_hessian(F, s.cof, mu * s.IC / den, R(2) * mu / (den * den), lam, lam * (s.J - s.alpha), H);
   }
 };
 
@@ -91,6 +97,7 @@ struct SNH {
 // (arap.py:86-145, polar_svd.py:209-218, rotation_gradient.py:42-72).
 template <typename R>
 struct ARAP {
+  static constexpr bool kPolar = true;
   static __device__ __forceinline__ void rotation(const R (&U)[9], const R (&V)[9], R (&Rm)[9]) {
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -144,5 +151,131 @@ struct ARAP {
       }
   }
 };
+
+// Linear (small-strain) elasticity: psi = mu ||eps||^2 + lam/2 tr(eps)^2,
+// eps = (F + F^T)/2 - I; P = mu (F + F^T - 2 I) + lam tr(F - I) I;
+// H = mu (I + T) + lam vec(I) vec(I)^T with T the vec-transpose permutation
+// (linear_elasticity.py: energy, gradient and hessian element tiers).
+template <typename R>
+struct Linear {
+  static constexpr bool kPolar = false;
+  static __device__ __forceinline__ R energy(const R (&F)[9], R mu, R lam) {
+    R acc = 0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const R e = R(0.5) * (F[3 * i + j] + F[3 * j + i]) - (i == j ? R(1) : R(0));
+        acc += e * e;
+      }
+    const R tr = F[0] + F[4] + F[8] - R(3);
+    return mu * acc + R(0.5) * lam * tr * tr;
+  }
+  static __device__ __forceinline__ void gradient(const R (&F)[9], R mu, R lam, R (&P)[9]) {
+    const R tr = F[0] + F[4] + F[8] - R(3);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        P[3 * i + j] = mu * (F[3 * i + j] + F[3 * j + i] - (i == j ? R(2) : R(0))) + (i == j ? lam * tr : R(0));
+  }
+  static __device__ __forceinline__ void hessian(const R (&/*F*/)[9], R mu, R lam, R (&H)[45]) {
+#pragma unroll
+    for (int p = 0; p < 9; ++p)
+#pragma unroll
+      for (int q = p; q < 9; ++q) {
+        const int i = p / 3, j = p % 3, k = q / 3, l = q % 3;
+        R h = R(0);
+        if (i == k && j == l) h += mu;
+        if (i == l && j == k) h += mu;
+        if (i == j && k == l) h += lam;
+        H[Sym<9>::idx(p, q)] = h;
+      }
+  }
+};
+
+// Macklin-Mueller neo-Hookean (macklin_mueller_neo_hookean.py, 3D element tier):
+//   psi = mu (1 - J) + lam/2 (1 - J)^2 + mu/2 (I_C - 3)
+//   P = mu F + (lam (J - 1) - mu) cof F
+//   H = mu I + lam vec(cof) vec(cof)^T + (lam (J - 1) - mu) dvec(cof)/dvecF
+template <typename R>
+struct MMNH {
+  static constexpr bool kPolar = false;
+  static __device__ __forceinline__ R energy(const R (&F)[9], R mu, R lam) {
+    R cof[9];
+    cofactor3(F, cof);
+    R ic = 0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) ic += F[k] * F[k];
+    const R c = R(1) - (F[0] * cof[0] + F[1] * cof[1] + F[2] * cof[2]);
+    return mu * c + R(0.5) * lam * c * c + R(0.5) * mu * (ic - R(3));
+  }
+  static __device__ __forceinline__ void gradient(const R (&F)[9], R mu, R lam, R (&P)[9]) {
+    R cof[9];
+    cofactor3(F, cof);
+    const R J = F[0] * cof[0] + F[1] * cof[1] + F[2] * cof[2];
+    const R d = lam * (J - R(1)) - mu;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) P[k] = mu * F[k] + d * cof[k];
+  }
+  static __device__ __forceinline__ void hessian(const R (&F)[9], R mu, R lam, R (&H)[45]) {
+    R cof[9];
+    cofactor3(F, cof);
+    const R J = F[0] * cof[0] + F[1] * cof[1] + F[2] * cof[2];
+    cofactor_form_hessian(F, cof, mu, R(0), lam, lam * (J - R(1)) - mu, H);
+  }
+};
+
+// Fixed corotational (fcr.py, 3D element tier): twice the ARAP density plus
+// a volume term,
+//   psi = mu ||F - R||^2 + lam/2 (J - 1)^2
+//   P = 2 mu (F - R) + lam (J - 1) cof F
+//   H = 2 mu (I - dR/dF) + lam vec(cof) vec(cof)^T + lam (J - 1) dvec(cof)/dvecF
+template <typename R>
+struct FCR {
+  static constexpr bool kPolar = true;
+  static __device__ __forceinline__ R energy(const R (&F)[9], R mu, R lam) {
+    R cof[9];
+    cofactor3(F, cof);
+    const R dj = F[0] * cof[0] + F[1] * cof[1] + F[2] * cof[2] - R(1);
+    return ARAP<R>::energy(F, R(2) * mu, lam) + R(0.5) * lam * dj * dj;
+  }
+  static __device__ __forceinline__ void gradient(const R (&F)[9], R mu, R lam, R (&P)[9]) {
+    R cof[9];
+    cofactor3(F, cof);
+    const R d = lam * (F[0] * cof[0] + F[1] * cof[1] + F[2] * cof[2] - R(1));
+    ARAP<R>::gradient(F, R(2) * mu, lam, P);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) P[k] += d * cof[k];
+  }
+  static __device__ __forceinline__ void hessian(const R (&F)[9], R mu, R lam, R (&H)[45]) {
+    R cof[9], Hv[45];
+    cofactor3(F, cof);
+    const R J = F[0] * cof[0] + F[1] * cof[1] + F[2] * cof[2];
+    cofactor_form_hessian(F, cof, R(0), R(0), lam, lam * (J - R(1)), Hv);
+    ARAP<R>::hessian(F, R(2) * mu, lam, H);
+#pragma unroll
+    for (int k = 0; k < 45; ++k) H[k] += Hv[k];
+  }
+};
+
+// Calls fn(tag) with tag::type<R> naming the material's density template.
+template <template <typename> class M>
+struct MaterialTag {
+  template <typename R>
+  using type = M<R>;
+};
+
+template <typename Fn>
+inline bool dispatch_material(int material, Fn&& fn) {
+  switch (material) {
+    case 0: fn(MaterialTag<SNH>{}); return true;
+    case 1: fn(MaterialTag<ARAP>{}); return true;
+    case 2: fn(MaterialTag<Linear>{}); return true;
+    case 3: fn(MaterialTag<MMNH>{}); return true;
+    case 4: fn(MaterialTag<FCR>{}); return true;
+    default: return false;
+  }
+}
 
 }  // namespace sk

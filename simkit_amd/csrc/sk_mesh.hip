@@ -155,39 +155,29 @@ __global__ __launch_bounds__(kBlock) void elem_grad_kernel(ElemArgs a) {
 
 // Element Hessian in precision R (double, or float for the mixed-precision
 // path). F is formed in FP64 from the FP64 positions and rounded to R; in FP32
-// the ARAP twists (the polar SVD) of elements with det F < 1e-3 are still
-// computed in FP64, where the FP32 SVD of a near-singular F loses the
-// rotation.
-template <class M, typename R>
-struct ElemHessian {
-  static __device__ __forceinline__ void eval(const double (&F)[9], double mu, double lam, R (&H)[45]) {
-    R Fr[9];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) Fr[k] = (R)F[k];
-    M::hessian(Fr, (R)mu, (R)lam, H);
-  }
-};
-
-template <>
-struct ElemHessian<ARAP<float>, float> {
-  static __device__ __forceinline__ void eval(const double (&F)[9], double mu, double lam, float (&H)[45]) {
+// the densities that use the polar decomposition (ARAP, FCR) still evaluate
+// elements with det F < 1e-3 in FP64, where the FP32 SVD of a near-singular F
+// loses the rotation.
+template <template <typename> class M, typename R>
+__device__ __forceinline__ void elem_hessian_eval(const double (&F)[9], double mu, double lam, R (&H)[45]) {
+  if (sizeof(R) == 4 && M<R>::kPolar) {
     const double det = F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) +
                        F[2] * (F[3] * F[7] - F[4] * F[6]);
     if (det < 1e-3) {
       double Hd[45];
-      ARAP<double>::hessian(F, mu, lam, Hd);
+      M<double>::hessian(F, mu, lam, Hd);
 #pragma unroll
-      for (int k = 0; k < 45; ++k) H[k] = (float)Hd[k];
-    } else {
-      float Fr[9];
-#pragma unroll
-      for (int k = 0; k < 9; ++k) Fr[k] = (float)F[k];
-      ARAP<float>::hessian(Fr, (float)mu, (float)lam, H);
+      for (int k = 0; k < 45; ++k) H[k] = (R)Hd[k];
+      return;
     }
   }
-};
+  R Fr[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) Fr[k] = (R)F[k];
+  M<R>::hessian(Fr, (R)mu, (R)lam, H);
+}
 
-template <class M, int PSD, typename R>
+template <template <typename> class M, int PSD, typename R>
 __global__ __launch_bounds__(kBlock) void elem_hess_kernel(ElemArgs a) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= a.m.nt) return;
@@ -198,7 +188,7 @@ __global__ __launch_bounds__(kBlock) void elem_hess_kernel(ElemArgs a) {
   make_F(xe, Dd, F);
   add_F0(a.F0, e, F);
   R H[45];
-  ElemHessian<M, R>::eval(F, a.mu[e], a.lam ? a.lam[e] : 0.0, H);
+  elem_hessian_eval<M, R>(F, a.mu[e], a.lam ? a.lam[e] : 0.0, H);
   const R vol = (R)a.vol[e];
 #pragma unroll
   for (int k = 0; k < 45; ++k) H[k] *= vol;
@@ -426,9 +416,10 @@ int launch_elem_energy(sk_mesh* m, int material, const double* x, const double* 
   const int64_t n = std::max<int64_t>(m->nt, mass ? m->nv : 0);
   const int g = grid_for(n);
   SK_REQUIRE(g <= Reducer::kMaxBlocks, "mesh too large for the energy reduction");
-  if (material == SK_SNH) elem_energy_kernel<SNH<double>><<<g, kBlock, 0, st>>>(a);
-  else if (material == SK_ARAP) elem_energy_kernel<ARAP<double>><<<g, kBlock, 0, st>>>(a);
-  else SK_REQUIRE(false, "unknown material");
+  SK_REQUIRE(dispatch_material(material, [&](auto tag) {
+               elem_energy_kernel<typename decltype(tag)::template type<double>><<<g, kBlock, 0, st>>>(a);
+             }),
+             "unknown material");
   finalize_sum_kernel<1><<<1, kBlock, 0, st>>>(m->red.partials.p, g, m->red.result.p);
   SK_HIP_CHECK(hipGetLastError());
   return SK_OK;
@@ -438,18 +429,25 @@ int launch_elem_grad(sk_mesh* m, int material, const double* x, const double* F0
                      const double* lam, const double* vol, hipStream_t st) {
   ElemArgs a{view(m), x, F0, mu, lam, vol, m->fst.p, m->Kst.p, 0};
   const int g = grid_for(m->nt);
-  if (material == SK_SNH) elem_grad_kernel<SNH<double>><<<g, kBlock, 0, st>>>(a);
-  else if (material == SK_ARAP) elem_grad_kernel<ARAP<double>><<<g, kBlock, 0, st>>>(a);
-  else SK_REQUIRE(false, "unknown material");
+  SK_REQUIRE(dispatch_material(material, [&](auto tag) {
+               elem_grad_kernel<typename decltype(tag)::template type<double>><<<g, kBlock, 0, st>>>(a);
+             }),
+             "unknown material");
   SK_HIP_CHECK(hipGetLastError());
   return SK_OK;
 }
 
-template <class M, typename R>
+template <template <typename> class M, typename R>
 static void hess_dispatch(int psd, int g, hipStream_t st, const ElemArgs& a) {
   if (psd == SK_PSD_PROJ) elem_hess_kernel<M, SK_PSD_PROJ, R><<<g, kBlock, 0, st>>>(a);
   else if (psd == SK_PSD_ABS) elem_hess_kernel<M, SK_PSD_ABS, R><<<g, kBlock, 0, st>>>(a);
   else elem_hess_kernel<M, SK_PSD_NONE, R><<<g, kBlock, 0, st>>>(a);
+}
+
+template <template <typename> class M>
+static void hess_dispatch_prec(bool fp32, int psd, int g, hipStream_t st, const ElemArgs& a) {
+  if (fp32) hess_dispatch<M, float>(psd, g, st, a);
+  else hess_dispatch<M, double>(psd, g, st, a);
 }
 
 int launch_elem_hess(sk_mesh* m, int material, const double* x, const double* F0, const double* mu,
@@ -457,15 +455,16 @@ int launch_elem_hess(sk_mesh* m, int material, const double* x, const double* F0
   SK_REQUIRE(psd == SK_PSD_NONE || psd == SK_PSD_PROJ || psd == SK_PSD_ABS, "invalid psd method");
   ElemArgs a{view(m), x, F0, mu, lam, vol, m->fst.p, m->Kst.p, psd};
   const int g = grid_for(m->nt);
-  if (material == SK_SNH) {
-    if (fp32) hess_dispatch<SNH<float>, float>(psd, g, st, a);
-    else hess_dispatch<SNH<double>, double>(psd, g, st, a);
-  } else if (material == SK_ARAP) {
-    if (fp32) hess_dispatch<ARAP<float>, float>(psd, g, st, a);
-    else hess_dispatch<ARAP<double>, double>(psd, g, st, a);
-  } else {
-    SK_REQUIRE(false, "unknown material");
+  bool known = true;
+  switch (material) {
+    case SK_SNH: hess_dispatch_prec<SNH>(fp32, psd, g, st, a); break;
+    case SK_ARAP: hess_dispatch_prec<ARAP>(fp32, psd, g, st, a); break;
+    case SK_LINEAR: hess_dispatch_prec<Linear>(fp32, psd, g, st, a); break;
+    case SK_MMNH: hess_dispatch_prec<MMNH>(fp32, psd, g, st, a); break;
+    case SK_FCR: hess_dispatch_prec<FCR>(fp32, psd, g, st, a); break;
+    default: known = false;
   }
+  SK_REQUIRE(known, "unknown material");
   SK_HIP_CHECK(hipGetLastError());
   return SK_OK;
 }
@@ -688,7 +687,7 @@ int sk_mesh_element_nnz(const sk_mesh* m, int32_t* elem_nnz, sk_stream_t s) {
 int sk_mesh_energy(sk_mesh* m, int material, const double* x, const double* F0, const double* mu,
                    const double* lam, const double* vol, double* energy_host, double* elem_energy, sk_stream_t s) {
   SK_REQUIRE(m && x && mu && vol, "sk_mesh_energy: null argument");
-  SK_REQUIRE(material == SK_ARAP || lam, "sk_mesh_energy: stable neo-Hookean needs lam");
+  SK_REQUIRE(!material_needs_lam(material) || lam, "sk_mesh_energy: this material needs lam");
   hipStream_t st = as_stream(s);
   SK_TRY(launch_elem_energy(m, material, x, F0, nullptr, 0.0, mu, lam, vol, elem_energy, nullptr, nullptr, nullptr,
                             nullptr, 0.0, st));
@@ -702,7 +701,7 @@ int sk_mesh_energy(sk_mesh* m, int material, const double* x, const double* F0, 
 int sk_mesh_gradient(sk_mesh* m, int material, const double* x, const double* F0, const double* mu,
                      const double* lam, const double* vol, double* g, sk_stream_t s) {
   SK_REQUIRE(m && x && mu && vol && g, "sk_mesh_gradient: null argument");
-  SK_REQUIRE(material == SK_ARAP || lam, "sk_mesh_gradient: stable neo-Hookean needs lam");
+  SK_REQUIRE(!material_needs_lam(material) || lam, "sk_mesh_gradient: this material needs lam");
   hipStream_t st = as_stream(s);
   SK_TRY(launch_elem_grad(m, material, x, F0, mu, lam, vol, st));
   SK_TRY(launch_gather_grad(m, g, nullptr, nullptr, nullptr, nullptr, nullptr, 0.0, st));
@@ -712,7 +711,7 @@ int sk_mesh_gradient(sk_mesh* m, int material, const double* x, const double* F0
 int sk_mesh_hessian(sk_mesh* m, int material, const double* x, const double* F0, const double* mu,
                     const double* lam, const double* vol, int psd, double* values, sk_stream_t s) {
   SK_REQUIRE(m && x && mu && vol && values, "sk_mesh_hessian: null argument");
-  SK_REQUIRE(material == SK_ARAP || lam, "sk_mesh_hessian: stable neo-Hookean needs lam");
+  SK_REQUIRE(!material_needs_lam(material) || lam, "sk_mesh_hessian: this material needs lam");
   hipStream_t st = as_stream(s);
   SK_TRY(launch_elem_hess(m, material, x, F0, mu, lam, vol, psd, false, st));
   SK_TRY(launch_gather_hess(m, values, nullptr, 0.0, nullptr, nullptr, false, false, st));

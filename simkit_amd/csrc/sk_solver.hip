@@ -473,6 +473,30 @@ __global__ void be_init_kernel(int64_t n, const double* xc, const double* xp, co
   x[k] = (fixed && fixed[k]) ? (xbc ? xbc[k] : c) : t;
 }
 
+// BDF2 inertial target (kinetic.py bdf2_target / velocity_bdf2, in the
+// reference's order of operations):
+//   v_c = (3 x_c - 4 x_p + x_p2) / (2h),  v_p = (3 x_p - 4 x_p2 + x_p3) / (2h)
+//   x~ = (4/3) x_c - (1/3) x_p + (8h/9) v_c - (2h/9) v_p.
+__global__ void bdf2_init_kernel(int64_t n, const double* xc, const double* xp, const double* xp2, const double* xp3,
+                                 const double* xbc, const uint8_t* fixed, double h, double* x, double* xt) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const double c = xc[k], p = xp[k], p2 = xp2[k], p3 = xp3[k];
+  const double vc = (3.0 * c - 4.0 * p + p2) / (2.0 * h);
+  const double vp = (3.0 * p - 4.0 * p2 + p3) / (2.0 * h);
+  const double t = (4.0 / 3.0) * c - (1.0 / 3.0) * p + (8.0 * h / 9.0) * vc - (2.0 * h / 9.0) * vp;
+  xt[k] = t;
+  x[k] = (fixed && fixed[k]) ? (xbc ? xbc[k] : c) : t;
+}
+
+// dx = guess - x on free DOFs, 0 on fixed DOFs (which keep their prescribed
+// values whichever start point is taken).
+__global__ void guess_dx_kernel(int64_t n, const double* guess, const double* x, const uint8_t* fixed, double* dx) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  dx[k] = (fixed && fixed[k]) ? 0.0 : guess[k] - x[k];
+}
+
 __global__ void axpy_kernel(int64_t n, double t, const double* dx, double* x) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) x[k] = x[k] + t * dx[k];
@@ -727,8 +751,8 @@ int sk_sim_create(sk_mesh* mesh, const sk_sim_params* p, const double* mu, const
   SK_REQUIRE(out, "sk_sim_create: null out");
   *out = nullptr;
   SK_REQUIRE(mesh && p && mu && vol && mass, "sk_sim_create: null argument");
-  SK_REQUIRE(p->material == SK_ARAP || lam, "sk_sim_create: stable neo-Hookean needs lam");
-  SK_REQUIRE(p->material == SK_SNH || p->material == SK_ARAP, "sk_sim_create: unknown material");
+  SK_REQUIRE(p->material >= SK_SNH && p->material <= SK_FCR, "sk_sim_create: unknown material");
+  SK_REQUIRE(!material_needs_lam(p->material) || lam, "sk_sim_create: this material needs lam");
   SK_REQUIRE(p->h > 0.0, "sk_sim_create: h must be positive");
   SK_REQUIRE(p->ls_alpha > 0.0 && p->ls_alpha <= 0.5, "sk_sim_create: ls_alpha must be in (0, 0.5]");
   SK_REQUIRE(p->ls_beta > 0.0 && p->ls_beta < 1.0, "sk_sim_create: ls_beta must be in (0, 1)");
@@ -791,23 +815,28 @@ int sk_sim_destroy(sk_sim* S) {
   return SK_OK;
 }
 
-int sk_sim_step(sk_sim* S, const double* x_curr, const double* x_prev, const double* x_bc, double* x_next,
-                sk_step_info* info, sk_stream_t s) {
-  SK_REQUIRE(S && x_curr && x_prev && x_next, "sk_sim_step: null argument");
+}  // extern "C"
+
+namespace {
+
+// Newton on E(x) = V(x) + c/(2h^2) |x - x~|_M^2 - f_ext.x for one implicit
+// step; S->x and S->xt hold the start point and x~ (written by the caller's
+// init kernel). With x_guess, the candidate start point is taken instead when
+// its incremental potential is lower (E(x_guess) - E(x~) < 0, evaluated in
+// FP64 in difference form); the stopping test stays relative to the gradient
+// norm at x~, so both start points converge to the same absolute tolerance.
+int step_core(sk_sim* S, double c_mass, const double* x_guess, double* x_next, sk_step_info* info, hipStream_t st) {
   sk_mesh* m = S->mesh;
-  hipStream_t st = as_stream(s);
   const sk_sim_params& P = S->prm;
   const bool mixed = S->mixed();
   const int64_t n3 = 3 * m->nv;
-  const double inv_h2 = 1.0 / (P.h * P.h);
+  const double inv_h2 = c_mass / (P.h * P.h);
   const double* lam = S->lam.p;
   const double* fext = S->fext.p;
   const uint8_t* fixed = S->fixed.p;
   sk_step_info inf{};
   inf.status = SK_OK;
 
-  be_init_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, x_curr, x_prev, x_bc, fixed, P.h, S->x.p, S->xt.p);
-  SK_HIP_CHECK(hipGetLastError());
   SK_TRY(launch_elem_grad(m, P.material, S->x.p, nullptr, S->mu.p, lam, S->vol.p, st));
   SK_TRY(launch_gather_grad(m, S->g.p, S->mass.p, S->x.p, S->xt.p, fext, fixed, inv_h2, st));
   SK_TRY(read_scalars(m, m->red.result.p, 1, st));
@@ -815,6 +844,22 @@ int sk_sim_step(sk_sim* S, const double* x_curr, const double* x_prev, const dou
   double gn = g0;
   inf.g0_norm = g0;
   if (!std::isfinite(g0)) inf.status = SK_ERR_NONFINITE;
+  if (x_guess && inf.status == SK_OK) {
+    guess_dx_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, x_guess, S->x.p, fixed, S->dx.p);
+    SK_TRY(launch_elem_energy(m, P.material, S->x.p, nullptr, S->dx.p, 1.0, S->mu.p, lam, S->vol.p, nullptr,
+                              S->mass.p, S->xt.p, fext, fixed, inv_h2, st));
+    SK_TRY(read_scalars(m, m->red.result.p, 1, st));
+    inf.guess_dE = m->pinned[0];
+    if (inf.guess_dE < 0.0) {
+      inf.guess_used = 1;
+      axpy_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, 1.0, S->dx.p, S->x.p);
+      SK_TRY(launch_elem_grad(m, P.material, S->x.p, nullptr, S->mu.p, lam, S->vol.p, st));
+      SK_TRY(launch_gather_grad(m, S->g.p, S->mass.p, S->x.p, S->xt.p, fext, fixed, inv_h2, st));
+      SK_TRY(read_scalars(m, m->red.result.p, 1, st));
+      gn = std::sqrt(m->pinned[0]);
+      if (!std::isfinite(gn)) inf.status = SK_ERR_NONFINITE;
+    }
+  }
 
   bool converged = false;
   while (inf.status == SK_OK) {
@@ -886,6 +931,41 @@ int sk_sim_step(sk_sim* S, const double* x_curr, const double* x_prev, const dou
   SK_HIP_CHECK(hipMemcpyAsync(x_next, S->x.p, n3 * sizeof(double), hipMemcpyDeviceToDevice, st));
   if (info) *info = inf;
   return SK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sk_sim_step(sk_sim* S, const double* x_curr, const double* x_prev, const double* x_bc, double* x_next,
+                sk_step_info* info, sk_stream_t s) {
+  SK_REQUIRE(S && x_curr && x_prev && x_next, "sk_sim_step: null argument");
+  hipStream_t st = as_stream(s);
+  const int64_t n3 = 3 * S->mesh->nv;
+  be_init_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, x_curr, x_prev, x_bc, S->fixed.p, S->prm.h, S->x.p, S->xt.p);
+  SK_HIP_CHECK(hipGetLastError());
+  return step_core(S, 1.0, nullptr, x_next, info, st);
+}
+
+int sk_sim_step_guess(sk_sim* S, const double* x_curr, const double* x_prev, const double* x_bc,
+                      const double* x_guess, double* x_next, sk_step_info* info, sk_stream_t s) {
+  SK_REQUIRE(S && x_curr && x_prev && x_next, "sk_sim_step_guess: null argument");
+  hipStream_t st = as_stream(s);
+  const int64_t n3 = 3 * S->mesh->nv;
+  be_init_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, x_curr, x_prev, x_bc, S->fixed.p, S->prm.h, S->x.p, S->xt.p);
+  SK_HIP_CHECK(hipGetLastError());
+  return step_core(S, 1.0, x_guess, x_next, info, st);
+}
+
+int sk_sim_step_bdf2(sk_sim* S, const double* x_curr, const double* x_prev, const double* x_prev2,
+                     const double* x_prev3, const double* x_bc, double* x_next, sk_step_info* info, sk_stream_t s) {
+  SK_REQUIRE(S && x_curr && x_prev && x_prev2 && x_prev3 && x_next, "sk_sim_step_bdf2: null argument");
+  hipStream_t st = as_stream(s);
+  const int64_t n3 = 3 * S->mesh->nv;
+  bdf2_init_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, x_curr, x_prev, x_prev2, x_prev3, x_bc, S->fixed.p,
+                                                     S->prm.h, S->x.p, S->xt.p);
+  SK_HIP_CHECK(hipGetLastError());
+  return step_core(S, 9.0 / 4.0, nullptr, x_next, info, st);
 }
 
 int sk_sim_spmv_reps(sk_sim* S, int reps, sk_stream_t s) {

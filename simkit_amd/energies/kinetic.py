@@ -1,9 +1,10 @@
-"""Backward-Euler kinetic (inertial) terms on the GPU (simkit/energies/kinetic.py:41-194, :291-345).
+"""Kinetic (inertial) terms on the GPU (simkit/energies/kinetic.py).
 
-``0.5 c / h^2 (x - x~)^T M (x - x~)`` with x~ = x_curr + h (x_curr - x_prev)/h
-and c = 1. M may be any scipy sparse matrix (the lumped diagonal mass in
-practice); the product M d runs as a CSR SpMV. The BDF2 variants are not on
-this path.
+``0.5 c / h^2 (x - x~)^T M (x - x~)``: backward Euler with
+x~ = x_curr + h (x_curr - x_prev)/h and c = 1 (:41-194, :291-345), BDF2 with
+x~ = (4/3) x_curr - (1/3) x_prev + (8h/9) v_curr - (2h/9) v_prev and c = 9/4
+(:63-101, :200-279). M may be any scipy sparse matrix (the lumped diagonal
+mass in practice); the product M d runs as a CSR SpMV.
 """
 
 import ctypes
@@ -15,6 +16,7 @@ from .. import _lib
 from .._device import I32, dev, empty, host, ptr, stream
 
 _BE_COEFF = 1.0
+_BDF2_COEFF = 9.0 / 4.0
 
 
 def _csr(M):
@@ -88,3 +90,45 @@ def kinetic_energy_be_u(u, x_curr, x_prev, M, h, x_bar):
 
 def kinetic_gradient_be_u(u, x_curr, x_prev, M, h, x_bar):
     return kinetic_gradient_be(x_bar + u, x_curr, x_prev, M, h)
+
+
+def _run_bdf2(shape, x, x_curr, x_prev, x_prev2, x_prev3, M=None, h=1.0, energy=False, grad=False, target=False,
+              velocity=False):
+    n = int(np.prod(shape))
+    arrs = [None if a is None else dev(np.asarray(a, dtype=np.float64).reshape(-1))
+            for a in (x, x_curr, x_prev, x_prev2, x_prev3)]
+    ip = ix = vals = None
+    if M is not None:
+        ip, ix, vals = _csr(M)
+    e = ctypes.c_double()
+    g = empty(n) if grad else None
+    xt = empty(n) if target else None
+    v = empty(n) if velocity else None
+    _lib.call("sk_kinetic_bdf2", n, *[ptr(a) for a in arrs], float(h), ptr(ip), ptr(ix), ptr(vals),
+              ctypes.byref(e) if energy else None, ptr(g), ptr(xt), ptr(v), stream())
+    return e.value, g, xt, v
+
+
+def velocity_bdf2(x_curr, x_prev, x_prev2, h):
+    """(3 x_curr - 4 x_prev + x_prev2) / (2 h)."""
+    shape = np.shape(x_curr)
+    return host(_run_bdf2(shape, None, x_curr, x_prev, x_prev2, x_prev2, h=h, velocity=True)[3]).reshape(shape)
+
+
+def bdf2_target(x_curr, x_prev, x_prev2, x_prev3, h):
+    """(4/3) x_curr - (1/3) x_prev + (8h/9) v_curr - (2h/9) v_prev."""
+    shape = np.shape(x_curr)
+    return host(_run_bdf2(shape, None, x_curr, x_prev, x_prev2, x_prev3, h=h, target=True)[2]).reshape(shape)
+
+
+def kinetic_energy_bdf2(x, x_curr, x_prev, x_prev2, x_prev3, M, h):
+    return _run_bdf2(np.shape(x), x, x_curr, x_prev, x_prev2, x_prev3, M=M, h=h, energy=True)[0]
+
+
+def kinetic_gradient_bdf2(x, x_curr, x_prev, x_prev2, x_prev3, M, h):
+    shape = np.shape(x)
+    return host(_run_bdf2(shape, x, x_curr, x_prev, x_prev2, x_prev3, M=M, h=h, grad=True)[1]).reshape(shape)
+
+
+def kinetic_hessian_bdf2(M, h):
+    return kinetic_hessian(M, h, _BDF2_COEFF)

@@ -41,9 +41,11 @@ class BackwardEulerSim:
         self.T = T
         self.nv, self.nt = X.shape[0], T.shape[0]
         self.h = float(h)
+        if material not in _lib.MATERIALS:
+            raise ValueError("Unknown material type: " + str(material))
         self.material = _lib.MATERIALS[material]
-        if self.material == _lib.SK_SNH and lam is None:
-            raise ValueError("stable neo-Hookean needs lam")
+        if self.material != _lib.SK_ARAP and lam is None:
+            raise ValueError(f"material {material!r} needs lam")
         self.mesh, vol = mesh_for_rest(X, T)
         self.vol = vol
         m_v = self.mesh.lumped_mass(vol, rho)
@@ -94,19 +96,43 @@ class BackwardEulerSim:
                   ctypes.byref(info), stream())
         return x_next, info.as_dict()
 
-    def rollout(self, steps, bc=None, x0=None, keep=None):
+    def step_guess(self, x_curr, x_prev, x_guess, x_bc=None, out=None):
+        """BE step that starts Newton from x_guess when its incremental potential is lower than at
+        x~ (info["guess_used"]), with the stopping test still relative to ||g(x~)||."""
+        info = _lib.SkStepInfo()
+        x_next = torch.empty_like(x_curr) if out is None else out
+        _lib.call("sk_sim_step_guess", self._h, ptr(x_curr), ptr(x_prev), ptr(x_bc), ptr(x_guess), ptr(x_next),
+                  ctypes.byref(info), stream())
+        return x_next, info.as_dict()
+
+    def step_bdf2(self, x_curr, x_prev, x_prev2, x_prev3, x_bc=None, out=None):
+        """One BDF2 step (c = 9/4, BDF2 target) on device tensors."""
+        info = _lib.SkStepInfo()
+        x_next = torch.empty_like(x_curr) if out is None else out
+        _lib.call("sk_sim_step_bdf2", self._h, ptr(x_curr), ptr(x_prev), ptr(x_prev2), ptr(x_prev3), ptr(x_bc),
+                  ptr(x_next), ctypes.byref(info), stream())
+        return x_next, info.as_dict()
+
+    def rollout(self, steps, bc=None, x0=None, keep=None, integrator="be"):
         """`steps` steps from rest (x_prev = x_curr = X, or x0). bc(n, t) -> device x_bc for step n
-        (t = (n+1) h) or None. Returns (kept states as host arrays {step: x}, infos)."""
-        x_prev = (self.X_dev if x0 is None else dev(x0)).clone()
-        x_curr = x_prev.clone()
+        (t = (n+1) h) or None. integrator "be" or "bdf2" (BDF2 with the history levels before
+        the start taken equal to the start state, i.e. from rest). Returns (kept states as host
+        arrays {step: x}, infos)."""
+        if integrator not in ("be", "bdf2"):
+            raise ValueError("integrator must be 'be' or 'bdf2'")
+        x0d = (self.X_dev if x0 is None else dev(x0)).clone()
+        hist = [x0d, x0d.clone(), x0d.clone(), x0d.clone()]  # x_curr, x_prev, x_prev2, x_prev3
         kept, infos = {}, []
         for n in range(steps):
             x_bc = bc(n, (n + 1) * self.h) if bc is not None else None
-            x_next, info = self.step(x_curr, x_prev, x_bc)
+            if integrator == "be":
+                x_next, info = self.step(hist[0], hist[1], x_bc)
+            else:
+                x_next, info = self.step_bdf2(hist[0], hist[1], hist[2], hist[3], x_bc)
             infos.append(info)
             if keep is None or (n + 1) in keep:
                 kept[n + 1] = x_next.cpu().numpy()
-            x_prev, x_curr = x_curr, x_next
+            hist = [x_next] + hist[:3]
         return kept, infos
 
 

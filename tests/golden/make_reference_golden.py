@@ -8,14 +8,18 @@ Covers only what closed forms, invariants and finite differences cannot pin:
 3. ``psd_project`` outputs on projected SNH Hessians;
 4. the assembled CSR of ``stable_neo_hookean_hessian_x`` / ``arap_hessian_x``
    on a small grid (which entries survive scipy's exact-zero dropping);
-5. a 20-step backward-Euler trajectory on the 10^3-cell cube (1331 vertices,
+5. the hand-written element tables of the Macklin-Mueller neo-Hookean and
+   fixed-corotational energies and the linear-elasticity element tier
+   (``ref_materials.npz``);
+6. a 20-step backward-Euler trajectory on the 10^3-cell cube (1331 vertices,
    5000 tets; E=1e6, nu=0.45, rho=1000, gravity -9.8 in y, y=0 vertices
    eliminated) produced by the reference's own ``backward_euler`` /
    ``newton_solver`` / ``backtracking_line_search`` (spsolve, step-norm stop at
    1e-10).
 
 Inputs are generated here with a fixed seed and stored next to the outputs.
-Usage: ``python tests/golden/make_reference_golden.py`` (needs /root/reference).
+Usage: ``python tests/golden/make_reference_golden.py [element assembly
+materials trajectory]`` (needs /root/reference; no argument writes all).
 """
 
 import os
@@ -40,6 +44,9 @@ sys.modules["simkit.energies"] = _stub
 
 from simkit.energies import stable_neo_hookean as snh  # noqa: E402
 from simkit.energies import arap  # noqa: E402
+from simkit.energies import fcr  # noqa: E402
+from simkit.energies import linear_elasticity as le  # noqa: E402
+from simkit.energies import macklin_mueller_neo_hookean as mmnh  # noqa: E402
 from simkit.psd_project import psd_project  # noqa: E402
 from simkit.rotation_gradient import rotation_gradient_F  # noqa: E402
 from simkit.deformation_jacobian import deformation_jacobian  # noqa: E402
@@ -75,6 +82,26 @@ def element_fixtures():
         arap_P=arap.arap_gradient_element_F(F, mu),
         arap_H=arap.arap_hessian_element_F(F, mu),
         dRdF=rotation_gradient_F(F),
+    )
+
+
+def material_fixtures():
+    rng = np.random.default_rng(20240502)
+    F = random_F(rng, 64, inverted=12)
+    mu = rng.uniform(1e4, 1e6, (64, 1))
+    lam = mu * rng.uniform(1.0, 20.0, (64, 1))
+    np.savez_compressed(
+        os.path.join(OUT, "ref_materials.npz"),
+        F=F, mu=mu, lam=lam,
+        mmnh_psi=mmnh.macklin_mueller_neo_hookean_energy_element_F(F, mu, lam),
+        mmnh_P=mmnh.macklin_mueller_neo_hookean_gradient_element_F(F, mu, lam),
+        mmnh_H=mmnh.macklin_mueller_neo_hookean_hessian_element_F(F, mu, lam),
+        fcr_psi=fcr.fcr_energy_element_F(F, mu, lam),
+        fcr_P=fcr.fcr_gradient_element_F(F, mu, lam),
+        fcr_H=fcr.fcr_hessian_element_F(F, mu, lam),
+        linear_psi=le.linear_elasticity_energy_element_F(F, mu, lam),
+        linear_P=le.linear_elasticity_gradient_element_F(F, mu, lam),
+        linear_H=le.linear_elasticity_hessian_element_F(F, mu, lam),
     )
 
 
@@ -155,7 +182,8 @@ def trajectory_fixture(steps=20, keep=(1, 2, 5, 10, 20)):
 
 
 if __name__ == "__main__":
-    element_fixtures()
-    assembly_fixtures()
-    trajectory_fixture()
+    jobs = {"element": element_fixtures, "assembly": assembly_fixtures, "materials": material_fixtures,
+            "trajectory": trajectory_fixture}
+    for name in sys.argv[1:] or list(jobs):
+        jobs[name]()
     print("wrote", sorted(f for f in os.listdir(OUT) if f.startswith("ref_")))

@@ -93,11 +93,26 @@ def test_x_tier_matches_reference_golden(ref_assembly, mat):
             assert np.array_equal(Q.indptr, ref.indptr) and np.array_equal(Q.indices, ref.indices)
 
 
-@pytest.mark.parametrize("mat", ["snh", "arap"])
+_DISPATCH = {"arap": "arap", "linear": "linear-elasticity", "mmnh": "macklin-mueller-neo-hookean", "fcr": "fcr"}
+
+
+def x_tier(mat):
+    """(energy_x, gradient_x, hessian_x) of the drop-in API for an oracle material key."""
+    if mat == "snh":
+        from simkit_amd.energies import stable_neo_hookean as snh
+        return (snh.stable_neo_hookean_energy_x, snh.stable_neo_hookean_gradient_x,
+                snh.stable_neo_hookean_hessian_x)
+    from simkit_amd.energies import elastic
+    name = _DISPATCH[mat]
+    return (lambda x, J, mu, lam, vol: elastic.elastic_energy_x(x, J, mu, lam, vol, name),
+            lambda x, J, mu, lam, vol: elastic.elastic_gradient_x(x, J, mu, lam, vol, name),
+            lambda x, J, mu, lam, vol, psd=True: elastic.elastic_hessian_x(x, J, mu, lam, vol, name, psd=psd))
+
+
+@pytest.mark.parametrize("mat", ["snh", "arap", "linear", "mmnh", "fcr"])
 @pytest.mark.parametrize("jit", [0.0, 0.2])
 def test_x_tier_vs_oracle_cube10(mat, jit):
     import simkit_amd as sk
-    from simkit_amd.energies import arap, stable_neo_hookean as snh
     X, T, x, mu, lam = problem(10, seed=3, jitter=jit, deform=0.3)
     J = sk.deformation_jacobian(X, T)
     vol = sk.volume(X, T)
@@ -105,22 +120,17 @@ def test_x_tier_vs_oracle_cube10(mat, jit):
     Jo = fem.deformation_jacobian(D, T, X.shape[0])
     volo = fem.tet_volumes(X, T)
     xf = x.reshape(-1)
+    energy_x, gradient_x, hessian_x = x_tier(mat)
     for psd in (True, False):
-        if mat == "snh":
-            Q = snh.stable_neo_hookean_hessian_x(x, J, mu, lam, vol, psd=psd)
-        else:
-            Q = arap.arap_hessian_x(x, J, mu, vol, psd=psd)
-        Qo = fem.hessian_x(mat, xf, Jo, mu, lam, volo, psd=psd)
+        Q = hessian_x(x, J, mu, lam, vol, psd=psd)
+        # linear elasticity never projects (its Hessian is constant and PSD)
+        Qo = fem.hessian_x(mat, xf, Jo, mu, lam, volo, psd=psd and mat != "linear")
         diff = (Q - Qo).tocsr()
         # row-wise relative error against each row's largest entry
         rowmax = np.maximum(abs(Qo).max(axis=1).toarray().ravel(), 1e-300)
         assert (abs(diff).max(axis=1).toarray().ravel() / rowmax).max() < TOL
-    if mat == "snh":
-        E = snh.stable_neo_hookean_energy_x(x, J, mu, lam, vol)
-        g = snh.stable_neo_hookean_gradient_x(x, J, mu, lam, vol)
-    else:
-        E = arap.arap_energy_x(x, J, mu, vol)
-        g = arap.arap_gradient_x(x, J, mu, vol)
+    E = energy_x(x, J, mu, lam, vol)
+    g = gradient_x(x, J, mu, lam, vol)
     Eo = fem.energy_x(mat, xf, Jo, mu, lam, volo)
     assert abs(E - Eo) <= TOL * abs(Eo)
     assert rel_err(g, fem.gradient_x(mat, xf, Jo, mu, lam, volo)) < TOL

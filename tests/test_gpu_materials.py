@@ -1,0 +1,213 @@
+"""GPU parity for the linear-elasticity, Macklin-Mueller neo-Hookean and
+fixed-corotational densities (element tier against the reference's golden
+tables and the oracle at 1e-10, FP64), the material dispatcher, BDF2
+(kinetic terms, closure integrator and the device-resident simulator against
+the oracle at 1e-6 per step) and the warm-start step entry point.
+"""
+
+import numpy as np
+import pytest
+import scipy.sparse as sps
+
+from conftest import golden, per_item_rel_err, rel_err
+from oracle import fem, solver
+from oracle.mesh import unit_cube
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+NAMES = {"linear": "linear-elasticity", "mmnh": "macklin-mueller-neo-hookean", "fcr": "fcr"}
+
+
+def random_F(seed, n=256, scale=0.3, inverted=0):
+    rng = np.random.default_rng(seed)
+    F = np.eye(3)[None] + scale * rng.standard_normal((n, 3, 3))
+    F[:inverted, :, 0] *= -1.0
+    return F
+
+
+@pytest.mark.parametrize("mat", ["linear", "mmnh", "fcr"])
+def test_element_tier_vs_golden_and_oracle(mat):
+    from simkit_amd.energies import elastic
+    d = golden("ref_materials.npz")
+    name = NAMES[mat]
+    F, mu, lam = d["F"], d["mu"], d["lam"]
+    assert per_item_rel_err(elastic.elastic_energy_element_F(F, mu, lam, name), d[f"{mat}_psi"]) < TOL
+    assert per_item_rel_err(elastic.elastic_gradient_element_F(F, mu, lam, name), d[f"{mat}_P"]) < TOL
+    assert per_item_rel_err(elastic.elastic_hessian_element_F(F, mu, lam, name, psd=False), d[f"{mat}_H"]) < TOL
+    e, g, h = fem._ELEMENT[mat]
+    F2 = random_F(31, 1000, 0.5, inverted=100)
+    assert per_item_rel_err(elastic.elastic_hessian_element_F(F2, 3e5, 2e6, name, psd=False), h(F2, 3e5, 2e6)) < TOL
+    assert per_item_rel_err(elastic.elastic_gradient_element_F(F2, 3e5, 2e6, name), g(F2, 3e5, 2e6)) < TOL
+    assert per_item_rel_err(elastic.elastic_energy_element_F(F2, 3e5, 2e6, name), e(F2, 3e5, 2e6)) < TOL
+    # psd=True projects the element Hessian (elastic.py:665-667)
+    assert per_item_rel_err(elastic.elastic_hessian_element_F(F2, 3e5, 2e6, name),
+                            fem.psd_project(h(F2, 3e5, 2e6))) < TOL
+
+
+def test_dispatcher_module_names_and_errors():
+    from simkit_amd.energies import elastic, fcr, linear_elasticity, macklin_mueller_neo_hookean as mm
+    F = random_F(32, 8)
+    assert np.array_equal(elastic.elastic_energy_element_F(F, 1.0, 2.0, "fcr"), fcr.fcr_energy_element_F(F, 1.0, 2.0))
+    assert np.array_equal(mm.macklin_mueller_neo_hookean_gradient_element_F(F, 1.0, 2.0),
+                          elastic.elastic_gradient_element_F(F, 1.0, 2.0, "macklin-mueller-neo-hookean"))
+    assert np.array_equal(linear_elasticity.linear_elasticity_hessian_element_F(F, 1.0, 2.0),
+                          elastic.elastic_hessian_element_F(F, 1.0, 2.0, "linear-elasticity", psd=False))
+    with pytest.raises(ValueError, match="Unknown material type: neo"):
+        elastic.elastic_energy_element_F(F, 1.0, 2.0, "neo")
+
+
+def test_self_contained_and_u_tiers():
+    from simkit_amd.energies import fcr, macklin_mueller_neo_hookean as mm
+    import simkit_amd as sk
+    X, T = unit_cube(3)
+    rng = np.random.default_rng(33)
+    U = X + 0.05 * rng.standard_normal(X.shape)
+    J = sk.deformation_jacobian(X, T)
+    vol = sk.volume(X, T)
+    assert rel_err(fcr.fcr_gradient(X, T, 2.0, 5.0, U=U), fcr.fcr_gradient_x(U, J, 2.0, 5.0, vol)) < 1e-14
+    Jx = J @ X.reshape(-1, 1)
+    E_u = mm.macklin_mueller_neo_hookean_energy_u(U - X, J, Jx, 2.0, 5.0, vol)
+    assert abs(E_u - mm.macklin_mueller_neo_hookean_energy_x(U, J, 2.0, 5.0, vol)) < 1e-12 * abs(E_u)
+
+
+def test_bdf2_kinetic_terms_vs_closed_form():
+    from simkit_amd.energies import kinetic as kin
+    rng = np.random.default_rng(34)
+    n = 3000
+    x, xc, xp, xp2, xp3 = (rng.standard_normal((n, 1)) for _ in range(5))
+    m = rng.uniform(0.5, 2.0, n)
+    M = sps.diags(m)
+    h = 1 / 60
+    xt = solver.bdf2_target(xc, xp, xp2, xp3, h)
+    d = x - xt
+    assert rel_err(kin.bdf2_target(xc, xp, xp2, xp3, h), xt) < 1e-15
+    assert rel_err(kin.velocity_bdf2(xc, xp, xp2, h), (3 * xc - 4 * xp + xp2) / (2 * h)) < 1e-15
+    e_ref = 0.5 * 2.25 * (d.ravel() @ (m * d.ravel())) / h ** 2
+    assert abs(kin.kinetic_energy_bdf2(x, xc, xp, xp2, xp3, M, h) - e_ref) < 1e-10 * abs(e_ref)
+    assert rel_err(kin.kinetic_gradient_bdf2(x, xc, xp, xp2, xp3, M, h), 2.25 * (m[:, None] * d) / h ** 2) < 1e-14
+    assert rel_err(kin.kinetic_hessian_bdf2(M, h).diagonal(), 2.25 * m / h ** 2) < 1e-15
+
+
+def test_bdf2_closure_integrator_matches_oracle():
+    import simkit_amd as sk
+    from simkit_amd.energies import macklin_mueller_neo_hookean as mm
+    from simkit_amd.integrators import bdf2
+    X, T = unit_cube(3)
+    mu, lam = fem.ympr_to_lame(1e6, 0.45)
+    J = sk.deformation_jacobian(X, T)
+    vol = sk.volume(X, T)
+    pinned = np.where(X[:, 1] == 0)[0]
+    fixed = np.zeros(X.size, bool)
+    for c in range(3):
+        fixed[3 * pinned + c] = True
+    free = ~fixed
+    xfix = X.reshape(-1)[fixed]
+    f_ext = sk.gravity_force(X, T, -9.8, 1000.0).reshape(-1)
+    m = np.repeat(sk.massmatrix(X, T, 1000.0).diagonal(), 3)
+
+    def expand(z):
+        x = np.empty(X.size)
+        x[free] = z.ravel()
+        x[fixed] = xfix
+        return x.reshape(-1, 3)
+
+    E = lambda z: mm.macklin_mueller_neo_hookean_energy_x(expand(z), J, mu, lam, vol) - f_ext @ expand(z).ravel()
+    G = lambda z: (mm.macklin_mueller_neo_hookean_gradient_x(expand(z), J, mu, lam, vol).ravel() - f_ext)[free][:, None]
+    H = lambda z: mm.macklin_mueller_neo_hookean_hessian_x(expand(z), J, mu, lam, vol)[free][:, free]
+    p = solver.BEProblem(X, T, mu, lam, 1000.0, 1 / 60, pinned, material="mmnh", gravity=-9.8, newton_rtol=1e-12,
+                         integrator="bdf2")
+    ref = p.rollout(3)
+    z = [X.reshape(-1)[free][:, None]] * 4
+    for s in range(3):
+        zn = bdf2(z[0], z[1], z[2], z[3], E, G, H, sps.diags(m[free]), 1 / 60, tolerance=1e-10, max_iter=50)
+        z = [zn] + z[:3]
+        assert np.abs(expand(zn).ravel() - ref[s]).max() <= 1e-6 * np.abs(ref[s]).max()
+
+
+@pytest.mark.parametrize("mat", ["mmnh", "fcr"])
+def test_sim_bdf2_matches_oracle(mat):
+    from simkit_amd.sim import BackwardEulerSim
+    X, T = unit_cube(6)
+    mu, lam = fem.ympr_to_lame(1e6, 0.45)
+    pinned = np.where(X[:, 1] == 0)[0]
+    steps = 8
+    ref = solver.BEProblem(X, T, mu, lam, 1000.0, 1 / 60, pinned, material=mat, gravity=-9.8,
+                           integrator="bdf2").rollout(steps)
+    sim = BackwardEulerSim(X, T, mu, lam, rho=1000.0, h=1 / 60, material=mat, fixed_vertices=pinned, gravity=-9.8,
+                           pcg_rtol=1e-10)
+    kept, infos = sim.rollout(steps, integrator="bdf2")
+    assert all(i["status"] == 0 for i in infos), infos
+    for s in range(1, steps + 1):
+        err = np.abs(kept[s] - ref[s - 1]).max() / np.abs(ref[s - 1]).max()
+        assert err < 1e-6, (s, err)
+
+
+@pytest.mark.parametrize("mat", ["linear", "mmnh", "fcr"])
+def test_sim_be_new_materials_match_oracle(mat):
+    from simkit_amd.sim import BackwardEulerSim
+    X, T = unit_cube(6)
+    mu, lam = fem.ympr_to_lame(1e6, 0.45)
+    pinned = np.where(X[:, 1] == 0)[0]
+    steps = 6
+    psd = mat != "linear"
+    p = solver.BEProblem(X, T, mu, lam, 1000.0, 1 / 60, pinned, material=mat, gravity=-9.8)
+    if not psd:
+        p.potential_hessian = lambda x: (fem.hessian_x("linear", x, p.J, p.mu, p.lam, p.vol, psd=False)
+                                         + sps.diags(p.m / p.h ** 2)).tocsr()
+    ref = p.rollout(steps)
+    sim = BackwardEulerSim(X, T, mu, lam, rho=1000.0, h=1 / 60, material=mat, fixed_vertices=pinned, gravity=-9.8,
+                           pcg_rtol=1e-10, psd=psd)
+    kept, infos = sim.rollout(steps)
+    assert all(i["status"] == 0 for i in infos), infos
+    for s in range(1, steps + 1):
+        assert np.abs(kept[s] - ref[s - 1]).max() / np.abs(ref[s - 1]).max() < 1e-6
+
+
+@pytest.mark.parametrize("mat", ["mmnh", "fcr"])
+def test_mixed_precision_new_materials(mat):
+    """FP32 element Hessians (FP64 polar SVD for det F < 1e-3 in FCR) with FP64 refinement reach the
+    same per-step positions as the FP64 oracle on a jittered 10^3 cube with the prescribed top face."""
+    from simkit_amd.mesh import jitter
+    from simkit_amd.sim import BackwardEulerSim, top_face_sinusoid
+    n = 10
+    X, T = unit_cube(n)
+    bottom, top = np.where(X[:, 1] == 0.0)[0], np.where(X[:, 1] == 1.0)[0]
+    X = jitter(X, 1.0 / n, 0.2, seed=1, keep=np.concatenate([bottom, top]))
+    mu, lam = fem.ympr_to_lame(1e6, 0.45)
+    P = lambda t: X + np.outer(np.isin(np.arange(X.shape[0]), top), [0.1 * np.sin(2 * np.pi * t), 0, 0])
+    steps = 5
+    ref = solver.BEProblem(X, T, mu, lam, 1000.0, 1 / 60, np.concatenate([bottom, top]), material=mat,
+                           prescribe=P).rollout(steps)
+    sim = BackwardEulerSim(X, T, mu, lam, rho=1000.0, h=1 / 60, material=mat,
+                           fixed_vertices=np.concatenate([bottom, top]), precision="mixed", pcg_rtol=1e-8)
+    kept, infos = sim.rollout(steps, bc=top_face_sinusoid(X, top, 0.1, 1.0))
+    assert all(i["status"] == 0 for i in infos), infos
+    for s in range(1, steps + 1):
+        assert np.abs(kept[s] - ref[s - 1]).max() / np.abs(ref[s - 1]).max() < 1e-6
+
+
+def test_step_guess_accepts_good_and_rejects_bad_guesses():
+    from simkit_amd.sim import BackwardEulerSim
+    X, T = unit_cube(8)
+    mu, lam = fem.ympr_to_lame(1e6, 0.45)
+    pinned = np.where(X[:, 1] == 0)[0]
+    sim = BackwardEulerSim(X, T, mu, lam, rho=1000.0, h=1 / 60, fixed_vertices=pinned, gravity=-9.8, pcg_rtol=1e-10)
+    x0 = sim.X_dev.clone()
+    x1, i1 = sim.step(x0, x0)
+    x2, i2 = sim.step(x1, x0)
+    # a guess at the converged answer is taken and needs at most one Newton iteration
+    xg, ig = sim.step_guess(x1, x0, x2.clone())
+    assert ig["guess_used"] == 1 and ig["guess_dE"] < 0 and ig["newton_iters"] <= 1
+    assert ig["g0_norm"] == i2["g0_norm"]                     # tolerance anchored at the plain start x~
+    assert np.abs((xg - x2).cpu().numpy()).max() <= 1e-6 * np.abs(x2.cpu().numpy()).max()
+    # a worse guess is rejected and the step is exactly the plain one
+    bad = x2 + 0.05 * torch_randn_like(x2, 35)
+    xb, ib = sim.step_guess(x1, x0, bad)
+    assert ib["guess_used"] == 0 and ib["guess_dE"] > 0
+    assert np.array_equal(xb.cpu().numpy(), x2.cpu().numpy()) and ib["newton_iters"] == i2["newton_iters"]
+
+
+def torch_randn_like(x, seed):
+    import torch
+    g = np.random.default_rng(seed).standard_normal(x.shape[0])
+    return torch.as_tensor(g, dtype=x.dtype, device=x.device)
