@@ -39,7 +39,8 @@ enum sk_status {
   SK_ERR_NONFINITE = -6,     /* NaN/Inf in a reduction */
   SK_ERR_INVERTED = -7,      /* reserved: inverted element count reported */
   SK_ERR_NEWTON_MAXIT = -8,  /* Newton hit its iteration limit before the gradient tolerance */
-  SK_ERR_UNSUPPORTED = -9
+  SK_ERR_UNSUPPORTED = -9,
+  SK_ERR_EIG_MAXIT = -10
 };
 
 /* Element densities (energies/ in the reference): stable_neo_hookean.py,
@@ -262,6 +263,36 @@ int sk_sim_step_guess(sk_sim* sim, const double* x_curr, const double* x_prev, c
 int sk_sim_step_bdf2(sk_sim* sim, const double* x_curr, const double* x_prev, const double* x_prev2,
                      const double* x_prev3, const double* x_bc, double* x_next, sk_step_info* info,
                      sk_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* Linear modal analysis                                                     */
+/* ------------------------------------------------------------------------ */
+typedef struct sk_modal_info {
+  int sweeps;                /* filter + Rayleigh-Ritz sweeps */
+  int64_t spmvs;             /* block sparse products */
+  int block;                 /* subspace block size (64 or 128) */
+  double max_rel_residual;   /* max_j ||A y_j - theta_j y_j|| / max(|theta_j|, 1e-6 spectrum_bound), j < k */
+  double spectrum_bound;     /* Gershgorin bound of D (K + shift M) D */
+  double ms;                 /* wall time of the call */
+} sk_modal_info;
+
+/* Lowest k (1..96) eigenpairs of (K + shift M) phi = lambda M phi over the
+ * free DOFs: K the assembled element Hessian of `material` at positions x
+ * (3nv, device; no PSD projection), M = diag(mass) (3nv per-DOF lumped mass,
+ * device), fixed (3nv 0/1, device, may be NULL) DOFs eliminated and returned
+ * as zero rows. basis (device, 3nv x k, row-major: k coefficients per DOF)
+ * receives M-orthonormal eigenvectors, eigenvalues (host, k) ascending.
+ * Chebyshev-filtered subspace iteration (degree `degree` per sweep, at most
+ * max_sweeps) until every returned pair has relative residual <= tol
+ * (against max(|lambda|, 1e-6 spectrum_bound), so zero modes converge), from a
+ * block seeded by `seed` (deterministic). Returns SK_ERR_EIG_MAXIT (outputs
+ * still written) when max_sweeps is reached first. Replaces
+ * linear_modal_analysis.py:18-73 (material = SK_LINEAR, mu = lam = 1,
+ * shift 1e-6 with bI given) and its eigs.py:66-110 shift-invert solve. */
+int sk_modal_basis(sk_mesh* mesh, int material, const double* x, const double* mu, const double* lam,
+                   const double* vol, const double* mass, const uint8_t* fixed, int k, double shift, double tol,
+                   int degree, int max_sweeps, uint64_t seed, double* basis, double* eigenvalues,
+                   sk_modal_info* info, sk_stream_t stream);
 
 /* Measurement hook, no counterpart in the reference: re-initialises PCG
  * scratch vectors on the system assembled by the last sk_sim_step (RHS = the

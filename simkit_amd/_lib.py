@@ -21,6 +21,7 @@ SK_ERR_NONFINITE = -6
 SK_ERR_INVERTED = -7
 SK_ERR_NEWTON_MAXIT = -8
 SK_ERR_UNSUPPORTED = -9
+SK_ERR_EIG_MAXIT = -10
 
 SK_SNH = 0
 SK_ARAP = 1
@@ -93,6 +94,20 @@ class SkStepInfo(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class SkModalInfo(ctypes.Structure):
+    _fields_ = [
+        ("sweeps", ctypes.c_int),
+        ("spmvs", ctypes.c_int64),
+        ("block", ctypes.c_int),
+        ("max_rel_residual", ctypes.c_double),
+        ("spectrum_bound", ctypes.c_double),
+        ("ms", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 _P = ctypes.c_void_p
 _I64 = ctypes.c_int64
 _D = ctypes.c_double
@@ -125,6 +140,8 @@ SIGNATURES = {
     "sk_sim_create": [_P, ctypes.POINTER(SkSimParams), _P, _P, _P, _P, _P, _P, ctypes.POINTER(_P)],
     "sk_sim_destroy": [_P],
     "sk_sim_step": [_P, _P, _P, _P, _P, ctypes.POINTER(SkStepInfo), _P],
+    "sk_modal_basis": [_P, _INT, _P, _P, _P, _P, _P, _P, _INT, _D, _D, _INT, _INT, ctypes.c_uint64, _P, _P,
+                       ctypes.POINTER(SkModalInfo), _P],
     "sk_sim_step_guess": [_P, _P, _P, _P, _P, _P, ctypes.POINTER(SkStepInfo), _P],
     "sk_sim_step_bdf2": [_P, _P, _P, _P, _P, _P, _P, ctypes.POINTER(SkStepInfo), _P],
     "sk_sim_spmv_reps": [_P, _INT, _P],

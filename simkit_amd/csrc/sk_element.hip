@@ -180,6 +180,7 @@ const char* sk_status_string(int status) {
     case SK_ERR_NONFINITE: return "non-finite value";
     case SK_ERR_NEWTON_MAXIT: return "Newton reached its iteration limit";
     case SK_ERR_UNSUPPORTED: return "unsupported";
+    case SK_ERR_EIG_MAXIT: return "eigen-solver sweep limit reached before tol";
     default: return "unknown status";
   }
 }
