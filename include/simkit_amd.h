@@ -294,6 +294,37 @@ int sk_modal_basis(sk_mesh* mesh, int material, const double* x, const double* m
                    int degree, int max_sweeps, uint64_t seed, double* basis, double* eigenvalues,
                    sk_modal_info* info, sk_stream_t stream);
 
+/* ------------------------------------------------------------------------ */
+/* Learned warm start: BF16 MFMA multilayer perceptron                       */
+/* No counterpart in the reference.                                          */
+/* ------------------------------------------------------------------------ */
+
+/* C (M x N, FP32, row stride ldc) = A B with BF16 operands given by element
+ * strides: A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn]; M, N, K
+ * positive multiples of 16; FP32 accumulation on the matrix cores. */
+int sk_gemm_bf16(int M, int N, int K, const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B, int64_t sbk,
+                 int64_t sbn, float* C, int64_t ldc, sk_stream_t stream);
+
+/* MLP dims[0] -> dims[1] -> ... -> dims[n_layers] with exact-erf GELU after
+ * every layer but the last; BF16 weights and activations, FP32 accumulation,
+ * FP32 master weights and Adam moments. Parameters (host, FP32) are laid out
+ * layer by layer as W_l (dims[l+1] x dims[l], row-major) then b_l. */
+typedef struct sk_mlp sk_mlp;
+int sk_mlp_create(int n_layers, const int* dims, int max_batch, sk_mlp** out);
+int sk_mlp_destroy(sk_mlp* net);
+int64_t sk_mlp_num_params(const sk_mlp* net);
+int sk_mlp_set_params(sk_mlp* net, const float* params, sk_stream_t stream);   /* also resets Adam */
+int sk_mlp_get_params(sk_mlp* net, float* params, sk_stream_t stream);
+/* Y (device, batch x dims[L], FP32) = net(X) (device, batch x dims[0], FP32). */
+int sk_mlp_forward(sk_mlp* net, const float* X, int batch, float* Y, sk_stream_t stream);
+/* Forward + backward of the mean squared error against T (device, batch x
+ * dims[L]); keeps the gradient for sk_mlp_adam / sk_mlp_get_grad; *loss_host
+ * (may be NULL: no synchronization) receives the loss. */
+int sk_mlp_backward(sk_mlp* net, const float* X, const float* T, int batch, double* loss_host, sk_stream_t stream);
+int sk_mlp_get_grad(sk_mlp* net, float* grad, sk_stream_t stream);
+/* One Adam step (bias-corrected) with the last gradient. */
+int sk_mlp_adam(sk_mlp* net, float lr, float beta1, float beta2, float eps, sk_stream_t stream);
+
 /* Measurement hook, no counterpart in the reference: re-initialises PCG
  * scratch vectors on the system assembled by the last sk_sim_step (RHS = the
  * last gradient) and enqueues `reps` block-CSR SpMV launches back to back on

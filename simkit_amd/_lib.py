@@ -142,11 +142,21 @@ SIGNATURES = {
     "sk_sim_step": [_P, _P, _P, _P, _P, ctypes.POINTER(SkStepInfo), _P],
     "sk_modal_basis": [_P, _INT, _P, _P, _P, _P, _P, _P, _INT, _D, _D, _INT, _INT, ctypes.c_uint64, _P, _P,
                        ctypes.POINTER(SkModalInfo), _P],
+    "sk_gemm_bf16": [_INT, _INT, _INT, _P, _I64, _I64, _P, _I64, _I64, _P, _I64, _P],
+    "sk_mlp_create": [_INT, _P, _INT, _P],
+    "sk_mlp_destroy": [_P],
+    "sk_mlp_num_params": [_P],
+    "sk_mlp_set_params": [_P, _P, _P],
+    "sk_mlp_get_params": [_P, _P, _P],
+    "sk_mlp_get_grad": [_P, _P, _P],
+    "sk_mlp_forward": [_P, _P, _INT, _P, _P],
+    "sk_mlp_backward": [_P, _P, _P, _INT, ctypes.POINTER(_D), _P],
+    "sk_mlp_adam": [_P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _P],
     "sk_sim_step_guess": [_P, _P, _P, _P, _P, _P, ctypes.POINTER(SkStepInfo), _P],
     "sk_sim_step_bdf2": [_P, _P, _P, _P, _P, _P, _P, ctypes.POINTER(SkStepInfo), _P],
     "sk_sim_spmv_reps": [_P, _INT, _P],
 }
-_RESTYPES = {"sk_last_error": ctypes.c_char_p, "sk_status_string": ctypes.c_char_p}
+_RESTYPES = {"sk_last_error": ctypes.c_char_p, "sk_status_string": ctypes.c_char_p, "sk_mlp_num_params": ctypes.c_int64}
 
 _lock = threading.Lock()
 _handle = None
