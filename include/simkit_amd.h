@@ -228,6 +228,7 @@ typedef struct sk_step_info {
   int refine_steps;      /* SK_PREC_MIXED: FP64 residual replacements summed over the Newton iterations */
   int guess_used;        /* sk_sim_step_guess: 1 if Newton started from the supplied guess */
   double guess_dE;       /* sk_sim_step_guess: E(x_guess) - E(x~), FP64 */
+  double guess_g_norm;   /* sk_sim_step_guess: ||g_free(x_guess)|| when the guess was taken, else 0 */
 } sk_step_info;
 
 /* mu, lam, vol: per element; mass: per DOF (3nv, M = kron(M_lumped, I3));
@@ -265,6 +266,67 @@ int sk_sim_step_bdf2(sk_sim* sim, const double* x_curr, const double* x_prev, co
                      sk_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
+/* Batched implicit steps over many independent meshes                       */
+/* No counterpart in the reference, whose backward_euler.py:27-91 advances one */
+/* mesh per call; each problem here gets exactly that step (the same Newton,  */
+/* line-search and PCG rules on its own scalars) with the mixed-precision     */
+/* linear solve.                                                             */
+/* ------------------------------------------------------------------------ */
+typedef struct sk_batch sk_batch;
+
+typedef struct sk_batch_params {
+  int material;          /* sk_material */
+  int psd;               /* sk_psd_method */
+  double h;
+  double newton_rtol;    /* per problem: stop when ||g_free|| <= newton_rtol ||g_free(x~)|| */
+  int newton_max_iter;
+  double pcg_rtol;       /* FP64 residual target of the per-problem solve */
+  int pcg_max_iter;
+  double inner_rtol;     /* FP32 residual drop between FP64 residual replacements */
+  double ls_alpha;
+  double ls_beta;
+  int ls_max_iter;
+  double ls_threshold;
+  int block;             /* threads per workgroup of the per-problem kernels (128, 256, 512, 1024); 0 = 256 */
+  int lanes;             /* lanes per matrix row in the per-problem SpMV (1 or 4); 0 = 1 */
+  int use_graph;         /* 1: one Newton iteration of the batch is captured in a hipGraph and replayed */
+} sk_batch_params;
+
+typedef struct sk_batch_info {
+  int status;
+  int newton_iters;
+  int64_t pcg_iters;
+  int ls_trials;
+  int refine_steps;
+  double g0_norm;
+  double g_norm;
+} sk_batch_info;
+
+/* mesh: the union of the problems' meshes, problem p owning vertices
+ * [vstart[p], vstart[p+1]) and tets [tstart[p], tstart[p+1]) (host arrays of
+ * nprob+1; T_host (4 nt) is the union's tet array, checked against the
+ * ranges). mu, lam, vol per union element; mass, f_ext, fixed per union DOF
+ * (device; f_ext and fixed may be NULL). */
+int sk_batch_create(sk_mesh* mesh, int nprob, const int64_t* vstart, const int64_t* tstart, const int32_t* T_host,
+                    const sk_batch_params* params, const double* mu, const double* lam, const double* vol,
+                    const double* mass, const double* f_ext, const uint8_t* fixed, sk_batch** out);
+int sk_batch_destroy(sk_batch* batch);
+/* One backward-Euler step of every problem (union DOF vectors on the device;
+ * x_bc may be NULL). info (host, nprob entries) and rounds (batch Newton
+ * iterations launched) may be NULL. */
+int sk_batch_step(sk_batch* batch, const double* x_curr, const double* x_prev, const double* x_bc, double* x_next,
+                  sk_batch_info* info, int* rounds, sk_stream_t stream);
+/* Sets the per-problem kernel shape (a recorded autotuning choice). */
+int sk_batch_set_shape(sk_batch* batch, int block, int lanes);
+/* Times the per-problem PCG kernel on the first Newton system of a step from
+ * (x_curr, x_prev, x_bc) for every (block, lanes) in {128, 256, 512, 1024} x
+ * {1, 4} (reps launches each, after one warm-up) and keeps the fastest.
+ * ms_out (host, 8 entries, may be NULL) receives the mean times in that
+ * order. The batch state is reset by the next sk_batch_step. */
+int sk_batch_autotune(sk_batch* batch, const double* x_curr, const double* x_prev, const double* x_bc, int reps,
+                      int* block_out, int* lanes_out, double* ms_out, sk_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
 /* Linear modal analysis                                                     */
 /* ------------------------------------------------------------------------ */
 typedef struct sk_modal_info {
@@ -293,6 +355,16 @@ int sk_modal_basis(sk_mesh* mesh, int material, const double* x, const double* m
                    const double* vol, const double* mass, const uint8_t* fixed, int k, double shift, double tol,
                    int degree, int max_sweeps, uint64_t seed, double* basis, double* eigenvalues,
                    sk_modal_info* info, sk_stream_t stream);
+
+/* Tall-skinny helpers for modal coordinates (no counterpart in the
+ * reference): q_j = sum_r basis[r][j] w[r] (x[r] - base[r]) for j < k
+ * (k <= 256; w, base may be NULL; q on the device), out = base + basis c
+ * (base may be NULL), and out = a x + b y. basis is rows x k row-major. */
+int sk_dense_project(int64_t rows, int k, const double* basis, const double* weight, const double* x,
+                     const double* base, double* q, sk_stream_t stream);
+int sk_dense_expand(int64_t rows, int k, const double* basis, const double* coef, const double* base, double* out,
+                    sk_stream_t stream);
+int sk_axpby(int64_t n, double a, const double* x, double b, const double* y, double* out, sk_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
 /* Learned warm start: BF16 MFMA multilayer perceptron                       */

@@ -88,6 +88,42 @@ class SkStepInfo(ctypes.Structure):
         ("refine_steps", ctypes.c_int),
         ("guess_used", ctypes.c_int),
         ("guess_dE", ctypes.c_double),
+        ("guess_g_norm", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class SkBatchParams(ctypes.Structure):
+    _fields_ = [
+        ("material", ctypes.c_int),
+        ("psd", ctypes.c_int),
+        ("h", ctypes.c_double),
+        ("newton_rtol", ctypes.c_double),
+        ("newton_max_iter", ctypes.c_int),
+        ("pcg_rtol", ctypes.c_double),
+        ("pcg_max_iter", ctypes.c_int),
+        ("inner_rtol", ctypes.c_double),
+        ("ls_alpha", ctypes.c_double),
+        ("ls_beta", ctypes.c_double),
+        ("ls_max_iter", ctypes.c_int),
+        ("ls_threshold", ctypes.c_double),
+        ("block", ctypes.c_int),
+        ("lanes", ctypes.c_int),
+        ("use_graph", ctypes.c_int),
+    ]
+
+
+class SkBatchInfo(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int),
+        ("newton_iters", ctypes.c_int),
+        ("pcg_iters", ctypes.c_int64),
+        ("ls_trials", ctypes.c_int),
+        ("refine_steps", ctypes.c_int),
+        ("g0_norm", ctypes.c_double),
+        ("g_norm", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -142,6 +178,14 @@ SIGNATURES = {
     "sk_sim_step": [_P, _P, _P, _P, _P, ctypes.POINTER(SkStepInfo), _P],
     "sk_modal_basis": [_P, _INT, _P, _P, _P, _P, _P, _P, _INT, _D, _D, _INT, _INT, ctypes.c_uint64, _P, _P,
                        ctypes.POINTER(SkModalInfo), _P],
+    "sk_batch_create": [_P, _INT, _P, _P, _P, ctypes.POINTER(SkBatchParams), _P, _P, _P, _P, _P, _P, _P],
+    "sk_batch_destroy": [_P],
+    "sk_batch_step": [_P, _P, _P, _P, _P, _P, _P, _P],
+    "sk_batch_set_shape": [_P, _INT, _INT],
+    "sk_batch_autotune": [_P, _P, _P, _P, _INT, _P, _P, _P, _P],
+    "sk_dense_project": [_I64, _INT, _P, _P, _P, _P, _P, _P],
+    "sk_dense_expand": [_I64, _INT, _P, _P, _P, _P, _P],
+    "sk_axpby": [_I64, _D, _P, _D, _P, _P, _P],
     "sk_gemm_bf16": [_INT, _INT, _INT, _P, _I64, _I64, _P, _I64, _I64, _P, _I64, _P],
     "sk_mlp_create": [_INT, _P, _INT, _P],
     "sk_mlp_destroy": [_P],

@@ -1,0 +1,858 @@
+// Batched implicit steps: many independent meshes (problems) of uneven size
+// advanced together, one launch per Newton phase for all of them.
+//
+// The problems are stored as one union mesh whose vertices and tets are
+// grouped per problem (contiguous ranges). Mesh-wide phases (element
+// gradient, element Hessian + PSD projection, block gather into BSR FP32
+// values) run once over the union. The phases with per-problem control flow
+// are persistent kernels with one workgroup per problem (looping over
+// problems when there are more problems than workgroups):
+//   * PCG: block-Jacobi CG on the problem's free DOFs in FP32 with FP64
+//     residual replacement (the mixed-precision scheme of the single-mesh
+//     simulator), all scalars reduced inside the workgroup;
+//   * line search: the Armijo backtracking of backtracking_line_search.py:8-67
+//     on the problem's own incremental potential, in FP64 difference form;
+//   * gradient norm and Newton stopping test (newton.py:46-69 with the
+//     relative-gradient criterion), which clears the problem's active flag.
+// Each problem keeps its own Newton iteration count, line-search step and
+// PCG state, so the answer of every problem is the one it would get alone.
+// One Newton iteration for the whole batch is a fixed kernel sequence that
+// reads the active flags on the device; it is captured once in a hipGraph
+// and replayed. Workgroup size and rows-per-lane layout of the per-problem
+// kernels are chosen by sk_batch_autotune (timed on the batch itself).
+#include "sk_elem.h"
+#include "sk_internal.h"
+#include "sk_materials.h"
+
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+namespace sk {
+namespace {
+
+struct BatchArgs {
+  int P;
+  const int32_t* vstart;   // [P+1]
+  const int32_t* tstart;   // [P+1]
+  const int32_t* pid;      // [nv] problem of each vertex
+  const int32_t* bptr;
+  const int32_t* bcol;
+  const float* vals;       // BSR, 9 per block
+  const float* binv;       // 9 per vertex
+  const uint8_t* fixed;
+  ElemView em;
+  const double* mu;
+  const double* lam;
+  const double* vol;
+  const double* mass;
+  const double* fext;
+  double inv_h2;
+  double* x;
+  const double* xt;
+  double* g;
+  double* dx;
+  double* r64;
+  float* r32;
+  float* p32;
+  float* Ap32;
+  float* x32;
+  // per problem
+  int* active;
+  int* status;
+  int* newton_iters;
+  long long* pcg_iters;
+  int* ls_trials;
+  int* refines;
+  double* g0;
+  double* gnorm;
+  double* tstep;
+  int* n_active;
+  // parameters
+  double pcg_rtol2, inner_rtol2;
+  int pcg_max_iter;
+  double ls_alpha, ls_beta, ls_threshold;
+  int ls_max_iter;
+  double newton_rtol;
+  int newton_max_iter;
+  int count_pcg;           // 0: autotuning run, counters untouched
+};
+
+// Every thread of an NT-thread block receives the NV totals (fixed order).
+// The leading barrier also orders the callers' earlier global stores.
+template <int NT, int NV>
+__device__ __forceinline__ void all_sum(double (&v)[NV], double* lds) {
+  constexpr int nw = NT / kWave;
+  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) lds[k * nw + wid] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < nw; ++w) s += lds[k * nw + w];
+    v[k] = s;
+  }
+}
+
+__device__ __forceinline__ bool is_fixed(const uint8_t* fixed, int64_t k) { return fixed && fixed[k]; }
+
+// y = A_row(v) . src over the row's blocks; LANES lanes share a row (lane j
+// takes slots j, j + LANES, ...) and combine with xor shuffles.
+template <int LANES, typename V>
+__device__ __forceinline__ void row_product(const BatchArgs& a, int64_t v, const V* __restrict__ src, int j,
+                                            double (&y)[3]) {
+  y[0] = y[1] = y[2] = 0.0;
+  if (v >= 0) {
+    const int32_t b1 = a.bptr[v + 1];
+    for (int32_t b = a.bptr[v] + j; b < b1; b += LANES) {
+      const int64_t w = 3 * (int64_t)a.bcol[b];
+      const float* A = a.vals + 9 * (int64_t)b;
+      const double s0 = (double)src[w], s1 = (double)src[w + 1], s2 = (double)src[w + 2];
+#pragma unroll
+      for (int p = 0; p < 3; ++p) y[p] += (double)A[3 * p] * s0 + (double)A[3 * p + 1] * s1 + (double)A[3 * p + 2] * s2;
+    }
+  }
+#pragma unroll
+  for (int off = 1; off < LANES; off <<= 1)
+#pragma unroll
+    for (int p = 0; p < 3; ++p) y[p] += __shfl_xor(y[p], off, kWave);
+}
+
+template <int LANES>
+__device__ __forceinline__ void row_product_f(const BatchArgs& a, int64_t v, const float* __restrict__ src, int j,
+                                              float (&y)[3]) {
+  y[0] = y[1] = y[2] = 0.f;
+  if (v >= 0) {
+    const int32_t b1 = a.bptr[v + 1];
+    for (int32_t b = a.bptr[v] + j; b < b1; b += LANES) {
+      const int64_t w = 3 * (int64_t)a.bcol[b];
+      const float* A = a.vals + 9 * (int64_t)b;
+      const float s0 = src[w], s1 = src[w + 1], s2 = src[w + 2];
+#pragma unroll
+      for (int p = 0; p < 3; ++p) y[p] += A[3 * p] * s0 + A[3 * p + 1] * s1 + A[3 * p + 2] * s2;
+    }
+  }
+#pragma unroll
+  for (int off = 1; off < LANES; off <<= 1)
+#pragma unroll
+    for (int p = 0; p < 3; ++p) y[p] += __shfl_xor(y[p], off, kWave);
+}
+
+__device__ __forceinline__ void apply_binv3(const float* B, float r0, float r1, float r2, float (&z)[3]) {
+#pragma unroll
+  for (int p = 0; p < 3; ++p) z[p] = B[3 * p] * r0 + B[3 * p + 1] * r1 + B[3 * p + 2] * r2;
+}
+
+// The workgroup's shared flag read: one thread reads, all threads branch alike.
+__device__ __forceinline__ int block_read(const int* flag) {
+  __shared__ int s;
+  __syncthreads();
+  if (threadIdx.x == 0) s = *flag;
+  __syncthreads();
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// Per-problem PCG: H dx = -g on the free DOFs, FP32 CG with FP64 residual
+// replacement whenever the FP32 recurrence residual has dropped by
+// inner_rtol; stops at ||b - H dx|| <= pcg_rtol ||b|| (FP64).
+// Thread rows: row slot i = threadIdx.x / LANES covers vertices v0 + i,
+// v0 + i + NT / LANES, ...
+// ---------------------------------------------------------------------------
+template <int NT, int LANES>
+__global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
+  __shared__ double lds[4 * (NT / kWave)];
+  constexpr int RS = NT / LANES;  // rows per sweep
+  const int j = threadIdx.x % LANES, slot = threadIdx.x / LANES;
+  for (int p = blockIdx.x; p < a.P; p += gridDim.x) {
+    if (!block_read(a.active + p)) continue;
+    const int64_t v0 = a.vstart[p], v1 = a.vstart[p + 1];
+    const int64_t nrow = v1 - v0;
+    const int64_t sweeps = (nrow + RS - 1) / RS;
+    // init: r64 = b = -g (free), dx = 0, r32 = b, x32 = 0, z = B^-1 r, p = z
+    double acc[2] = {0.0, 0.0};
+    for (int64_t s = 0; s < sweeps; ++s) {
+      const int64_t v = v0 + s * RS + slot;
+      if (v < v1 && j == 0) {
+        float r[3], z[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          const int64_t k = 3 * v + q;
+          const double b = is_fixed(a.fixed, k) ? 0.0 : -a.g[k];
+          a.r64[k] = b;
+          a.dx[k] = 0.0;
+          r[q] = (float)b;
+          a.r32[k] = r[q];
+          a.x32[k] = 0.f;
+          acc[1] += b * b;
+        }
+        apply_binv3(a.binv + 9 * v, r[0], r[1], r[2], z);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          a.p32[3 * v + q] = z[q];
+          acc[0] += (double)r[q] * (double)z[q];
+        }
+      }
+    }
+    all_sum<NT, 2>(acc, lds);
+    double rz = acc[0];
+    const double bb = acc[1];
+    const double target = a.pcg_rtol2 * bb;
+    double seg_rr = bb;
+    int it = 0, refines = 0, st = SK_OK;
+    bool done = !(bb > 0.0);
+    if (!isfinite(bb)) {
+      st = SK_ERR_NONFINITE;
+      done = true;
+    }
+    while (!done && it < a.pcg_max_iter) {
+      // Ap = A p (fixed rows 0), p.Ap
+      double d1[1] = {0.0};
+      for (int64_t s = 0; s < sweeps; ++s) {
+        const int64_t v = v0 + s * RS + slot;
+        float y[3];
+        row_product_f<LANES>(a, v < v1 ? v : -1, a.p32, j, y);
+        if (v < v1 && j == 0) {
+#pragma unroll
+          for (int q = 0; q < 3; ++q) {
+            const int64_t k = 3 * v + q;
+            const float ap = is_fixed(a.fixed, k) ? 0.f : y[q];
+            a.Ap32[k] = ap;
+            d1[0] += (double)a.p32[k] * (double)ap;
+          }
+        }
+      }
+      all_sum<NT, 1>(d1, lds);
+      const double pAp = d1[0];
+      if (!(pAp > 0.0)) {
+        st = isfinite(pAp) ? SK_ERR_PCG_BREAKDOWN : SK_ERR_NONFINITE;
+        break;
+      }
+      const float alpha = (float)(rz / pAp);
+      double d2[1] = {0.0};
+      for (int64_t s = 0; s < sweeps; ++s) {
+        const int64_t v = v0 + s * RS + slot;
+        if (v < v1 && j == 0) {
+#pragma unroll
+          for (int q = 0; q < 3; ++q) {
+            const int64_t k = 3 * v + q;
+            a.x32[k] += alpha * a.p32[k];
+            const float r = a.r32[k] - alpha * a.Ap32[k];
+            a.r32[k] = r;
+            d2[0] += (double)r * (double)r;
+          }
+        }
+      }
+      all_sum<NT, 1>(d2, lds);
+      ++it;
+      if (!isfinite(d2[0])) {
+        st = SK_ERR_NONFINITE;
+        break;
+      }
+      if (d2[0] <= a.inner_rtol2 * seg_rr || d2[0] <= target) {
+        // residual replacement: dx += x32 (FP64), r64 = b - A dx, r32 = r64
+        for (int64_t s = 0; s < sweeps; ++s) {
+          const int64_t v = v0 + s * RS + slot;
+          if (v < v1 && j == 0) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+              const int64_t k = 3 * v + q;
+              a.dx[k] += (double)a.x32[k];
+              a.x32[k] = 0.f;
+            }
+          }
+        }
+        __syncthreads();
+        double d3[1] = {0.0};
+        for (int64_t s = 0; s < sweeps; ++s) {
+          const int64_t v = v0 + s * RS + slot;
+          double y[3];
+          row_product<LANES, double>(a, v < v1 ? v : -1, a.dx, j, y);
+          if (v < v1 && j == 0) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+              const int64_t k = 3 * v + q;
+              const double r = is_fixed(a.fixed, k) ? 0.0 : -a.g[k] - y[q];
+              a.r64[k] = r;
+              a.r32[k] = (float)r;
+              d3[0] += r * r;
+            }
+          }
+        }
+        all_sum<NT, 1>(d3, lds);
+        ++refines;
+        if (d3[0] <= target) break;
+        seg_rr = d3[0];
+      }
+      // z = B^-1 r, r.z, then p = z + beta p
+      double d4[1] = {0.0};
+      for (int64_t s = 0; s < sweeps; ++s) {
+        const int64_t v = v0 + s * RS + slot;
+        if (v < v1 && j == 0) {
+          float z[3];
+          apply_binv3(a.binv + 9 * v, a.r32[3 * v], a.r32[3 * v + 1], a.r32[3 * v + 2], z);
+#pragma unroll
+          for (int q = 0; q < 3; ++q) d4[0] += (double)a.r32[3 * v + q] * (double)z[q];
+        }
+      }
+      all_sum<NT, 1>(d4, lds);
+      const float beta = (float)(d4[0] / rz);
+      rz = d4[0];
+      for (int64_t s = 0; s < sweeps; ++s) {
+        const int64_t v = v0 + s * RS + slot;
+        if (v < v1 && j == 0) {
+          float z[3];
+          apply_binv3(a.binv + 9 * v, a.r32[3 * v], a.r32[3 * v + 1], a.r32[3 * v + 2], z);
+#pragma unroll
+          for (int q = 0; q < 3; ++q) a.p32[3 * v + q] = z[q] + beta * a.p32[3 * v + q];
+        }
+      }
+      __syncthreads();
+    }
+    if (it >= a.pcg_max_iter && st == SK_OK) {
+      // iteration limit: the FP64 correction so far is used as the direction
+      for (int64_t s = 0; s < sweeps; ++s) {
+        const int64_t v = v0 + s * RS + slot;
+        if (v < v1 && j == 0)
+#pragma unroll
+          for (int q = 0; q < 3; ++q) {
+            a.dx[3 * v + q] += (double)a.x32[3 * v + q];
+            a.x32[3 * v + q] = 0.f;
+          }
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && a.count_pcg) {
+      a.pcg_iters[p] += it;
+      a.refines[p] += refines;
+      if (st != SK_OK) {
+        a.status[p] = st;
+        a.active[p] = 0;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Per-problem Armijo line search in difference form:
+//   dE(t) = sum_e vol (psi(x + t dx) - psi(x)) + sum_free [c/(2h^2) m ((x1-xt)^2 - (x0-xt)^2) - f (x1 - x0)]
+// accepted at the first t in {1, beta, beta^2, ...} with dE <= alpha t g.dx + threshold.
+// ---------------------------------------------------------------------------
+template <class M, int NT>
+__global__ __launch_bounds__(NT) void batch_ls_kernel(BatchArgs a) {
+  __shared__ double lds[2 * (NT / kWave)];
+  for (int p = blockIdx.x; p < a.P; p += gridDim.x) {
+    if (!block_read(a.active + p)) continue;
+    const int64_t v0 = a.vstart[p], v1 = a.vstart[p + 1];
+    const int64_t t0 = a.tstart[p], t1 = a.tstart[p + 1];
+    double sl[1] = {0.0};
+    for (int64_t k = 3 * v0 + threadIdx.x; k < 3 * v1; k += NT)
+      if (!is_fixed(a.fixed, k)) sl[0] += a.g[k] * a.dx[k];
+    all_sum<NT, 1>(sl, lds);
+    const double slope = sl[0];
+    double t = 1.0;
+    int trials = 0;
+    bool accepted = false;
+    for (int ls = 0; ls < a.ls_max_iter; ++ls) {
+      double de[1] = {0.0};
+      for (int64_t e = t0 + threadIdx.x; e < t1; e += NT) {
+        int32_t v[4];
+        double D[12], xe[12], F[9], F1[9], de12[12];
+        load_elem(a.em, e, v, D);
+        gather_x(a.x, v, xe);
+        gather_x(a.dx, v, de12);
+        make_F(xe, D, F);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) de12[k] = xe[k] + t * de12[k];
+        make_F(de12, D, F1);
+        const double mu = a.mu[e], lam = a.lam ? a.lam[e] : 0.0;
+        de[0] += a.vol[e] * M::energy(F1, mu, lam) - a.vol[e] * M::energy(F, mu, lam);
+      }
+      for (int64_t k = 3 * v0 + threadIdx.x; k < 3 * v1; k += NT) {
+        if (is_fixed(a.fixed, k)) continue;
+        const double x0 = a.x[k];
+        const double x1 = x0 + t * a.dx[k];
+        const double d0 = x0 - a.xt[k], d1 = x1 - a.xt[k];
+        de[0] += 0.5 * a.inv_h2 * a.mass[k] * (d1 - d0) * (d1 + d0);
+        if (a.fext) de[0] -= a.fext[k] * (x1 - x0);
+      }
+      all_sum<NT, 1>(de, lds);
+      ++trials;
+      if (de[0] <= a.ls_alpha * t * slope + a.ls_threshold) {
+        accepted = true;
+        break;
+      }
+      t *= a.ls_beta;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      a.ls_trials[p] += trials;
+      a.newton_iters[p] += 1;
+      a.tstep[p] = accepted ? t : 0.0;
+      if (!accepted) {
+        a.status[p] = SK_ERR_LS_FAIL;
+        a.active[p] = 0;
+      }
+    }
+  }
+}
+
+// x += t_p dx on the free DOFs of problems still active.
+__global__ void batch_update_kernel(int64_t n3, BatchArgs a) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n3) return;
+  const int p = a.pid[k / 3];
+  if (a.active[p] && !is_fixed(a.fixed, k)) a.x[k] += a.tstep[p] * a.dx[k];
+}
+
+// Per-problem ||g_free||. mode 0: start of a step (g0, reset counters and
+// flags); mode 1: Newton stopping test of an active problem.
+template <int NT>
+__global__ __launch_bounds__(NT) void batch_gnorm_kernel(BatchArgs a, int mode) {
+  __shared__ double lds[NT / kWave];
+  for (int p = blockIdx.x; p < a.P; p += gridDim.x) {
+    if (mode == 1 && !block_read(a.active + p)) continue;
+    const int64_t v0 = a.vstart[p], v1 = a.vstart[p + 1];
+    double s[1] = {0.0};
+    for (int64_t k = 3 * v0 + threadIdx.x; k < 3 * v1; k += NT)
+      if (!is_fixed(a.fixed, k)) s[0] += a.g[k] * a.g[k];
+    all_sum<NT, 1>(s, lds);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const double gn = sqrt(s[0]);
+      a.gnorm[p] = gn;
+      if (mode == 0) {
+        a.g0[p] = gn;
+        a.newton_iters[p] = 0;
+        a.pcg_iters[p] = 0;
+        a.ls_trials[p] = 0;
+        a.refines[p] = 0;
+        a.status[p] = isfinite(gn) ? SK_OK : SK_ERR_NONFINITE;
+        a.active[p] = (isfinite(gn) && gn > 0.0) ? 1 : 0;
+      } else if (!isfinite(gn)) {
+        a.status[p] = SK_ERR_NONFINITE;
+        a.active[p] = 0;
+      } else if (gn <= a.newton_rtol * a.g0[p]) {
+        a.active[p] = 0;
+      } else if (a.newton_iters[p] >= a.newton_max_iter) {
+        a.status[p] = SK_ERR_NEWTON_MAXIT;
+        a.active[p] = 0;
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void count_active_kernel(const int* active, int P, int* n_active) {
+  __shared__ double lds[kBlock / kWave];
+  double s[1] = {0.0};
+  for (int p = threadIdx.x; p < P; p += kBlock) s[0] += active[p];
+  all_sum<kBlock, 1>(s, lds);
+  if (threadIdx.x == 0) *n_active = (int)s[0];
+}
+
+__global__ void batch_init_kernel(int64_t n, const double* xc, const double* xp, const double* xbc,
+                                  const uint8_t* fixed, double h, double* x, double* xt) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const double c = xc[k];
+  const double t = c + h * ((c - xp[k]) / h);
+  xt[k] = t;
+  x[k] = (fixed && fixed[k]) ? (xbc ? xbc[k] : c) : t;
+}
+
+hipStream_t as_stream(sk_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+}  // namespace
+}  // namespace sk
+
+using namespace sk;
+
+struct sk_batch {
+  sk_mesh* mesh = nullptr;
+  int P = 0;
+  sk_batch_params prm{};
+  DevBuf<int32_t> vstart, tstart, pid;
+  DevBuf<double> mu, lam, vol, mass, fext;
+  DevBuf<uint8_t> fixed;
+  DevBuf<double> x, xt, g, dx, r64;
+  DevBuf<float> vals32, binv32, r32, p32, Ap32, x32;
+  DevBuf<int> active, status, newton_iters, ls_trials, refines, n_active;
+  DevBuf<long long> pcg_iters;
+  DevBuf<double> g0, gnorm, tstep;
+  int* n_active_host = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int graph_block = 0, graph_lanes = 0;
+  int grid = 0;
+  BatchArgs args() const;
+};
+
+BatchArgs sk_batch::args() const {
+  BatchArgs a{};
+  a.P = P;
+  a.vstart = vstart.p;
+  a.tstart = tstart.p;
+  a.pid = pid.p;
+  a.bptr = mesh->bptr.p;
+  a.bcol = mesh->bcol.p;
+  a.vals = vals32.p;
+  a.binv = binv32.p;
+  a.fixed = fixed.p;
+  a.em = view(mesh);
+  a.mu = mu.p;
+  a.lam = lam.p;
+  a.vol = vol.p;
+  a.mass = mass.p;
+  a.fext = fext.p;
+  a.inv_h2 = 1.0 / (prm.h * prm.h);
+  a.x = x.p;
+  a.xt = xt.p;
+  a.g = g.p;
+  a.dx = dx.p;
+  a.r64 = r64.p;
+  a.r32 = r32.p;
+  a.p32 = p32.p;
+  a.Ap32 = Ap32.p;
+  a.x32 = x32.p;
+  a.active = active.p;
+  a.status = status.p;
+  a.newton_iters = newton_iters.p;
+  a.pcg_iters = pcg_iters.p;
+  a.ls_trials = ls_trials.p;
+  a.refines = refines.p;
+  a.g0 = g0.p;
+  a.gnorm = gnorm.p;
+  a.tstep = tstep.p;
+  a.n_active = n_active.p;
+  a.pcg_rtol2 = prm.pcg_rtol * prm.pcg_rtol;
+  a.inner_rtol2 = prm.inner_rtol * prm.inner_rtol;
+  a.pcg_max_iter = prm.pcg_max_iter;
+  a.ls_alpha = prm.ls_alpha;
+  a.ls_beta = prm.ls_beta;
+  a.ls_threshold = prm.ls_threshold;
+  a.ls_max_iter = prm.ls_max_iter;
+  a.newton_rtol = prm.newton_rtol;
+  a.newton_max_iter = prm.newton_max_iter;
+  a.count_pcg = 1;
+  return a;
+}
+
+namespace {
+
+int launch_pcg(const sk_batch* B, const BatchArgs& a, int block, int lanes) {
+  const int g = B->grid;
+#define SK_PCG_CASE(NT, L) \
+  if (block == NT && lanes == L) { batch_pcg_kernel<NT, L><<<g, NT, 0, B->st>>>(a); return SK_OK; }
+  SK_PCG_CASE(128, 1) SK_PCG_CASE(128, 4) SK_PCG_CASE(256, 1) SK_PCG_CASE(256, 4)
+  SK_PCG_CASE(512, 1) SK_PCG_CASE(512, 4) SK_PCG_CASE(1024, 1) SK_PCG_CASE(1024, 4)
+#undef SK_PCG_CASE
+  set_error("sk_batch: unsupported block/lanes");
+  return SK_ERR_INVALID;
+}
+
+template <int NT>
+int launch_ls_nt(const sk_batch* B, const BatchArgs& a) {
+  const bool ok = dispatch_material(B->prm.material, [&](auto tag) {
+    batch_ls_kernel<typename decltype(tag)::template type<double>, NT><<<B->grid, NT, 0, B->st>>>(a);
+  });
+  SK_REQUIRE(ok, "sk_batch: unknown material");
+  return SK_OK;
+}
+
+int launch_ls(const sk_batch* B, const BatchArgs& a, int block) {
+  if (block == 128) return launch_ls_nt<128>(B, a);
+  if (block == 256) return launch_ls_nt<256>(B, a);
+  if (block == 512) return launch_ls_nt<512>(B, a);
+  if (block == 1024) return launch_ls_nt<1024>(B, a);
+  set_error("sk_batch: unsupported block");
+  return SK_ERR_INVALID;
+}
+
+int gradient_phase(sk_batch* B) {
+  sk_mesh* m = B->mesh;
+  SK_TRY(launch_elem_grad(m, B->prm.material, B->x.p, nullptr, B->mu.p, B->lam.p, B->vol.p, B->st));
+  return launch_gather_grad(m, B->g.p, B->mass.p, B->x.p, B->xt.p, B->fext.p, B->fixed.p,
+                            1.0 / (B->prm.h * B->prm.h), B->st);
+}
+
+// Element Hessian + BSR FP32 gather (every problem; inactive ones are not read).
+int hessian_phase(sk_batch* B) {
+  sk_mesh* m = B->mesh;
+  SK_TRY(launch_elem_hess(m, B->prm.material, B->x.p, nullptr, B->mu.p, B->lam.p, B->vol.p, B->prm.psd, true,
+                          B->st));
+  return launch_gather_hess(m, B->vals32.p, B->mass.p, 1.0 / (B->prm.h * B->prm.h), B->fixed.p, B->binv32.p,
+                            kLayoutBSR, true, B->st);
+}
+
+// One Newton iteration of every active problem (no host synchronization).
+int newton_iteration(sk_batch* B, int block, int lanes) {
+  const BatchArgs a = B->args();
+  const int64_t n3 = 3 * B->mesh->nv;
+  SK_TRY(hessian_phase(B));
+  SK_TRY(launch_pcg(B, a, block, lanes));
+  SK_TRY(launch_ls(B, a, block));
+  batch_update_kernel<<<grid_for(n3), kBlock, 0, B->st>>>(n3, a);
+  SK_TRY(gradient_phase(B));
+  batch_gnorm_kernel<256><<<B->grid, 256, 0, B->st>>>(a, 1);
+  count_active_kernel<<<1, kBlock, 0, B->st>>>(B->active.p, B->P, B->n_active.p);
+  SK_HIP_CHECK(hipGetLastError());
+  return SK_OK;
+}
+
+int start_step(sk_batch* B, const double* x_curr, const double* x_prev, const double* x_bc) {
+  const int64_t n3 = 3 * B->mesh->nv;
+  batch_init_kernel<<<grid_for(n3), kBlock, 0, B->st>>>(n3, x_curr, x_prev, x_bc, B->fixed.p, B->prm.h, B->x.p,
+                                                        B->xt.p);
+  SK_TRY(gradient_phase(B));
+  const BatchArgs a = B->args();
+  batch_gnorm_kernel<256><<<B->grid, 256, 0, B->st>>>(a, 0);
+  count_active_kernel<<<1, kBlock, 0, B->st>>>(B->active.p, B->P, B->n_active.p);
+  SK_HIP_CHECK(hipGetLastError());
+  return SK_OK;
+}
+
+int read_active(sk_batch* B, int* n) {
+  SK_HIP_CHECK(hipMemcpyAsync(B->n_active_host, B->n_active.p, sizeof(int), hipMemcpyDeviceToHost, B->st));
+  SK_HIP_CHECK(hipStreamSynchronize(B->st));
+  *n = *B->n_active_host;
+  return SK_OK;
+}
+
+int ensure_graph(sk_batch* B, int block, int lanes) {
+  if (B->exec && B->graph_block == block && B->graph_lanes == lanes) return SK_OK;
+  if (B->exec) (void)hipGraphExecDestroy(B->exec);
+  if (B->graph) (void)hipGraphDestroy(B->graph);
+  B->exec = nullptr;
+  B->graph = nullptr;
+  SK_HIP_CHECK(hipStreamBeginCapture(B->st, hipStreamCaptureModeThreadLocal));
+  const int s = newton_iteration(B, block, lanes);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(B->st, &g);
+  if (s != SK_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return s;
+  }
+  SK_HIP_CHECK(e);
+  B->graph = g;
+  SK_HIP_CHECK(hipGraphInstantiate(&B->exec, g, nullptr, nullptr, 0));
+  B->graph_block = block;
+  B->graph_lanes = lanes;
+  return SK_OK;
+}
+
+int batch_block(const sk_batch* B) { return B->prm.block > 0 ? B->prm.block : 256; }
+int batch_lanes(const sk_batch* B) { return B->prm.lanes > 0 ? B->prm.lanes : 1; }
+
+}  // namespace
+
+extern "C" {
+
+int sk_batch_create(sk_mesh* mesh, int nprob, const int64_t* vstart, const int64_t* tstart, const int32_t* T_host,
+                    const sk_batch_params* p, const double* mu, const double* lam, const double* vol,
+                    const double* mass, const double* f_ext, const uint8_t* fixed, sk_batch** out) {
+  SK_REQUIRE(out, "sk_batch_create: null out");
+  *out = nullptr;
+  SK_REQUIRE(mesh && p && vstart && tstart && T_host && mu && vol && mass, "sk_batch_create: null argument");
+  SK_REQUIRE(nprob >= 1, "sk_batch_create: need at least one problem");
+  SK_REQUIRE(p->material >= SK_SNH && p->material <= SK_FCR, "sk_batch_create: unknown material");
+  SK_REQUIRE(!material_needs_lam(p->material) || lam, "sk_batch_create: this material needs lam");
+  SK_REQUIRE(p->h > 0.0 && p->ls_alpha > 0.0 && p->ls_alpha <= 0.5 && p->ls_beta > 0.0 && p->ls_beta < 1.0,
+             "sk_batch_create: invalid h or line-search constants");
+  SK_REQUIRE(p->inner_rtol > 0.0 && p->inner_rtol < 1.0, "sk_batch_create: inner_rtol must be in (0, 1)");
+  SK_REQUIRE(p->psd == SK_PSD_NONE || p->psd == SK_PSD_PROJ || p->psd == SK_PSD_ABS, "sk_batch_create: bad psd");
+  SK_REQUIRE(p->block == 0 || p->block == 128 || p->block == 256 || p->block == 512 || p->block == 1024,
+             "sk_batch_create: block must be 0, 128, 256, 512 or 1024");
+  SK_REQUIRE(p->lanes == 0 || p->lanes == 1 || p->lanes == 4, "sk_batch_create: lanes must be 0, 1 or 4");
+  SK_REQUIRE(vstart[0] == 0 && vstart[nprob] == mesh->nv && tstart[0] == 0 && tstart[nprob] == mesh->nt,
+             "sk_batch_create: problem ranges must cover the mesh");
+  std::vector<int32_t> vs(nprob + 1), ts(nprob + 1), pid(mesh->nv);
+  for (int q = 0; q <= nprob; ++q) {
+    vs[q] = (int32_t)vstart[q];
+    ts[q] = (int32_t)tstart[q];
+    if (q > 0) SK_REQUIRE(vstart[q] > vstart[q - 1] && tstart[q] >= tstart[q - 1], "sk_batch_create: bad ranges");
+  }
+  for (int q = 0; q < nprob; ++q) {
+    for (int64_t v = vstart[q]; v < vstart[q + 1]; ++v) pid[v] = q;
+    for (int64_t e = tstart[q]; e < tstart[q + 1]; ++e)
+      for (int c = 0; c < 4; ++c) {
+        const int64_t v = T_host[4 * e + c];
+        SK_REQUIRE(v >= vstart[q] && v < vstart[q + 1], "sk_batch_create: a tet references another problem");
+      }
+  }
+  sk_batch* B = new sk_batch();
+  B->mesh = mesh;
+  B->P = nprob;
+  B->prm = *p;
+  B->grid = nprob;
+  const size_t nt = mesh->nt, n3 = 3 * mesh->nv;
+  auto copy = [](DevBuf<double>& d, const double* s, size_t n) -> int {
+    SK_TRY(d.alloc(n));
+    SK_HIP_CHECK(hipMemcpy(d.p, s, n * sizeof(double), hipMemcpyDeviceToDevice));
+    return SK_OK;
+  };
+  auto run = [&]() -> int {
+    SK_TRY(B->vstart.upload(vs));
+    SK_TRY(B->tstart.upload(ts));
+    SK_TRY(B->pid.upload(pid));
+    SK_TRY(copy(B->mu, mu, nt));
+    if (lam) SK_TRY(copy(B->lam, lam, nt));
+    SK_TRY(copy(B->vol, vol, nt));
+    SK_TRY(copy(B->mass, mass, n3));
+    if (f_ext) SK_TRY(copy(B->fext, f_ext, n3));
+    if (fixed) {
+      SK_TRY(B->fixed.alloc(n3));
+      SK_HIP_CHECK(hipMemcpy(B->fixed.p, fixed, n3, hipMemcpyDeviceToDevice));
+    }
+    for (DevBuf<double>* b : {&B->x, &B->xt, &B->g, &B->dx, &B->r64}) SK_TRY(b->alloc(n3));
+    for (DevBuf<float>* b : {&B->r32, &B->p32, &B->Ap32, &B->x32}) SK_TRY(b->alloc(n3));
+    SK_TRY(B->vals32.alloc(9 * (size_t)mesh->nblk));
+    SK_TRY(B->binv32.alloc(9 * (size_t)mesh->nv));
+    for (DevBuf<int>* b : {&B->active, &B->status, &B->newton_iters, &B->ls_trials, &B->refines}) {
+      SK_TRY(b->alloc(nprob));
+      SK_HIP_CHECK(hipMemset(b->p, 0, nprob * sizeof(int)));
+    }
+    SK_TRY(B->pcg_iters.alloc(nprob));
+    SK_HIP_CHECK(hipMemset(B->pcg_iters.p, 0, nprob * sizeof(long long)));
+    for (DevBuf<double>* b : {&B->g0, &B->gnorm, &B->tstep}) SK_TRY(b->alloc(nprob));
+    SK_TRY(B->n_active.alloc(1));
+    SK_HIP_CHECK(hipHostMalloc((void**)&B->n_active_host, sizeof(int), hipHostMallocDefault));
+    SK_HIP_CHECK(hipStreamCreate(&B->st));
+    SK_HIP_CHECK(hipEventCreateWithFlags(&B->ev_in, hipEventDisableTiming));
+    SK_HIP_CHECK(hipEventCreateWithFlags(&B->ev_out, hipEventDisableTiming));
+    return SK_OK;
+  };
+  const int s = run();
+  if (s != SK_OK) {
+    sk_batch_destroy(B);
+    return s;
+  }
+  *out = B;
+  return SK_OK;
+}
+
+int sk_batch_destroy(sk_batch* B) {
+  if (!B) return SK_OK;
+  if (B->st) (void)hipStreamSynchronize(B->st);
+  if (B->exec) (void)hipGraphExecDestroy(B->exec);
+  if (B->graph) (void)hipGraphDestroy(B->graph);
+  if (B->ev_in) (void)hipEventDestroy(B->ev_in);
+  if (B->ev_out) (void)hipEventDestroy(B->ev_out);
+  if (B->st) (void)hipStreamDestroy(B->st);
+  if (B->n_active_host) (void)hipHostFree(B->n_active_host);
+  delete B;
+  return SK_OK;
+}
+
+int sk_batch_step(sk_batch* B, const double* x_curr, const double* x_prev, const double* x_bc, double* x_next,
+                  sk_batch_info* info, int* rounds, sk_stream_t s) {
+  SK_REQUIRE(B && x_curr && x_prev && x_next, "sk_batch_step: null argument");
+  hipStream_t caller = as_stream(s);
+  SK_HIP_CHECK(hipEventRecord(B->ev_in, caller));
+  SK_HIP_CHECK(hipStreamWaitEvent(B->st, B->ev_in, 0));
+  const int block = batch_block(B), lanes = batch_lanes(B);
+  SK_TRY(start_step(B, x_curr, x_prev, x_bc));
+  int n_active = 0, r = 0;
+  SK_TRY(read_active(B, &n_active));
+  if (B->prm.use_graph) SK_TRY(ensure_graph(B, block, lanes));
+  while (n_active > 0 && r < B->prm.newton_max_iter) {
+    if (B->prm.use_graph) SK_HIP_CHECK(hipGraphLaunch(B->exec, B->st));
+    else SK_TRY(newton_iteration(B, block, lanes));
+    ++r;
+    SK_TRY(read_active(B, &n_active));
+  }
+  const int64_t n3 = 3 * B->mesh->nv;
+  SK_HIP_CHECK(hipMemcpyAsync(x_next, B->x.p, n3 * sizeof(double), hipMemcpyDeviceToDevice, B->st));
+  if (info) {
+    std::vector<int> st(B->P), ni(B->P), ls(B->P), rf(B->P);
+    std::vector<long long> pc(B->P);
+    std::vector<double> g0(B->P), gn(B->P);
+    SK_HIP_CHECK(hipMemcpyAsync(st.data(), B->status.p, B->P * sizeof(int), hipMemcpyDeviceToHost, B->st));
+    SK_HIP_CHECK(hipMemcpyAsync(ni.data(), B->newton_iters.p, B->P * sizeof(int), hipMemcpyDeviceToHost, B->st));
+    SK_HIP_CHECK(hipMemcpyAsync(ls.data(), B->ls_trials.p, B->P * sizeof(int), hipMemcpyDeviceToHost, B->st));
+    SK_HIP_CHECK(hipMemcpyAsync(rf.data(), B->refines.p, B->P * sizeof(int), hipMemcpyDeviceToHost, B->st));
+    SK_HIP_CHECK(hipMemcpyAsync(pc.data(), B->pcg_iters.p, B->P * sizeof(long long), hipMemcpyDeviceToHost, B->st));
+    SK_HIP_CHECK(hipMemcpyAsync(g0.data(), B->g0.p, B->P * sizeof(double), hipMemcpyDeviceToHost, B->st));
+    SK_HIP_CHECK(hipMemcpyAsync(gn.data(), B->gnorm.p, B->P * sizeof(double), hipMemcpyDeviceToHost, B->st));
+    SK_HIP_CHECK(hipStreamSynchronize(B->st));
+    for (int q = 0; q < B->P; ++q) {
+      info[q].status = (st[q] == SK_OK && gn[q] > B->prm.newton_rtol * g0[q] && g0[q] > 0.0) ? SK_ERR_NEWTON_MAXIT
+                                                                                              : st[q];
+      info[q].newton_iters = ni[q];
+      info[q].pcg_iters = pc[q];
+      info[q].ls_trials = ls[q];
+      info[q].refine_steps = rf[q];
+      info[q].g0_norm = g0[q];
+      info[q].g_norm = gn[q];
+    }
+  }
+  if (rounds) *rounds = r;
+  SK_HIP_CHECK(hipEventRecord(B->ev_out, B->st));
+  SK_HIP_CHECK(hipStreamWaitEvent(caller, B->ev_out, 0));
+  return SK_OK;
+}
+
+int sk_batch_set_shape(sk_batch* B, int block, int lanes) {
+  SK_REQUIRE(B, "sk_batch_set_shape: null batch");
+  SK_REQUIRE(block == 128 || block == 256 || block == 512 || block == 1024, "sk_batch_set_shape: bad block");
+  SK_REQUIRE(lanes == 1 || lanes == 4, "sk_batch_set_shape: lanes must be 1 or 4");
+  B->prm.block = block;
+  B->prm.lanes = lanes;
+  return SK_OK;
+}
+
+int sk_batch_autotune(sk_batch* B, const double* x_curr, const double* x_prev, const double* x_bc, int reps,
+                      int* block_out, int* lanes_out, double* ms_out, sk_stream_t s) {
+  SK_REQUIRE(B && x_curr && x_prev && block_out && lanes_out, "sk_batch_autotune: null argument");
+  SK_REQUIRE(reps >= 1, "sk_batch_autotune: reps must be >= 1");
+  hipStream_t caller = as_stream(s);
+  SK_HIP_CHECK(hipEventRecord(B->ev_in, caller));
+  SK_HIP_CHECK(hipStreamWaitEvent(B->st, B->ev_in, 0));
+  // The first Newton system of a step: gradient, Hessian, gather.
+  SK_TRY(start_step(B, x_curr, x_prev, x_bc));
+  SK_TRY(hessian_phase(B));
+  BatchArgs a = B->args();
+  a.count_pcg = 0;
+  hipEvent_t e0, e1;
+  SK_HIP_CHECK(hipEventCreate(&e0));
+  SK_HIP_CHECK(hipEventCreate(&e1));
+  const int blocks[4] = {128, 256, 512, 1024}, lanes[2] = {1, 4};
+  double best = 1e300;
+  int bb = 256, bl = 1, c = 0;
+  for (int bi = 0; bi < 4; ++bi)
+    for (int li = 0; li < 2; ++li, ++c) {
+      SK_TRY(launch_pcg(B, a, blocks[bi], lanes[li]));  // warm-up
+      SK_HIP_CHECK(hipEventRecord(e0, B->st));
+      for (int k = 0; k < reps; ++k) SK_TRY(launch_pcg(B, a, blocks[bi], lanes[li]));
+      SK_HIP_CHECK(hipEventRecord(e1, B->st));
+      SK_HIP_CHECK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      SK_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      const double t = ms / reps;
+      if (ms_out) ms_out[c] = t;
+      if (t < best) {
+        best = t;
+        bb = blocks[bi];
+        bl = lanes[li];
+      }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  B->prm.block = bb;
+  B->prm.lanes = bl;
+  *block_out = bb;
+  *lanes_out = bl;
+  SK_HIP_CHECK(hipEventRecord(B->ev_out, B->st));
+  SK_HIP_CHECK(hipStreamWaitEvent(caller, B->ev_out, 0));
+  return SK_OK;
+}
+
+}  // extern "C"

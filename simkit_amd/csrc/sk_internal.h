@@ -130,11 +130,13 @@ int launch_gather_grad(sk_mesh* m, double* g, const double* mass, const double* 
 // Gathers staged element blocks into CSR values; with mass != nullptr adds
 // mass * inv_h2 on the diagonal and writes the fixed-DOF-masked inverse of each
 // diagonal 3x3 block into binv (9 per vertex).
-// sell = true stores the values in the SELL layout (the simulator's private
-// matrix); false uses the scalar CSR layout. fp32: values / binv are float
-// arrays summed in FP32, otherwise double.
+// layout: kLayoutCSR (scalar CSR of the sorted pattern), kLayoutSELL (the
+// simulator's private SELL matrix) or kLayoutBSR (block b's 3x3 row-major at
+// values[9 b]). fp32: values / binv are float arrays summed in FP32, otherwise
+// double.
+enum ValueLayout : int { kLayoutCSR = 0, kLayoutSELL = 1, kLayoutBSR = 2 };
 int launch_gather_hess(sk_mesh* m, void* values, const double* mass, double inv_h2, const uint8_t* fixed,
-                       void* binv, bool sell, bool fp32, hipStream_t st);
+                       void* binv, int layout, bool fp32, hipStream_t st);
 // Copies n doubles from device to the mesh's pinned buffer and waits.
 int read_scalars(sk_mesh* m, const double* dev, int n, hipStream_t st);
 

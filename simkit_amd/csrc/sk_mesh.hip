@@ -11,6 +11,7 @@
 //      order and writes the 3 row segments of the block. No two lanes write the
 //      same value and the summation order is fixed, so values are
 //      bit-reproducible.
+#include "sk_elem.h"
 #include "sk_internal.h"
 #include "sk_materials.h"
 
@@ -18,45 +19,6 @@
 #include <numeric>
 
 namespace sk {
-
-// ---------------------------------------------------------------------------
-// Element helpers (SoA mesh layout)
-// ---------------------------------------------------------------------------
-struct ElemView {
-  int64_t nt;
-  const int32_t* __restrict__ T;  // [4][nt]
-  const double* __restrict__ D;   // [12][nt]
-};
-
-__device__ __forceinline__ void load_elem(const ElemView& m, int64_t e, int32_t (&v)[4], double (&D)[12]) {
-#pragma unroll
-  for (int k = 0; k < 4; ++k) v[k] = m.T[k * m.nt + e];
-#pragma unroll
-  for (int k = 0; k < 12; ++k) D[k] = m.D[k * m.nt + e];
-}
-
-__device__ __forceinline__ void gather_x(const double* __restrict__ x, const int32_t (&v)[4], double (&xe)[12]) {
-#pragma unroll
-  for (int k = 0; k < 4; ++k)
-#pragma unroll
-    for (int i = 0; i < 3; ++i) xe[3 * k + i] = x[3 * (int64_t)v[k] + i];
-}
-
-// F[3i+c] = sum_v xe[3v+i] D[4c+v]
-__device__ __forceinline__ void make_F(const double (&xe)[12], const double (&D)[12], double (&F)[9]) {
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-      F[3 * i + c] = xe[i] * D[4 * c] + xe[3 + i] * D[4 * c + 1] + xe[6 + i] * D[4 * c + 2] + xe[9 + i] * D[4 * c + 3];
-}
-
-__device__ __forceinline__ void add_F0(const double* __restrict__ F0, int64_t e, double (&F)[9]) {
-  if (F0) {
-#pragma unroll
-    for (int k = 0; k < 9; ++k) F[k] += F0[9 * e + k];
-  }
-}
 
 // ---------------------------------------------------------------------------
 // Element passes
@@ -281,6 +243,7 @@ struct GatherHArgs {
   const uint8_t* fixed;
   void* binv;
   const int32_t* sell_pos;  // non-null: SELL layout
+  int bsr;                  // 1: BSR layout (block b at values[9 b + 3 p + q])
 };
 
 // One lane per block: sums the staged contributions in list order (precision
@@ -325,6 +288,10 @@ __global__ __launch_bounds__(kBlock) void gather_hess_kernel(GatherHArgs a) {
     R* out = values + (q - i) * 9 + i;
 #pragma unroll
     for (int m = 0; m < 9; ++m) out[m * kSellC] = acc[m];
+  } else if (a.bsr) {
+    R* out = values + 9 * b;
+#pragma unroll
+    for (int m = 0; m < 9; ++m) out[m] = acc[m];
   } else {
     R* out = values + 9 * (int64_t)b0 + 3 * r;
 #pragma unroll
@@ -404,7 +371,6 @@ __global__ void aos_to_soa_kernel(const double* __restrict__ D, int64_t nt, doub
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-static ElemView view(sk_mesh* m) { return ElemView{m->nt, m->T.p, m->D.p}; }
 
 int launch_elem_energy(sk_mesh* m, int material, const double* x, const double* F0, const double* dx, double t,
                        const double* mu,
@@ -481,9 +447,11 @@ int launch_gather_grad(sk_mesh* m, double* g, const double* mass, const double* 
 }
 
 int launch_gather_hess(sk_mesh* m, void* values, const double* mass, double inv_h2, const uint8_t* fixed,
-                       void* binv, bool sell, bool fp32, hipStream_t st) {
+                       void* binv, int layout, bool fp32, hipStream_t st) {
+  SK_REQUIRE(layout == kLayoutCSR || layout == kLayoutSELL || layout == kLayoutBSR, "invalid value layout");
   GatherHArgs a{m->nblk, m->bptr.p, m->bcol.p, m->brow.p, m->cptr.p, m->clist.p, m->Kst.p,
-                values, mass, inv_h2, fixed, binv, sell ? m->sell_pos.p : nullptr};
+                values, mass, inv_h2, fixed, binv, layout == kLayoutSELL ? m->sell_pos.p : nullptr,
+                layout == kLayoutBSR ? 1 : 0};
   if (fp32) gather_hess_kernel<float><<<grid_for(m->nblk), kBlock, 0, st>>>(a);
   else gather_hess_kernel<double><<<grid_for(m->nblk), kBlock, 0, st>>>(a);
   SK_HIP_CHECK(hipGetLastError());
@@ -714,7 +682,7 @@ int sk_mesh_hessian(sk_mesh* m, int material, const double* x, const double* F0,
   SK_REQUIRE(!material_needs_lam(material) || lam, "sk_mesh_hessian: this material needs lam");
   hipStream_t st = as_stream(s);
   SK_TRY(launch_elem_hess(m, material, x, F0, mu, lam, vol, psd, false, st));
-  SK_TRY(launch_gather_hess(m, values, nullptr, 0.0, nullptr, nullptr, false, false, st));
+  SK_TRY(launch_gather_hess(m, values, nullptr, 0.0, nullptr, nullptr, kLayoutCSR, false, st));
   return SK_OK;
 }
 

@@ -281,6 +281,50 @@ __global__ void dscale_kernel(int64_t n, const double* mass, const uint8_t* fixe
   d[i] = (fixed && fixed[i]) || !(mass[i] > 0.0) ? 0.0 : 1.0 / sqrt(mass[i]);
 }
 
+// Per-workgroup partials of q_j = sum_r basis[r][j] w[r] (x[r] - base[r]),
+// j < k (k <= 256): thread t handles column t % k on rows t / k, t / k + 256/k, ...
+__global__ __launch_bounds__(kBlock) void project_partial_kernel(int64_t rows, int k, const double* __restrict__ B,
+                                                                 const double* __restrict__ w,
+                                                                 const double* __restrict__ x,
+                                                                 const double* __restrict__ base,
+                                                                 int64_t rows_per_block, double* partials) {
+  __shared__ double red[kBlock];
+  const int t = threadIdx.x;
+  const int lanes = kBlock / k;  // row lanes per column
+  const int j = t % k, rl = t / k;
+  const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
+  const int64_t r1 = r0 + rows_per_block < rows ? r0 + rows_per_block : rows;
+  double s = 0.0;
+  if (rl < lanes) {
+    for (int64_t r = r0 + rl; r < r1; r += lanes) {
+      const double d = (x[r] - (base ? base[r] : 0.0)) * (w ? w[r] : 1.0);
+      s += B[r * k + j] * d;
+    }
+  }
+  red[t] = s;
+  __syncthreads();
+  if (t < k) {
+    double tot = 0.0;
+    for (int q = 0; q < lanes; ++q) tot += red[q * k + t];
+    partials[(int64_t)blockIdx.x * k + t] = tot;
+  }
+}
+
+// out[r] = base[r] + sum_j basis[r][j] c[j].
+__global__ void expand_kernel(int64_t rows, int k, const double* __restrict__ B, const double* __restrict__ c,
+                              const double* base, double* out) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  double s = base ? base[r] : 0.0;
+  for (int j = 0; j < k; ++j) s += B[r * k + j] * c[j];
+  out[r] = s;
+}
+
+__global__ void axpby_kernel(int64_t n, double a, const double* x, double b, const double* y, double* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = a * x[i] + b * y[i];
+}
+
 // ---------------------------------------------------------------------------
 // Host: symmetric eigen-decomposition (cyclic Jacobi) of a small dense matrix.
 // A (n x n, row-major) is overwritten; eigenvalues ascending in w, vectors as
@@ -478,7 +522,7 @@ int sk_modal_basis(sk_mesh* m, int material, const double* x, const double* mu, 
   // K at x (no PSD projection unless the density needs it: psd = NONE keeps
   // the reference's linear-elasticity stiffness exact).
   SK_TRY(launch_elem_hess(m, material, x, nullptr, mu, lam, vol, SK_PSD_NONE, false, st));
-  SK_TRY(launch_gather_hess(m, K.p, nullptr, 0.0, nullptr, nullptr, false, false, st));
+  SK_TRY(launch_gather_hess(m, K.p, nullptr, 0.0, nullptr, nullptr, kLayoutCSR, false, st));
   dscale_kernel<<<grid_for(md.rows), kBlock, 0, st>>>(md.rows, mass, fixed, md.dscale.p);
   scale_blocks_kernel<<<grid_for(m->nblk), kBlock, 0, st>>>(m->nblk, m->bptr.p, m->brow.p, m->bcol.p, K.p,
                                                             md.dscale.p, shift, md.A.p);
@@ -544,6 +588,41 @@ int sk_modal_basis(sk_mesh* m, int material, const double* x, const double* mu, 
     set_error("sk_modal_basis: residual tolerance not reached within max_sweeps");
     return SK_ERR_EIG_MAXIT;
   }
+  return SK_OK;
+}
+
+int sk_dense_project(int64_t rows, int k, const double* basis, const double* weight, const double* x,
+                     const double* base, double* q, sk_stream_t s) {
+  SK_REQUIRE(rows >= 0 && k >= 1 && k <= kBlock, "sk_dense_project: need rows >= 0 and 1 <= k <= 256");
+  SK_REQUIRE(basis && x && q, "sk_dense_project: null pointer");
+  hipStream_t st = reinterpret_cast<hipStream_t>(s);
+  const int64_t rpb = 4096;
+  const int nb = (int)std::max<int64_t>(1, (rows + rpb - 1) / rpb);
+  DevBuf<double> partials;
+  SK_TRY(partials.alloc((size_t)nb * k));
+  project_partial_kernel<<<nb, kBlock, 0, st>>>(rows, k, basis, weight, x, base, rpb, partials.p);
+  sum_partials_kernel<<<grid_for(k), kBlock, 0, st>>>(partials.p, nb, k, q);
+  SK_HIP_CHECK(hipGetLastError());
+  SK_HIP_CHECK(hipStreamSynchronize(st));  // partials are released on return
+  return SK_OK;
+}
+
+int sk_dense_expand(int64_t rows, int k, const double* basis, const double* coef, const double* base, double* out,
+                    sk_stream_t s) {
+  SK_REQUIRE(rows >= 0 && k >= 1, "sk_dense_expand: need rows >= 0 and k >= 1");
+  SK_REQUIRE(basis && coef && out, "sk_dense_expand: null pointer");
+  if (rows == 0) return SK_OK;
+  expand_kernel<<<grid_for(rows), kBlock, 0, reinterpret_cast<hipStream_t>(s)>>>(rows, k, basis, coef, base, out);
+  SK_HIP_CHECK(hipGetLastError());
+  return SK_OK;
+}
+
+int sk_axpby(int64_t n, double a, const double* x, double b, const double* y, double* out, sk_stream_t s) {
+  SK_REQUIRE(n >= 0, "sk_axpby: negative size");
+  if (n == 0) return SK_OK;
+  SK_REQUIRE(x && y && out, "sk_axpby: null pointer");
+  axpby_kernel<<<grid_for(n), kBlock, 0, reinterpret_cast<hipStream_t>(s)>>>(n, a, x, b, y, out);
+  SK_HIP_CHECK(hipGetLastError());
   return SK_OK;
 }
 

@@ -857,6 +857,7 @@ int step_core(sk_sim* S, double c_mass, const double* x_guess, double* x_next, s
       SK_TRY(launch_gather_grad(m, S->g.p, S->mass.p, S->x.p, S->xt.p, fext, fixed, inv_h2, st));
       SK_TRY(read_scalars(m, m->red.result.p, 1, st));
       gn = std::sqrt(m->pinned[0]);
+      inf.guess_g_norm = gn;
       if (!std::isfinite(gn)) inf.status = SK_ERR_NONFINITE;
     }
   }
@@ -871,9 +872,9 @@ int step_core(sk_sim* S, double c_mass, const double* x_guess, double* x_next, s
     if (P.timing) SK_HIP_CHECK(hipEventRecord(S->ev[0], st));
     SK_TRY(launch_elem_hess(m, P.material, S->x.p, nullptr, S->mu.p, lam, S->vol.p, P.psd, mixed, st));
     if (mixed)
-      SK_TRY(launch_gather_hess(m, S->vals32.p, S->mass.p, inv_h2, fixed, S->binv32.p, true, true, st));
+      SK_TRY(launch_gather_hess(m, S->vals32.p, S->mass.p, inv_h2, fixed, S->binv32.p, kLayoutSELL, true, st));
     else
-      SK_TRY(launch_gather_hess(m, S->vals.p, S->mass.p, inv_h2, fixed, S->binv.p, true, false, st));
+      SK_TRY(launch_gather_hess(m, S->vals.p, S->mass.p, inv_h2, fixed, S->binv.p, kLayoutSELL, false, st));
     if (P.timing) SK_HIP_CHECK(hipEventRecord(S->ev[1], st));
     // Newton direction: H dx = -g on the free DOFs. The RHS is negated in place.
     axpy_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, -2.0, S->g.p, S->g.p);

@@ -1,0 +1,351 @@
+"""Learned Newton warm start in a linear-modal subspace (no counterpart in the
+reference, whose backward_euler.py:85 always starts Newton from the inertial
+target x~).
+
+A mesh whose fixed vertices are either pinned or driven by a uniform
+displacement d (3-vector) of a face is described by
+
+    x = X + L d + Phi q (+ a remainder M-orthogonal to Phi),
+
+with Phi (3nv x k) the k lowest linear modes of the mesh with every fixed DOF
+eliminated (linear_modal_analysis conventions: linear elasticity with
+mu = lam = 1, lumped mass with rho = 1, K + 1e-6 M) and L (3nv x 3) the static
+linear-elastic responses to a unit displacement of the driven face along
+each axis (the "lift", so that X + L d meets the boundary values).
+
+The model is an MLP (simkit_amd.mlp, BF16 matrix cores) that maps
+[q_curr, q_prev, 12 boundary-motion parameters] to the correction
+dq = q_next - q~, where q~ are the modal coordinates of the plain guess
+x~ = 2 x_curr - x_prev. The warm start is x~ + Phi dq: the plain guess with
+its modal part replaced by the prediction. sk_sim_step_guess takes it only
+when its incremental potential is lower than x~'s, and Newton still stops at
+||g|| <= rtol ||g(x~)||, so the answer meets the same tolerance either way.
+
+The 12 parameters of a top-face motion d(t) = A sin(2 pi f t) (per axis) at
+the step's end time t are [A (3), f (3), sin(2 pi f t) (3), cos(2 pi f t) (3)].
+"""
+
+import json
+import time
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._device import dev, per_element, ptr, stream
+from ._mesh import mesh_for_rest
+from .linear_modal_analysis import modal_basis
+from .mlp import MLP
+from .sim import BackwardEulerSim
+
+
+def _project(basis, weight, x, base, out):
+    _lib.call("sk_dense_project", basis.shape[0], basis.shape[1], ptr(basis), ptr(weight), ptr(x), ptr(base),
+              ptr(out), stream())
+    return out
+
+
+def _expand(basis, coef, base, out):
+    _lib.call("sk_dense_expand", basis.shape[0], basis.shape[1], ptr(basis), ptr(coef), ptr(base), ptr(out),
+              stream())
+    return out
+
+
+def _axpby(a, x, b, y, out):
+    _lib.call("sk_axpby", x.numel(), float(a), ptr(x), float(b), ptr(y), ptr(out), stream())
+    return out
+
+
+class ModalCoordinates:
+    """Phi, L and the projections q = Phi^T M (x - X - L d) for one mesh."""
+
+    def __init__(self, X, T, pinned, driven, k=64, tol=1e-6, seed=0, lift_rtol=1e-10):
+        X = np.asarray(X, np.float64)
+        self.X_np = X
+        self.nv = X.shape[0]
+        self.k = int(k)
+        pinned = np.asarray(pinned, np.int64).reshape(-1)
+        driven = np.asarray(driven, np.int64).reshape(-1)
+        fixed_v = np.concatenate([pinned, driven])
+        mesh, vol = mesh_for_rest(X, T)
+        self.mass = mesh.lumped_mass(vol, 1.0).repeat_interleave(3).contiguous()
+        f = np.zeros(3 * self.nv, np.uint8)
+        for c in range(3):
+            f[3 * fixed_v + c] = 1
+        ones = per_element(1.0, mesh.nt)
+        self.X = dev(X.reshape(-1))
+        t0 = time.perf_counter()
+        self.Phi, self.eigenvalues, self.modal_info = modal_basis(
+            mesh, self.X, ones, ones, vol, self.mass, dev(f, torch.uint8), self.k, shift=1e-6, tol=tol, seed=seed)
+        torch.cuda.synchronize()
+        self.t_modes = time.perf_counter() - t0
+        # lift: static linear-elastic response to a unit displacement of the driven face per axis
+        # (one implicit step with a negligible mass term, h = 1e4).
+        t0 = time.perf_counter()
+        stat = BackwardEulerSim(X, T, 1.0, 1.0, rho=1.0, h=1e4, material="linear", fixed_vertices=fixed_v,
+                                psd=False, pcg_rtol=lift_rtol, newton_rtol=1e-8)
+        cols = []
+        self.lift_iters = []
+        for a in range(3):
+            xbc = X.copy()
+            xbc[driven, a] += 1.0
+            x1, info = stat.step(self.X, self.X, dev(xbc.reshape(-1)))
+            if info["status"] != 0:
+                raise RuntimeError(f"lift solve failed: {info}")
+            self.lift_iters.append(int(info["pcg_iters"]))
+            cols.append((x1 - self.X).cpu().numpy())
+        self.L = dev(np.stack(cols, axis=1))
+        torch.cuda.synchronize()
+        self.t_lift = time.perf_counter() - t0
+        self._d = torch.zeros(3, dtype=torch.float64, device=self.X.device)
+
+    def lift(self, d, out=None):
+        """X + L d (device 3nv), d a host 3-vector."""
+        self._d.copy_(torch.as_tensor(np.asarray(d, np.float64)))
+        out = torch.empty_like(self.X) if out is None else out
+        return _expand(self.L, self._d, self.X, out)
+
+    def project(self, x, base, out=None):
+        """Phi^T M (x - base) (device k)."""
+        out = torch.empty(self.k, dtype=torch.float64, device=x.device) if out is None else out
+        return _project(self.Phi, self.mass, x, base, out)
+
+    def add_modes(self, base, q, out=None):
+        """base + Phi q."""
+        out = torch.empty_like(base) if out is None else out
+        return _expand(self.Phi, q, base, out)
+
+
+def motion_params(A, f, t):
+    """The 12 boundary-motion parameters at time t."""
+    A = np.asarray(A, np.float64)
+    f = np.asarray(f, np.float64)
+    ph = 2.0 * np.pi * f * t
+    return np.concatenate([A, f, np.sin(ph), np.cos(ph)])
+
+
+def motion_d(A, f, t):
+    return np.asarray(A, np.float64) * np.sin(2.0 * np.pi * np.asarray(f, np.float64) * t)
+
+
+class WarmStartModel:
+    """The MLP and its input/output normalization."""
+
+    def __init__(self, k, hidden=(256, 256, 256), seed=0, max_batch=256):
+        self.k = int(k)
+        self.dims = [2 * self.k + 12, *[int(h) for h in hidden], self.k]
+        self.net = MLP(self.dims, max_batch=max_batch, seed=seed)
+        dev_ = torch.device("cuda")
+        self.in_mu = torch.zeros(self.dims[0], dtype=torch.float64, device=dev_)
+        self.in_sd = torch.ones(self.dims[0], dtype=torch.float64, device=dev_)
+        self.out_mu = torch.zeros(self.k, dtype=torch.float64, device=dev_)
+        self.out_sd = torch.ones(self.k, dtype=torch.float64, device=dev_)
+        self._x32 = torch.empty((1, self.dims[0]), dtype=torch.float32, device=dev_)
+        self._y32 = torch.empty((1, self.k), dtype=torch.float32, device=dev_)
+
+    def set_normalization(self, F, D):
+        self.in_mu = F.mean(0)
+        self.in_sd = F.std(0).clamp_min(1e-12)
+        self.out_mu = D.mean(0)
+        self.out_sd = D.std(0).clamp_min(1e-12)
+
+    def fit(self, F, D, epochs=200, batch=256, lr=1e-3, lr_decay=0.98, seed=0, log=None):
+        """F (n, 2k+12) features, D (n, k) targets dq, device float64; normalizes and trains."""
+        self.set_normalization(F, D)
+        Xn = ((F - self.in_mu) / self.in_sd).float().contiguous()
+        Tn = ((D - self.out_mu) / self.out_sd).float().contiguous()
+        return self.net.fit(Xn, Tn, epochs=epochs, batch=batch, lr=lr, seed=seed, lr_decay=lr_decay, log=log)
+
+    def predict(self, feat, out=None):
+        """dq (device k, float64) for one feature vector (device 2k+12, float64)."""
+        self._x32[0].copy_((feat - self.in_mu) / self.in_sd)
+        self.net.forward(self._x32, out=self._y32)
+        y = self._y32[0].double() * self.out_sd + self.out_mu
+        if out is not None:
+            out.copy_(y)
+            return out
+        return y
+
+    def state_dict(self):
+        return {"dims": np.array(self.dims), "params": self.net.get_params(),
+                "in_mu": self.in_mu.cpu().numpy(), "in_sd": self.in_sd.cpu().numpy(),
+                "out_mu": self.out_mu.cpu().numpy(), "out_sd": self.out_sd.cpu().numpy()}
+
+    def load_state_dict(self, d):
+        assert list(d["dims"]) == self.dims
+        self.net.set_params(d["params"])
+        for name in ("in_mu", "in_sd", "out_mu", "out_sd"):
+            setattr(self, name, dev(np.asarray(d[name], np.float64)))
+
+
+class Rollout:
+    """Steps a BackwardEulerSim through a driven-face sinusoid, tracking modal
+    coordinates; with a model, every step is warm-started by it."""
+
+    def __init__(self, sim, coords, model=None):
+        self.sim, self.c, self.model = sim, coords, model
+        n3 = 3 * coords.nv
+        z = lambda: torch.empty(n3, dtype=torch.float64, device="cuda")
+        self.xt, self.guess, self.lift_next = z(), z(), z()
+        self.qt = torch.empty(coords.k, dtype=torch.float64, device="cuda")
+        self.feat = torch.empty(2 * coords.k + 12, dtype=torch.float64, device="cuda")
+
+    def run(self, A, f, steps, record=False, keep_last=False):
+        c, sim = self.c, self.sim
+        h = sim.h
+        x_prev = c.X.clone()
+        x_curr = c.X.clone()
+        q_prev = torch.zeros(c.k, dtype=torch.float64, device="cuda")
+        q_curr = torch.zeros_like(q_prev)
+        feats, targets, infos = [], [], []
+        for n in range(steps):
+            t = (n + 1) * h
+            d = motion_d(A, f, t)
+            c.lift(d, out=self.lift_next)                      # also the boundary values
+            self.feat[:c.k].copy_(q_curr)
+            self.feat[c.k:2 * c.k].copy_(q_prev)
+            self.feat[2 * c.k:].copy_(torch.as_tensor(motion_params(A, f, t)))
+            if self.model is not None or record:
+                _axpby(2.0, x_curr, -1.0, x_prev, self.xt)
+                c.project(self.xt, self.lift_next, out=self.qt)
+            if self.model is not None:
+                dq = self.model.predict(self.feat)
+                c.add_modes(self.xt, dq, out=self.guess)
+                x_next, info = sim.step_guess(x_curr, x_prev, self.guess, self.lift_next)
+            else:
+                x_next, info = sim.step(x_curr, x_prev, self.lift_next)
+            if info["status"] != 0:
+                raise RuntimeError(f"step {n} failed: {info}")
+            q_next = c.project(x_next, self.lift_next)
+            if record:
+                feats.append(self.feat.clone())
+                targets.append(q_next - self.qt)
+            infos.append(info)
+            x_prev, x_curr = x_curr, x_next
+            q_prev, q_curr = q_curr, q_next
+        out = {"infos": infos}
+        if record:
+            out["F"] = torch.stack(feats)
+            out["D"] = torch.stack(targets)
+        if keep_last:
+            out["x"] = x_curr
+        return out
+
+
+def random_motion(rng):
+    return rng.uniform(0.0, 0.15, 3), rng.uniform(0.5, 3.0, 3)
+
+
+def cube_problem(n, E=1e6, nu=0.45, rho=1000.0, h=1.0 / 60.0, precision="mixed", k=64, seed=0, **sim_kw):
+    """n^3-cell unit cube, bottom pinned, top face driven; returns (sim, coords)."""
+    from .mesh import unit_cube
+    from .ympr_to_lame import ympr_to_lame
+    X, T = unit_cube(n)
+    bottom = np.where(X[:, 1] == 0.0)[0]
+    top = np.where(X[:, 1] == 1.0)[0]
+    mu, lam = ympr_to_lame(E, nu)
+    sim = BackwardEulerSim(X, T, float(np.asarray(mu).reshape(-1)[0]), float(np.asarray(lam).reshape(-1)[0]),
+                           rho=rho, h=h, fixed_vertices=np.concatenate([bottom, top]), precision=precision,
+                           **sim_kw)
+    coords = ModalCoordinates(X, T, bottom, top, k=k, seed=seed)
+    return sim, coords
+
+
+def generate(sim, coords, episodes, steps, seed, log=None):
+    """Solver-generated training data: `episodes` random top-face sinusoids from rest."""
+    rng = np.random.default_rng(seed)
+    ro = Rollout(sim, coords)
+    F, D = [], []
+    for ep in range(episodes):
+        A, f = random_motion(rng)
+        out = ro.run(A, f, steps, record=True)
+        F.append(out["F"])
+        D.append(out["D"])
+        if log:
+            its = sum(i["newton_iters"] for i in out["infos"])
+            log(f"episode {ep + 1}/{episodes}: A={np.round(A, 3).tolist()} f={np.round(f, 2).tolist()} "
+                f"newton {its}")
+    return torch.cat(F), torch.cat(D)
+
+
+def evaluate(sim, coords, model, episodes, steps, seed):
+    """Held-out episodes run plain and warm-started; Newton/PCG counts, wall time and agreement."""
+    rng = np.random.default_rng(seed)
+    motions = [random_motion(rng) for _ in range(episodes)]
+    res = {}
+    finals = {}
+    for name, mdl in (("plain", None), ("learned", model)):
+        ro = Rollout(sim, coords, mdl)
+        ro.run(*motions[0], 2)                                 # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        infos, xs = [], []
+        for A, f in motions:
+            out = ro.run(A, f, steps, keep_last=True)
+            infos += out["infos"]
+            xs.append(out["x"].cpu().numpy())
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        n = len(infos)
+        res[name] = {"steps": n, "seconds": dt, "steps_per_s": n / dt,
+                     "newton_per_step": sum(i["newton_iters"] for i in infos) / n,
+                     "pcg_per_step": sum(i["pcg_iters"] for i in infos) / n,
+                     "ls_per_step": sum(i["ls_trials"] for i in infos) / n}
+        if name == "learned":
+            used = [i for i in infos if i["guess_used"]]
+            res[name]["guess_accepted"] = len(used) / n
+            # gradient norm at the taken guess relative to the plain start x~ (geometric mean)
+            res[name]["guess_g_ratio"] = float(np.exp(np.mean([np.log(i["guess_g_norm"] / i["g0_norm"])
+                                                               for i in used]))) if used else None
+        finals[name] = xs
+    dev_ = max(np.abs(a - b).max() / np.abs(b).max() for a, b in zip(finals["learned"], finals["plain"]))
+    res["final_position_rel_dev"] = float(dev_)
+    res["speedup"] = res["learned"]["steps_per_s"] / res["plain"]["steps_per_s"]
+    res["newton_reduction"] = res["plain"]["newton_per_step"] / res["learned"]["newton_per_step"]
+    return res
+
+
+def main(argv=None):
+    import argparse
+    ap = argparse.ArgumentParser(description="Train and evaluate the learned warm start on a driven cube.")
+    ap.add_argument("--n", type=int, default=32)
+    ap.add_argument("--k", type=int, default=64)
+    ap.add_argument("--episodes", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--eval-episodes", type=int, default=5)
+    ap.add_argument("--eval-steps", type=int, default=100)
+    ap.add_argument("--epochs", type=int, default=200)
+    ap.add_argument("--precision", default="mixed")
+    ap.add_argument("--save", default=None, help="write the trained model (npz)")
+    ap.add_argument("--json", default=None)
+    a = ap.parse_args(argv)
+    log = lambda s: print(s, flush=True)
+    t0 = time.perf_counter()
+    sim, coords = cube_problem(a.n, precision=a.precision, k=a.k)
+    log(f"modes: {coords.modal_info} ({coords.t_modes:.2f}s); lift {coords.t_lift:.2f}s pcg {coords.lift_iters}")
+    tg = time.perf_counter()
+    F, D = generate(sim, coords, a.episodes, a.steps, seed=1, log=log)
+    tg = time.perf_counter() - tg
+    log(f"generated {F.shape[0]} steps in {tg:.1f}s")
+    model = WarmStartModel(a.k)
+    tt = time.perf_counter()
+    losses = model.fit(F, D, epochs=a.epochs, log=lambda ep, l: (ep % 20 == 19) and log(f"epoch {ep + 1} loss {l:.4e}"))
+    torch.cuda.synchronize()
+    tt = time.perf_counter() - tt
+    log(f"trained in {tt:.1f}s; loss {losses[0]:.3e} -> {losses[-1]:.3e}")
+    res = evaluate(sim, coords, model, a.eval_episodes, a.eval_steps, seed=2)
+    res.update({"n": a.n, "k": a.k, "train_steps": int(F.shape[0]), "gen_seconds": tg, "train_seconds": tt,
+                "epochs": a.epochs, "loss_first": losses[0], "loss_last": losses[-1],
+                "modes": coords.modal_info, "modes_seconds": coords.t_modes, "lift_seconds": coords.t_lift,
+                "total_seconds": time.perf_counter() - t0, "precision": a.precision})
+    log(json.dumps(res))
+    if a.json:
+        with open(a.json, "w") as fh:
+            json.dump(res, fh, indent=1)
+    if a.save:
+        np.savez(a.save, **model.state_dict())
+    return res
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,88 @@
+"""Batched steps (sk_batch_*): every problem of a batch of uneven cantilever
+beams against the same beam advanced alone by the single-mesh simulator
+(mixed precision and FP64) and by the oracle's FP64 harness, per kept step
+at 1e-6 relative (max |x - x_ref| / max |x_ref| of that beam); every
+per-problem kernel shape, with and without the hipGraph replay; bitwise
+determinism; the autotuner and its cache."""
+
+import json
+
+import numpy as np
+import pytest
+
+from oracle import fem, solver
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-6
+
+
+def _beams(seed, lengths):
+    from simkit_amd.mesh import beam
+    rng = np.random.default_rng(seed)
+    meshes, fixed = [], []
+    for L in lengths:
+        X, T = beam(int(L), 4, 0.025)
+        meshes.append((X, T))
+        fixed.append(np.where(X[:, 0] == 0.0)[0])
+    E = np.exp(rng.uniform(np.log(1e5), np.log(1e7), len(lengths)))
+    nu = rng.uniform(0.30, 0.48, len(lengths))
+    mu, lam = fem.ympr_to_lame(E, nu)
+    return meshes, fixed, mu, lam
+
+
+def _batch(meshes, fixed, mu, lam, **kw):
+    from simkit_amd.batch import BatchSim
+    return BatchSim(meshes, mu, lam, rho=1000.0, fixed_vertices=fixed, gravity=-9.8, **kw)
+
+
+@pytest.mark.parametrize("block,lanes,graph", [(256, 1, True), (128, 4, True), (1024, 4, False), (512, 1, False)])
+def test_batch_matches_single_mesh_sims(block, lanes, graph):
+    from simkit_amd.sim import BackwardEulerSim
+    lengths = [8, 13, 9, 16, 11, 10]
+    meshes, fixed, mu, lam = _beams(1, lengths)
+    steps = 8
+    b = _batch(meshes, fixed, mu, lam, block=block, lanes=lanes, use_graph=graph)
+    kept, infos, rounds = b.rollout(steps)
+    assert all(i["status"] == 0 for step in infos for i in step), infos
+    assert max(rounds) == max(i["newton_iters"] for i in infos[-1])
+    for p, (X, T) in enumerate(meshes):
+        s = BackwardEulerSim(X, T, mu[p], lam[p], rho=1000.0, fixed_vertices=fixed[p], gravity=-9.8,
+                             precision="mixed")
+        ref, sinfos = s.rollout(steps)
+        sl = b.problem_slice(p)
+        for n in range(1, steps + 1):
+            err = np.abs(kept[n][sl] - ref[n]).max() / np.abs(ref[n]).max()
+            assert err < TOL, (p, n, err)
+
+
+def test_batch_one_beam_matches_oracle():
+    lengths = [8, 12]
+    meshes, fixed, mu, lam = _beams(2, lengths)
+    steps = 5
+    b = _batch(meshes, fixed, mu, lam)
+    kept, infos, _ = b.rollout(steps)
+    for p in range(2):
+        X, T = meshes[p]
+        ref = solver.BEProblem(X, T, mu[p], lam[p], 1000.0, 1 / 60, fixed[p], gravity=-9.8).rollout(steps)
+        sl = b.problem_slice(p)
+        for n in range(steps):
+            err = np.abs(kept[n + 1][sl] - ref[n]).max() / np.abs(ref[n]).max()
+            assert err < TOL, (p, n, err)
+
+
+def test_batch_deterministic_and_autotune(tmp_path):
+    meshes, fixed, mu, lam = _beams(3, [8, 20, 9, 14])
+    b = _batch(meshes, fixed, mu, lam)
+    cache = tmp_path / "tune.json"
+    tuned = b.autotune(reps=1, cache=str(cache))
+    assert tuned["block"] in (128, 256, 512, 1024) and tuned["lanes"] in (1, 4)
+    assert all(v > 0 for v in tuned["ms"].values())
+    table = json.loads(cache.read_text())
+    assert len(table) == 1 and list(table.values())[0]["block"] == tuned["block"]
+    k1, i1, r1 = b.rollout(4)
+    k2, i2, r2 = b.rollout(4)
+    assert all(np.array_equal(k1[n], k2[n]) for n in k1) and i1 == i2 and r1 == r2
+    b2 = _batch(meshes, fixed, mu, lam)
+    assert b2.autotune(cache=str(cache)) == tuned      # served from the cache
+    k3, _, _ = b2.rollout(4)
+    assert all(np.array_equal(k1[n], k3[n]) for n in k1)
