@@ -229,6 +229,7 @@ typedef struct sk_step_info {
   int guess_used;        /* sk_sim_step_guess: 1 if Newton started from the supplied guess */
   double guess_dE;       /* sk_sim_step_guess: E(x_guess) - E(x~), FP64 */
   double guess_g_norm;   /* sk_sim_step_guess: ||g_free(x_guess)|| when the guess was taken, else 0 */
+  int stalled;           /* 1: stopped because the update fell below position resolution (see sk_sim_step) */
 } sk_step_info;
 
 /* mu, lam, vol: per element; mass: per DOF (3nv, M = kron(M_lumped, I3));
@@ -242,6 +243,11 @@ int sk_sim_destroy(sk_sim* sim);
 /* One backward-Euler step: Newton on E(x) = V(x) + 1/(2h^2)|x - x~|_M^2 - f_ext.x
  * from x0 = x~ = x_curr + h (x_curr - x_prev)/h, fixed DOFs held at x_bc
  * (or at x_curr when x_bc is NULL). Writes x_next; info is a host struct.
+ * Newton stops when ||g_free|| <= newton_rtol ||g_free(x0)||, or when the
+ * accepted update no longer moves the positions (max |t dx| <= 4 eps max |x|
+ * over the free DOFs; info->stalled = 1): then x0 was already an equilibrium
+ * to within the resolution of the positions (a damped mesh at rest) and the
+ * relative test cannot be met in floating point.
  * Replaces backward_euler.py:27-91 -> newton.py:7-75 -> backtracking_line_search.py:8-67. */
 int sk_sim_step(sk_sim* sim, const double* x_curr, const double* x_prev, const double* x_bc, double* x_next,
                 sk_step_info* info, sk_stream_t stream);
@@ -300,6 +306,7 @@ typedef struct sk_batch_info {
   int refine_steps;
   double g0_norm;
   double g_norm;
+  int stalled;           /* as in sk_step_info */
 } sk_batch_info;
 
 /* mesh: the union of the problems' meshes, problem p owning vertices

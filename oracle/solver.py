@@ -122,6 +122,10 @@ class BEProblem:
     free DOFs of x (backward_euler.py:76-91 with the kinetic terms of
     kinetic.py:107-194 and the gravity load of gravity_force.py:37-41).
     integrator="bdf2" uses c = 9/4 and the BDF2 target (bdf2.py:86-101).
+    Newton stops at ||g_free|| <= newton_rtol ||g_free(x0)||, or when an accepted
+    update is below the position resolution (max |t dx| <= 4 eps max |x|): x0
+    was then already an equilibrium to within roundoff and the relative test
+    cannot be met (newton.py:69 likewise stops on the step length).
     """
 
     def __init__(self, X, T, mu, lam, rho, h, fixed_vertices, material="snh",
@@ -183,12 +187,14 @@ class BEProblem:
         else:
             x[self.fixed] = x_curr[self.fixed]
         fr = self.free
-        info = {"newton": 0, "ls": 0, "pcg": 0, "status": "ok"}
+        info = {"newton": 0, "ls": 0, "pcg": 0, "status": "ok", "stalled": False}
         g = self.potential_gradient(x, xt)
         g0 = np.linalg.norm(g[fr])
+        stalled = False
         for _ in range(self.newton_max_iter):
             gn = np.linalg.norm(g[fr])
-            if gn <= self.newton_rtol * g0 or gn == 0.0:
+            if gn <= self.newton_rtol * g0 or gn == 0.0 or stalled:
+                info["stalled"] = bool(stalled and gn > self.newton_rtol * g0)
                 break
             H = self.potential_hessian(x)
             dx = np.zeros_like(x)
@@ -212,6 +218,8 @@ class BEProblem:
                 info["status"] = "line_search_failed"
                 break
             x = x + t * dx
+            # an update below the position resolution (4 eps max |x|) cannot improve x further
+            stalled = np.abs(t * dx[fr]).max() <= 4.0 * np.finfo(float).eps * np.abs(x[fr]).max()
             g = self.potential_gradient(x, xt)
         else:
             info["status"] = "newton_max_iter"

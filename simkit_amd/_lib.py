@@ -89,6 +89,7 @@ class SkStepInfo(ctypes.Structure):
         ("guess_used", ctypes.c_int),
         ("guess_dE", ctypes.c_double),
         ("guess_g_norm", ctypes.c_double),
+        ("stalled", ctypes.c_int),
     ]
 
     def as_dict(self):
@@ -124,6 +125,7 @@ class SkBatchInfo(ctypes.Structure):
         ("refine_steps", ctypes.c_int),
         ("g0_norm", ctypes.c_double),
         ("g_norm", ctypes.c_double),
+        ("stalled", ctypes.c_int),
     ]
 
     def as_dict(self):

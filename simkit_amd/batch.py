@@ -197,10 +197,13 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--compare-serial", type=int, default=0,
+                    help="also advance the first K beams one at a time with the single-mesh simulator")
     ap.add_argument("--json", default=None)
     a = ap.parse_args(argv)
     t0 = time.perf_counter()
     sim, desc = cantilever_batch(a.n, use_graph=not a.no_graph)
+    meshes_desc = desc
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t0
     print(f"batch: {desc['problems']} beams, {desc['vertices']} vertices, {desc['tets']} tets ({t_setup:.1f}s setup)",
@@ -226,6 +229,26 @@ def main(argv=None):
            "pcg_per_beam_step": float(np.mean([i["pcg_iters"] for i in flat])),
            "max_newton": int(max(i["newton_iters"] for i in flat)),
            "statuses": sorted(set(i["status"] for i in flat))}
+    if a.compare_serial:
+        from .mesh import beam
+        from .sim import BackwardEulerSim
+        from .ympr_to_lame import ympr_to_lame
+        tot, nsteps = 0.0, 0
+        for p in range(a.compare_serial):
+            X, T = beam(int(meshes_desc["lengths"][p]), 4, meshes_desc["cell"])
+            mu, lam = ympr_to_lame(meshes_desc["E"][p], meshes_desc["nu"][p])
+            s1 = BackwardEulerSim(X, T, mu, lam, rho=1000.0, fixed_vertices=np.where(X[:, 0] == 0.0)[0],
+                                  gravity=-9.8, precision="mixed")
+            s1.rollout(1, keep=set())
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            s1.rollout(a.steps, keep=set())
+            torch.cuda.synchronize()
+            tot += time.perf_counter() - t2
+            nsteps += a.steps
+        res["serial_sample_beams"] = a.compare_serial
+        res["serial_beam_steps_per_s"] = nsteps / tot
+        res["batch_speedup_vs_serial"] = res["beam_steps_per_s"] / res["serial_beam_steps_per_s"]
     res.pop("lengths"), res.pop("E"), res.pop("nu")
     print(json.dumps(res), flush=True)
     if a.json:

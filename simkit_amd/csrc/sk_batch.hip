@@ -66,6 +66,7 @@ struct BatchArgs {
   int* refines;
   double* g0;
   double* gnorm;
+  int* stalled;            // 1: the last accepted update was below position resolution
   double* tstep;
   int* n_active;
   // parameters
@@ -98,6 +99,30 @@ __device__ __forceinline__ void all_sum(double (&v)[NV], double* lds) {
 #pragma unroll
     for (int w = 0; w < nw; ++w) s += lds[k * nw + w];
     v[k] = s;
+  }
+}
+
+// Every thread receives the NV maxima (non-negative inputs).
+template <int NT, int NV>
+__device__ __forceinline__ void all_max(double (&v)[NV], double* lds) {
+  constexpr int nw = NT / kWave;
+  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < NV; ++k)
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v[k] = fmax(v[k], __shfl_down(v[k], off, kWave));
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) lds[k * nw + wid] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double m = 0.0;
+#pragma unroll
+    for (int w = 0; w < nw; ++w) m = fmax(m, lds[k * nw + w]);
+    v[k] = m;
   }
 }
 
@@ -347,7 +372,7 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
 // ---------------------------------------------------------------------------
 template <class M, int NT>
 __global__ __launch_bounds__(NT) void batch_ls_kernel(BatchArgs a) {
-  __shared__ double lds[2 * (NT / kWave)];
+  __shared__ double lds[2 * (NT / kWave)];  // all_max<NT, 2> uses 2 * NT / 64
   for (int p = blockIdx.x; p < a.P; p += gridDim.x) {
     if (!block_read(a.active + p)) continue;
     const int64_t v0 = a.vstart[p], v1 = a.vstart[p + 1];
@@ -391,8 +416,19 @@ __global__ __launch_bounds__(NT) void batch_ls_kernel(BatchArgs a) {
       }
       t *= a.ls_beta;
     }
+    // does the accepted update still move the positions?
+    double mx[2] = {0.0, 0.0};
+    if (accepted)
+      for (int64_t k = 3 * v0 + threadIdx.x; k < 3 * v1; k += NT)
+        if (!is_fixed(a.fixed, k)) {
+          const double step = t * a.dx[k];
+          mx[0] = fmax(mx[0], fabs(step));
+          mx[1] = fmax(mx[1], fabs(a.x[k] + step));
+        }
+    all_max<NT, 2>(mx, lds);
     __syncthreads();
     if (threadIdx.x == 0) {
+      a.stalled[p] = (accepted && mx[0] <= kStepFloor * mx[1]) ? 1 : 0;
       a.ls_trials[p] += trials;
       a.newton_iters[p] += 1;
       a.tstep[p] = accepted ? t : 0.0;
@@ -435,11 +471,12 @@ __global__ __launch_bounds__(NT) void batch_gnorm_kernel(BatchArgs a, int mode) 
         a.ls_trials[p] = 0;
         a.refines[p] = 0;
         a.status[p] = isfinite(gn) ? SK_OK : SK_ERR_NONFINITE;
+        a.stalled[p] = 0;
         a.active[p] = (isfinite(gn) && gn > 0.0) ? 1 : 0;
       } else if (!isfinite(gn)) {
         a.status[p] = SK_ERR_NONFINITE;
         a.active[p] = 0;
-      } else if (gn <= a.newton_rtol * a.g0[p]) {
+      } else if (gn <= a.newton_rtol * a.g0[p] || a.stalled[p]) {
         a.active[p] = 0;
       } else if (a.newton_iters[p] >= a.newton_max_iter) {
         a.status[p] = SK_ERR_NEWTON_MAXIT;
@@ -483,7 +520,7 @@ struct sk_batch {
   DevBuf<uint8_t> fixed;
   DevBuf<double> x, xt, g, dx, r64;
   DevBuf<float> vals32, binv32, r32, p32, Ap32, x32;
-  DevBuf<int> active, status, newton_iters, ls_trials, refines, n_active;
+  DevBuf<int> active, status, newton_iters, ls_trials, refines, stalled, n_active;
   DevBuf<long long> pcg_iters;
   DevBuf<double> g0, gnorm, tstep;
   int* n_active_host = nullptr;
@@ -531,6 +568,7 @@ BatchArgs sk_batch::args() const {
   a.refines = refines.p;
   a.g0 = g0.p;
   a.gnorm = gnorm.p;
+  a.stalled = stalled.p;
   a.tstep = tstep.p;
   a.n_active = n_active.p;
   a.pcg_rtol2 = prm.pcg_rtol * prm.pcg_rtol;
@@ -716,7 +754,7 @@ int sk_batch_create(sk_mesh* mesh, int nprob, const int64_t* vstart, const int64
     for (DevBuf<float>* b : {&B->r32, &B->p32, &B->Ap32, &B->x32}) SK_TRY(b->alloc(n3));
     SK_TRY(B->vals32.alloc(9 * (size_t)mesh->nblk));
     SK_TRY(B->binv32.alloc(9 * (size_t)mesh->nv));
-    for (DevBuf<int>* b : {&B->active, &B->status, &B->newton_iters, &B->ls_trials, &B->refines}) {
+    for (DevBuf<int>* b : {&B->active, &B->status, &B->newton_iters, &B->ls_trials, &B->refines, &B->stalled}) {
       SK_TRY(b->alloc(nprob));
       SK_HIP_CHECK(hipMemset(b->p, 0, nprob * sizeof(int)));
     }
@@ -775,6 +813,7 @@ int sk_batch_step(sk_batch* B, const double* x_curr, const double* x_prev, const
     std::vector<int> st(B->P), ni(B->P), ls(B->P), rf(B->P);
     std::vector<long long> pc(B->P);
     std::vector<double> g0(B->P), gn(B->P);
+    std::vector<int> sl(B->P);
     SK_HIP_CHECK(hipMemcpyAsync(st.data(), B->status.p, B->P * sizeof(int), hipMemcpyDeviceToHost, B->st));
     SK_HIP_CHECK(hipMemcpyAsync(ni.data(), B->newton_iters.p, B->P * sizeof(int), hipMemcpyDeviceToHost, B->st));
     SK_HIP_CHECK(hipMemcpyAsync(ls.data(), B->ls_trials.p, B->P * sizeof(int), hipMemcpyDeviceToHost, B->st));
@@ -782,16 +821,19 @@ int sk_batch_step(sk_batch* B, const double* x_curr, const double* x_prev, const
     SK_HIP_CHECK(hipMemcpyAsync(pc.data(), B->pcg_iters.p, B->P * sizeof(long long), hipMemcpyDeviceToHost, B->st));
     SK_HIP_CHECK(hipMemcpyAsync(g0.data(), B->g0.p, B->P * sizeof(double), hipMemcpyDeviceToHost, B->st));
     SK_HIP_CHECK(hipMemcpyAsync(gn.data(), B->gnorm.p, B->P * sizeof(double), hipMemcpyDeviceToHost, B->st));
+    SK_HIP_CHECK(hipMemcpyAsync(sl.data(), B->stalled.p, B->P * sizeof(int), hipMemcpyDeviceToHost, B->st));
     SK_HIP_CHECK(hipStreamSynchronize(B->st));
     for (int q = 0; q < B->P; ++q) {
-      info[q].status = (st[q] == SK_OK && gn[q] > B->prm.newton_rtol * g0[q] && g0[q] > 0.0) ? SK_ERR_NEWTON_MAXIT
-                                                                                              : st[q];
+      const bool rel = gn[q] <= B->prm.newton_rtol * g0[q] || g0[q] == 0.0;
+      const bool conv = rel || sl[q];
+      info[q].status = (st[q] == SK_OK && !conv) ? SK_ERR_NEWTON_MAXIT : st[q];
       info[q].newton_iters = ni[q];
       info[q].pcg_iters = pc[q];
       info[q].ls_trials = ls[q];
       info[q].refine_steps = rf[q];
       info[q].g0_norm = g0[q];
       info[q].g_norm = gn[q];
+      info[q].stalled = (st[q] == SK_OK && !rel && sl[q]) ? 1 : 0;
     }
   }
   if (rounds) *rounds = r;

@@ -10,6 +10,15 @@
 
 namespace sk {
 
+// Newton's relative-gradient test ||g|| <= rtol ||g(x0)|| cannot be met once
+// the accepted update no longer changes the positions: when
+// max |t dx| <= kStepFloor max |x| over the free DOFs (4 machine epsilons, the
+// spacing of doubles near max |x|), further iterations only re-round x, and
+// the iterate is converged to working precision. This only binds when x0 is
+// already an equilibrium to within that resolution (a damped mesh at rest),
+// where ||g(x0)|| itself is tiny. newton.py:69 stops on the step length too.
+constexpr double kStepFloor = 4.0 * 2.220446049250313e-16;
+
 // Whether a material reads lam (every density except ARAP).
 constexpr bool material_needs_lam(int material) { return material != SK_ARAP; }
 

@@ -497,6 +497,64 @@ __global__ void guess_dx_kernel(int64_t n, const double* guess, const double* x,
   dx[k] = (fixed && fixed[k]) ? 0.0 : guess[k] - x[k];
 }
 
+// Largest of each of NV values over a block (fixed tree), result in thread 0.
+template <int NV>
+__device__ __forceinline__ void block_max(double (&v)[NV]) {
+  __shared__ double lds[NV * kBlock / kWave];
+  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < NV; ++k)
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v[k] = fmax(v[k], __shfl_down(v[k], off, kWave));
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) lds[k * (kBlock / kWave) + wid] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double m = 0.0;
+      for (int w = 0; w < kBlock / kWave; ++w) m = fmax(m, lds[k * (kBlock / kWave) + w]);
+      v[k] = m;
+    }
+}
+
+// x += t dx; per-block maxima of |t dx| and |x| (after the update) over the
+// free DOFs into partials[k * gridDim.x + block].
+__global__ __launch_bounds__(kBlock) void update_absmax_kernel(int64_t n, double t, const double* dx,
+                                                               const uint8_t* fixed, double* x, double* partials) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double v[2] = {0.0, 0.0};
+  if (k < n) {
+    const double step = t * dx[k];
+    const double xn = x[k] + step;
+    x[k] = xn;
+    if (!(fixed && fixed[k])) {
+      v[0] = fabs(step);
+      v[1] = fabs(xn);
+    }
+  }
+  block_max<2>(v);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = v[0];
+    partials[gridDim.x + blockIdx.x] = v[1];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void finalize_max2_kernel(const double* partials, int n, double* out) {
+  double v[2] = {0.0, 0.0};
+  for (int i = threadIdx.x; i < n; i += kBlock) {
+    v[0] = fmax(v[0], partials[i]);
+    v[1] = fmax(v[1], partials[n + i]);
+  }
+  block_max<2>(v);
+  if (threadIdx.x == 0) {
+    out[0] = v[0];
+    out[1] = v[1];
+  }
+}
+
 __global__ void axpy_kernel(int64_t n, double t, const double* dx, double* x) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) x[k] = x[k] + t * dx[k];
@@ -842,6 +900,7 @@ int step_core(sk_sim* S, double c_mass, const double* x_guess, double* x_next, s
   SK_TRY(read_scalars(m, m->red.result.p, 1, st));
   const double g0 = std::sqrt(m->pinned[0]);
   double gn = g0;
+  bool stalled = false;
   inf.g0_norm = g0;
   if (!std::isfinite(g0)) inf.status = SK_ERR_NONFINITE;
   if (x_guess && inf.status == SK_OK) {
@@ -864,7 +923,7 @@ int step_core(sk_sim* S, double c_mass, const double* x_guess, double* x_next, s
 
   bool converged = false;
   while (inf.status == SK_OK) {
-    if (gn <= P.newton_rtol * g0 || gn == 0.0) {
+    if (gn <= P.newton_rtol * g0 || gn == 0.0 || stalled) {
       converged = true;
       break;
     }
@@ -912,13 +971,16 @@ int step_core(sk_sim* S, double c_mass, const double* x_guess, double* x_next, s
       inf.status = SK_ERR_LS_FAIL;
       break;
     }
-    axpy_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, t, S->dx.p, S->x.p);
+    // x += t dx, with the largest |t dx| and |x| over the free DOFs into result[2..3]
+    update_absmax_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, t, S->dx.p, fixed, S->x.p, m->red.partials.p);
+    finalize_max2_kernel<<<1, kBlock, 0, st>>>(m->red.partials.p, grid_for(n3), m->red.result.p + 2);
     if (P.timing) SK_HIP_CHECK(hipEventRecord(S->ev[3], st));
     SK_TRY(launch_elem_grad(m, P.material, S->x.p, nullptr, S->mu.p, lam, S->vol.p, st));
     SK_TRY(launch_gather_grad(m, S->g.p, S->mass.p, S->x.p, S->xt.p, fext, fixed, inv_h2, st));
     if (P.timing) SK_HIP_CHECK(hipEventRecord(S->ev[4], st));
-    SK_TRY(read_scalars(m, m->red.result.p, 1, st));
+    SK_TRY(read_scalars(m, m->red.result.p, 4, st));
     gn = std::sqrt(m->pinned[0]);
+    stalled = m->pinned[2] <= kStepFloor * m->pinned[3];
     if (!std::isfinite(gn)) inf.status = SK_ERR_NONFINITE;
     if (P.timing) {
       inf.ms_hessian += elapsed(S->ev[0], S->ev[1]);
@@ -929,6 +991,7 @@ int step_core(sk_sim* S, double c_mass, const double* x_guess, double* x_next, s
   }
   if (inf.status == SK_OK && !converged) inf.status = SK_ERR_NEWTON_MAXIT;
   inf.g_norm = gn;
+  inf.stalled = (converged && stalled && !(gn <= P.newton_rtol * g0)) ? 1 : 0;
   SK_HIP_CHECK(hipMemcpyAsync(x_next, S->x.p, n3 * sizeof(double), hipMemcpyDeviceToDevice, st));
   if (info) *info = inf;
   return SK_OK;

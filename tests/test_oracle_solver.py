@@ -81,3 +81,17 @@ def test_be_cube_matches_reference_trajectory(ref_be_cube10):
     for s in (1, 2):
         ref = ref_be_cube10[f"x_{s}"]
         assert np.abs(states[s - 1] - ref).max() <= 1e-6 * np.abs(ref).max()
+
+
+def test_harness_stops_when_updates_fall_below_position_resolution():
+    """A stiff short cantilever damps to rest under gravity; from then on ||g(x~)|| is tiny and the
+    relative test would demand sub-roundoff gradients. The harness must stop once an accepted
+    update no longer moves x (4 eps max |x|) instead of running to the iteration cap."""
+    from simkit_amd.mesh import beam
+    X, T = beam(8, 4, 0.025)
+    mu, lam = fem.ympr_to_lame(6.22e6, 0.479)
+    p = solver.BEProblem(X, T, mu, lam, 1000.0, 1 / 60, np.where(X[:, 0] == 0)[0], gravity=-9.8,
+                         newton_max_iter=40)
+    _, infos = p.rollout(12, return_info=True)
+    assert all(i["status"] == "ok" for i in infos)
+    assert any(i["stalled"] for i in infos) and max(i["newton"] for i in infos) <= 6
