@@ -142,6 +142,108 @@ def run_path(args, X, T, bottom, top, mu, lam, precision):
     return res
 
 
+def learned_leg(cells, steps, warmup):
+    """Learned warm start + FP32/FP64-refinement path on the same trajectory as the headline, with the
+    committed model for this mesh (trained on random top-face sinusoids, so this trajectory is held
+    out). None when no model exists for `cells`."""
+    import torch
+    from simkit_amd import learned
+    model = learned.load_model(cells)
+    if model is None:
+        return None
+    t0 = time.perf_counter()
+    sim, coords = learned.cube_problem(cells, precision="mixed", k=model.k)
+    setup = time.perf_counter() - t0
+    A, f = np.array([0.1, 0.0, 0.0]), np.array([1.0, 1.0, 1.0])   # d(t) = (0.1 sin 2 pi t, 0, 0)
+    ro = learned.Rollout(sim, coords, model)
+    ro.run(A, f, warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = ro.run(A, f, steps, keep_last=True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    infos = out["infos"]
+    used = [i for i in infos if i["guess_used"]]
+    res = {"steps_per_s": steps / dt, "ms_per_step": 1e3 * dt / steps,
+           "newton_iters": sum(i["newton_iters"] for i in infos), "pcg_iters": sum(i["pcg_iters"] for i in infos),
+           "guess_accepted": len(used) / len(infos),
+           "guess_g_ratio": float(np.exp(np.mean([np.log(i["guess_g_norm"] / i["g0_norm"]) for i in used])))
+           if used else None,
+           "setup_s_modes_and_lift": round(setup, 2), "modes": coords.modal_info,
+           "x_final": out["x"].cpu().numpy()}
+    log(f"bench[learned]: {dt:.2f}s, {res['steps_per_s']:.3f} steps/s, newton {res['newton_iters']}, "
+        f"pcg {res['pcg_iters']}, guess taken {res['guess_accepted']:.2f}")
+    return res
+
+
+def warmstart_eval_leg(cells, episodes, steps):
+    """Learned warm start on its own workload: held-out random sinusoids of the driven cube, plain
+    vs warm-started, with the committed model."""
+    from simkit_amd import learned
+    model = learned.load_model(cells)
+    if model is None:
+        return None
+    sim, coords = learned.cube_problem(cells, precision="mixed", k=model.k)
+    res = learned.evaluate(sim, coords, model, episodes, steps, seed=2)
+    log(f"bench[warmstart{cells}]: speedup {res['speedup']:.3f}, newton/step {res['plain']['newton_per_step']:.2f}"
+        f" -> {res['learned']['newton_per_step']:.2f}")
+    return {"workload": f"driven cube {cells}^3 cells, random top-face sinusoids (held out), stable neo-Hookean",
+            **res}
+
+
+def batch_leg(n, steps):
+    """Batched solve of n cantilever beams, one launch per Newton phase."""
+    import torch
+    from simkit_amd.batch import cantilever_batch
+    sim, desc = cantilever_batch(n)
+    tuned = sim.autotune()
+    x = sim.X_dev.clone()
+    sim.step(x, x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, infos, rounds = sim.rollout(steps, keep=set())
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    flat = [i for st in infos for i in st]
+    res = {"workload": f"{n} cantilever beams (4x4 cells x U{{8..64}}, cell {desc['cell']} m), stable neo-Hookean,"
+                       f" FP32 + FP64 refinement, gravity", "beams": n, "vertices": desc["vertices"],
+           "tets": desc["tets"], "steps": steps, "batch_steps_per_s": round(steps / dt, 4),
+           "beam_steps_per_s": round(steps * n / dt, 1), "newton_rounds_per_step": float(np.mean(rounds)),
+           "newton_per_beam_step": float(np.mean([i["newton_iters"] for i in flat])),
+           "statuses": sorted(set(int(i["status"]) for i in flat)), "autotune": tuned, "graph": True}
+    log(f"bench[batch]: {dt:.2f}s, {res['beam_steps_per_s']} beam-steps/s")
+    return res
+
+
+def pmc_probe(args):
+    """Short run for hardware-counter passes: a 256 MiB device-to-device copy (known traffic, the
+    calibration kernel) and a few SpMV launches of each path on the assembled 48^3 system. Prints
+    the algorithmic bytes the counters are compared with."""
+    import torch
+    from simkit_amd import _lib
+    from simkit_amd.sim import BackwardEulerSim, top_face_sinusoid
+    from simkit_amd.ympr_to_lame import ympr_to_lame
+    X, T, bottom, top = build_workload(args.cells)
+    mu, lam = ympr_to_lame(1e6, 0.45)
+    nbytes = 256 << 20
+    src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    info = {"copy_bytes_read": nbytes, "copy_bytes_written": nbytes}
+    for prec in args.paths.split(","):
+        sim = BackwardEulerSim(X, T, mu, lam, rho=1000.0, fixed_vertices=np.concatenate([bottom, top]),
+                               precision=prec)
+        x = sim.X_dev.clone()
+        sim.step(x, x, top_face_sinusoid(X, top, 0.1, 1.0)(0, sim.h))
+        _lib.call("sk_sim_spmv_reps", sim._h, 5, torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        info[f"spmv_{prec}_algorithmic_bytes"] = spmv_bytes(sim.nv, sim.mesh.nblocks, 8 if prec == "fp64" else 4)
+        del sim
+    print(json.dumps(info), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,6 +255,13 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--inner-rtol", type=float, default=1e-2)
     ap.add_argument("--paths", default="fp64,mixed", help="comma list of fp64, mixed")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start legs")
+    ap.add_argument("--warmstart-episodes", type=int, default=2)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-beams leg")
+    ap.add_argument("--batch-beams", type=int, default=512)
+    ap.add_argument("--batch-steps", type=int, default=30)
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="only run the short hardware-counter probe (copy calibration + SpMV launches)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -163,6 +272,9 @@ def main():
     from simkit_amd.ympr_to_lame import ympr_to_lame
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.pmc_probe:
+        pmc_probe(args)
+        return
     X, T, bottom, top = build_workload(args.cells)
     mu, lam = ympr_to_lame(1e6, 0.45)
     log(f"bench: {args.cells}^3 cube, {X.shape[0]} vertices, {T.shape[0]} tets")
@@ -210,6 +322,21 @@ def main():
                      "bytes_per_launch": int(mx["spmv_bytes"]), "achieved_GBs": round(ach, 1),
                      "frac": round(ach / HBM_PEAK_GBS, 4)},
         }
+    if not args.no_learned and head_name == "fp64":
+        lr = learned_leg(args.cells, args.steps, args.warmup)
+        if lr is not None:
+            dev = float(np.abs(lr["x_final"] - head["x_final"]).max() / np.abs(head["x_final"]).max())
+            lr.pop("x_final")
+            out["learned_fp32"] = {"value": round(lr["steps_per_s"], 4),
+                                   "speedup_vs_fp64": round(lr["steps_per_s"] / head["steps_per_s"], 3),
+                                   "speedup_vs_fp32_refined": round(lr["steps_per_s"] / results["mixed"]["steps_per_s"], 3)
+                                   if "mixed" in results else None,
+                                   "final_position_rel_dev_vs_fp64": dev, **lr}
+        ws = warmstart_eval_leg(32, args.warmstart_episodes, 100)
+        if ws is not None:
+            out["learned_warm_start_32"] = ws
+    if not args.no_batch:
+        out["batched_beams"] = batch_leg(args.batch_beams, args.batch_steps)
     if not args.no_cpu_baseline:
         log("bench: CPU baseline sample")
         out["cpu_baseline"] = cpu_baseline(X, T, bottom, top, mu, lam, head["infos"], args.steps,

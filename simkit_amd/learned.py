@@ -26,6 +26,7 @@ the step's end time t are [A (3), f (3), sin(2 pi f t) (3), cos(2 pi f t) (3)].
 """
 
 import json
+import os
 import time
 
 import numpy as np
@@ -230,6 +231,21 @@ class Rollout:
         if keep_last:
             out["x"] = x_curr
         return out
+
+
+MODELS_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "models")
+
+
+def load_model(cells, k=64):
+    """The committed model for the n^3 driven cube (trained by `python -m simkit_amd.learned --n <cells>
+    ... --save`), or None."""
+    path = os.path.join(MODELS_DIR, f"warmstart_cube{cells}.npz")
+    if not os.path.exists(path):
+        return None
+    d = np.load(path, allow_pickle=False)
+    model = WarmStartModel(k, hidden=tuple(int(h) for h in d["dims"][1:-1]))
+    model.load_state_dict(d)
+    return model
 
 
 def random_motion(rng):
