@@ -51,6 +51,21 @@ def spmv_bytes(nv, nblk, value_bytes):
     return 9 * nblk * value_bytes + 4 * nblk + 4 * n3 * value_bytes + n3
 
 
+def pmc_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed hardware-counter summary
+    (profiles/pmc_spmv_cube48.md: FETCH_SIZE and WRITE_SIZE passes scaled by a copy of known size),
+    or None when absent."""
+    path = os.path.join(ROOT, "profiles", "pmc_spmv_cube48.md")
+    if not os.path.exists(path):
+        return None
+    text = open(path).read()
+    try:
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        return int(json.loads(block)[kernel]["traffic_bytes"])
+    except (IndexError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(X, T, bottom, top, mu, lam, gpu_infos, steps, pcg_sample):
     """Time the CPU oracle (numpy/scipy FP64) on a bounded sample of the same workload and
     compose the per-step cost with the GPU run's own Newton / PCG / line-search counts."""
@@ -305,7 +320,9 @@ def main():
         },
         "roofline": {"bound": "hbm", "kernel": "bpcg_spmv_kernel<double>" if head_name == "fp64"
                      else "bpcg_spmv_kernel<float>", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": pmc_traffic("bpcg_spmv_kernel<double>" if head_name == "fp64"
+                                            else "bpcg_spmv_kernel<float>") if args.cells == 48 else None,
                      "bytes_per_launch": int(head["spmv_bytes"]), "avg_launch_ms": round(head["spmv_ms"], 5)},
     }
     if "mixed" in results and head_name == "fp64":
@@ -320,7 +337,8 @@ def main():
             "phase_ms_total": mx["phase_ms_total"],
             "spmv": {"kernel": "bpcg_spmv_kernel<float>", "avg_launch_ms": round(mx["spmv_ms"], 5),
                      "bytes_per_launch": int(mx["spmv_bytes"]), "achieved_GBs": round(ach, 1),
-                     "frac": round(ach / HBM_PEAK_GBS, 4)},
+                     "frac": round(ach / HBM_PEAK_GBS, 4),
+                     "traffic": pmc_traffic("bpcg_spmv_kernel<float>") if args.cells == 48 else None},
         }
     if not args.no_learned and head_name == "fp64":
         lr = learned_leg(args.cells, args.steps, args.warmup)

@@ -596,7 +596,8 @@ int sk_dense_project(int64_t rows, int k, const double* basis, const double* wei
   SK_REQUIRE(rows >= 0 && k >= 1 && k <= kBlock, "sk_dense_project: need rows >= 0 and 1 <= k <= 256");
   SK_REQUIRE(basis && x && q, "sk_dense_project: null pointer");
   hipStream_t st = reinterpret_cast<hipStream_t>(s);
-  const int64_t rpb = 4096;
+  // about 1024 workgroups (each row lane then walks a few dozen rows), at least 64 rows each
+  const int64_t rpb = std::max<int64_t>(64, (rows + 1023) / 1024);
   const int nb = (int)std::max<int64_t>(1, (rows + rpb - 1) / rpb);
   DevBuf<double> partials;
   SK_TRY(partials.alloc((size_t)nb * k));
