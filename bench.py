@@ -27,6 +27,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# dominant kernel of each path (the FP32 SpMV walks chunk ranges with 1024 workgroups)
+SPMV_KERNEL = {"fp64": "bpcg_spmv_kernel<double, false>", "mixed": "bpcg_spmv_kernel<float, true>"}
 METRIC = "BE steps/s, 48^3-cell neo-Hookean (353k DOF), learned+FP32 vs plain FP64"
 
 
@@ -51,17 +53,17 @@ def spmv_bytes(nv, nblk, value_bytes):
     return 9 * nblk * value_bytes + 4 * nblk + 4 * n3 * value_bytes + n3
 
 
-def pmc_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed hardware-counter summary
-    (profiles/pmc_spmv_cube48.md: FETCH_SIZE and WRITE_SIZE passes scaled by a copy of known size),
-    or None when absent."""
+def pmc_traffic(path_key):
+    """HBM bytes per SpMV launch of one path ("spmv_fp64" / "spmv_fp32") from the committed
+    hardware-counter summary (profiles/pmc_spmv_cube48.md: FETCH_SIZE and WRITE_SIZE passes scaled
+    by a copy of known size), or None when absent."""
     path = os.path.join(ROOT, "profiles", "pmc_spmv_cube48.md")
     if not os.path.exists(path):
         return None
     text = open(path).read()
     try:
         block = text.split("```json", 1)[1].split("```", 1)[0]
-        return int(json.loads(block)[kernel]["traffic_bytes"])
+        return int(json.loads(block)[path_key]["traffic_bytes"])
     except (IndexError, KeyError, ValueError):
         return None
 
@@ -318,11 +320,11 @@ def main():
             "newton_iters": head["newton_iters"], "pcg_iters": head["pcg_iters"], "ls_trials": head["ls_trials"],
             "phase_ms_total": head["phase_ms_total"],
         },
-        "roofline": {"bound": "hbm", "kernel": "bpcg_spmv_kernel<double>" if head_name == "fp64"
-                     else "bpcg_spmv_kernel<float>", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": SPMV_KERNEL[head_name], "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": pmc_traffic("bpcg_spmv_kernel<double>" if head_name == "fp64"
-                                            else "bpcg_spmv_kernel<float>") if args.cells == 48 else None,
+                     "traffic": pmc_traffic(f"spmv_{head_name}" if head_name == "fp64" else "spmv_fp32")
+                     if args.cells == 48 else None,
                      "bytes_per_launch": int(head["spmv_bytes"]), "avg_launch_ms": round(head["spmv_ms"], 5)},
     }
     if "mixed" in results and head_name == "fp64":
@@ -335,10 +337,10 @@ def main():
             "final_position_rel_dev_vs_fp64": dev, "inner_rtol": args.inner_rtol,
             "newton_iters": mx["newton_iters"], "pcg_iters": mx["pcg_iters"], "refine_steps": mx["refine_steps"],
             "phase_ms_total": mx["phase_ms_total"],
-            "spmv": {"kernel": "bpcg_spmv_kernel<float>", "avg_launch_ms": round(mx["spmv_ms"], 5),
+            "spmv": {"kernel": SPMV_KERNEL["mixed"], "avg_launch_ms": round(mx["spmv_ms"], 5),
                      "bytes_per_launch": int(mx["spmv_bytes"]), "achieved_GBs": round(ach, 1),
                      "frac": round(ach / HBM_PEAK_GBS, 4),
-                     "traffic": pmc_traffic("bpcg_spmv_kernel<float>") if args.cells == 48 else None},
+                     "traffic": pmc_traffic("spmv_fp32") if args.cells == 48 else None},
         }
     if not args.no_learned and head_name == "fp64":
         lr = learned_leg(args.cells, args.steps, args.warmup)

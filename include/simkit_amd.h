@@ -211,7 +211,9 @@ typedef struct sk_sim_params {
                             correction is accumulated in FP64 and the FP64 residual b - A x replaces the
                             FP32 one (CG continues with its search direction) */
   int max_refine;        /* SK_PREC_MIXED: residual replacements per Newton solve at most */
-  int spmv_blocks;       /* SpMV launch width (workgroups); 0 = default */
+  int spmv_blocks;       /* SpMV launch width cap (workgroups walk chunk ranges); 0 = default (FP64: one
+                            chunk per wave, FP32: 1024) */
+  int update_blocks;     /* launch width cap of the vertex-wise PCG kernels; 0 = one lane per vertex */
 } sk_sim_params;
 
 typedef struct sk_step_info {

@@ -70,6 +70,7 @@ class SkSimParams(ctypes.Structure):
         ("inner_rtol", ctypes.c_double),
         ("max_refine", ctypes.c_int),
         ("spmv_blocks", ctypes.c_int),
+        ("update_blocks", ctypes.c_int),
     ]
 
 

@@ -151,9 +151,8 @@ __device__ __forceinline__ void apply_binv(const T* __restrict__ binv, int64_t v
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void bpcg_init_kernel(BlockPcgArgs<T> a) {
-  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double rz = 0.0, rr = 0.0;
-  if (v < a.nv) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < a.nv; v += (int64_t)gridDim.x * blockDim.x) {
     T r[3], z[3];
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
@@ -184,9 +183,8 @@ __global__ __launch_bounds__(kBlock) void bpcg_init_kernel(BlockPcgArgs<T> a) {
 // the correction accumulator x restarts from 0, the search direction is kept.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void bpcg_restart_kernel(BlockPcgArgs<T> a) {
-  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double rz = 0.0, rr = 0.0;
-  if (v < a.nv) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < a.nv; v += (int64_t)gridDim.x * blockDim.x) {
     T r[3], z[3];
 #pragma unroll
     for (int p = 0; p < 3; ++p) r[p] = a.r[3 * v + p];
@@ -203,6 +201,19 @@ __global__ __launch_bounds__(kBlock) void bpcg_restart_kernel(BlockPcgArgs<T> a)
   double vals[2] = {rz, rr};
   store_block_partials<2>(vals, a.upart);
   if (blockIdx.x == 0 && threadIdx.x == 0) a.st->done = 0;
+}
+
+// SpMV workgroups walk a contiguous range of `per` chunk groups (kWaves
+// chunks each, one per wave); consecutive logical workgroups share an XCD.
+__device__ __forceinline__ int64_t chunks_per_block(int64_t nchunk) {
+  constexpr int kWaves = kBlock / kWave;
+  const int64_t groups = (nchunk + kWaves - 1) / kWaves;
+  return (groups + gridDim.x - 1) / gridDim.x;
+}
+
+__device__ __forceinline__ int64_t block_chunk(int wid, int64_t q, int64_t per) {
+  constexpr int kWaves = kBlock / kWave;
+  return ((int64_t)xcd_block(blockIdx.x, gridDim.x) * per + q) * kWaves + wid;
 }
 
 // SELL chunk of this wave: one chunk of kSellC rows per wave, kSellLanes
@@ -242,35 +253,52 @@ __device__ __forceinline__ void sell_row_product(const SellView& sv, const T* __
 // SpMV of PCG iteration k on the SELL matrix: forms p = z + beta p_prev on
 // the fly for every column touched, writes the new p and A p of its rows and
 // a partial of p.Ap.
+// One SELL chunk's rows of the PCG SpMV (see bpcg_spmv_kernel).
 template <typename T>
+__device__ __forceinline__ void spmv_chunk(const BlockPcgArgs<T>& a, T beta, int64_t ch, int lane, double& dot) {
+  const int i = lane % kSellC, j = lane / kSellC;
+  T y[3] = {T(0), T(0), T(0)};
+  const T* __restrict__ z = a.z;
+  const T* __restrict__ pin = a.pin;
+  sell_row_product<T, T>(a.sv, a.vals, ch, i, j, [&](int64_t w, T (&pv)[3]) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) pv[q] = z[w + q] + beta * pin[w + q];
+  }, y);
+  const int32_t row = a.sv.sell_row[ch * kSellC + i];
+  if (row >= 0 && j == 0) {
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+      const int64_t k = 3 * (int64_t)row + p;
+      const T pn = z[k] + beta * pin[k];
+      const T ap = (a.fixed && a.fixed[k]) ? T(0) : y[p];
+      a.pout[k] = pn;
+      a.Ap[k] = ap;
+      dot += (double)pn * (double)ap;
+    }
+  }
+}
+
+// SpMV of PCG iteration k on the SELL matrix: forms p = z + beta p_prev on
+// the fly for every column touched, writes the new p and A p of its rows and
+// a partial of p.Ap. kRange = false: one chunk per wave (the grid covers every
+// chunk); true: each workgroup walks a contiguous range of chunk groups.
+template <typename T, bool kRange>
 __global__ __launch_bounds__(kBlock) void bpcg_spmv_kernel(BlockPcgArgs<T> a) {
   const PcgCtl c = spmv_prologue(a.st, a.upart, a.gu, a.k, a.rtol2, a.restart);
   if (!c.run) return;
   const T beta = (T)c.beta;
   const int lane = threadIdx.x % kWave;
-  const int64_t ch = sell_chunk(threadIdx.x / kWave);
   double dot = 0.0;
-  if (ch < a.sv.nchunk) {
-    const int i = lane % kSellC, j = lane / kSellC;
-    T y[3] = {T(0), T(0), T(0)};
-    const T* __restrict__ z = a.z;
-    const T* __restrict__ pin = a.pin;
-    sell_row_product<T, T>(a.sv, a.vals, ch, i, j, [&](int64_t w, T (&pv)[3]) {
-#pragma unroll
-      for (int q = 0; q < 3; ++q) pv[q] = z[w + q] + beta * pin[w + q];
-    }, y);
-    const int32_t row = a.sv.sell_row[ch * kSellC + i];
-    if (row >= 0 && j == 0) {
-#pragma unroll
-      for (int p = 0; p < 3; ++p) {
-        const int64_t k = 3 * (int64_t)row + p;
-        const T pn = z[k] + beta * pin[k];
-        const T ap = (a.fixed && a.fixed[k]) ? T(0) : y[p];
-        a.pout[k] = pn;
-        a.Ap[k] = ap;
-        dot += (double)pn * (double)ap;
-      }
+  if (kRange) {
+    const int64_t per = chunks_per_block(a.sv.nchunk);
+    for (int64_t q = 0; q < per; ++q) {
+      const int64_t ch = block_chunk(threadIdx.x / kWave, q, per);
+      if (ch >= a.sv.nchunk) break;
+      spmv_chunk(a, beta, ch, lane, dot);
     }
+  } else {
+    const int64_t ch = sell_chunk(threadIdx.x / kWave);
+    if (ch < a.sv.nchunk) spmv_chunk(a, beta, ch, lane, dot);
   }
   double vals[1] = {dot};
   store_block_partials<1>(vals, a.spart);
@@ -281,9 +309,8 @@ __global__ __launch_bounds__(kBlock) void bpcg_update_kernel(BlockPcgArgs<T> a) 
   double alpha_d;
   if (!update_prologue(a.st, a.spart, a.gs, a.k, alpha_d)) return;
   const T alpha = (T)alpha_d;
-  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double rz = 0.0, rr = 0.0;
-  if (v < a.nv) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < a.nv; v += (int64_t)gridDim.x * blockDim.x) {
     T r[3], z[3];
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
@@ -574,13 +601,15 @@ struct sk_sim {
   sk_mesh* mesh = nullptr;
   sk_sim_params prm{};
   int spmv_blocks = 0;
+  int update_blocks = 0;
   DevBuf<double> mu, lam, vol, mass, fext;
   DevBuf<uint8_t> fixed;
   DevBuf<double> x, xt, g, dx, r, z, p0, p1, Ap, vals, binv;
   // mixed precision: FP32 matrix, preconditioner and inner-PCG vectors
   DevBuf<float> vals32, binv32, x32, r32, z32, p0_32, p1_32, Ap32;
   DevBuf<PcgState> st;
-  PcgState* st_host = nullptr;
+  PcgState* st_host = nullptr;  // 2 pinned slots (pipelined chunk checks)
+  hipEvent_t chunk_ev[2] = {};
   hipEvent_t ev[6] = {};
   bool have_events = false;
   bool mixed() const { return prm.precision == SK_PREC_MIXED; }
@@ -597,6 +626,7 @@ int copy_dev(DevBuf<double>& dst, const double* src, size_t n) {
 }
 
 constexpr int kPcgChunk = 16;
+constexpr int kSpmvBlocks32 = 1024;
 
 int spmv_grid(int64_t rows) {
   const int64_t g = (rows + (kBlock / kRowLanes) - 1) / (kBlock / kRowLanes);
@@ -633,11 +663,22 @@ PcgBufs<float> bufs32(sk_sim* S) {
 template <typename T>
 BlockPcgArgs<T> block_pcg_args(sk_sim* S, const PcgBufs<T>& B, const T* b, double rtol2) {
   sk_mesh* m = S->mesh;
-  const int gu = grid_for(m->nv);
-  const int gs = sell_grid(m->nchunk);
+  // Launch widths (measured on the 48^3 cube): the vertex-wise kernels run one
+  // lane per vertex unless update_blocks caps them (they stride over the
+  // vertices); the SpMV runs one chunk per wave in FP64 and walks contiguous
+  // chunk ranges with 1024 workgroups in FP32, unless spmv_blocks caps it.
+  const int gu = S->update_blocks > 0 ? std::min(grid_for(m->nv), S->update_blocks) : grid_for(m->nv);
+  const int cap = S->spmv_blocks > 0 ? S->spmv_blocks : (sizeof(T) == 8 ? 0 : kSpmvBlocks32);
+  const int gs = cap > 0 ? std::min(sell_grid(m->nchunk), (cap + 7) / 8 * 8) : sell_grid(m->nchunk);
   double* part = m->red.partials.p;
   return BlockPcgArgs<T>{m->nv, sell_view(m), B.vals, B.binv, S->fixed.p, b, B.x, B.r, B.z, B.p1, B.p0, B.Ap,
                          S->st.p, rtol2, part, gu, part + 2 * gu, gs, 0, 0};
+}
+
+template <typename T>
+void launch_spmv(const BlockPcgArgs<T>& a, int64_t nchunk, hipStream_t st) {
+  if (a.gs >= sell_grid(nchunk)) bpcg_spmv_kernel<T, false><<<a.gs, kBlock, 0, st>>>(a);
+  else bpcg_spmv_kernel<T, true><<<a.gs, kBlock, 0, st>>>(a);
 }
 
 // Block-Jacobi PCG on the assembled system with right-hand side b (x0 = 0).
@@ -658,25 +699,34 @@ int run_block_pcg(sk_sim* S, const PcgBufs<T>& B, const T* b, double rtol, int64
     bpcg_restart_kernel<T><<<a.gu, kBlock, 0, st>>>(a);
   }
   SK_HIP_CHECK(hipGetLastError());
+  // Chunks of kPcgChunk iterations; the state copied after chunk c is checked
+  // only once chunk c + 1 has been enqueued, so the GPU is not idle while the
+  // host reads it. Launches past convergence exit in their prologue.
   int64_t launched = k0;
-  for (;;) {
+  int c = 0;
+  for (bool stop = false; !stop; ++c) {
     for (int k = 0; k < kPcgChunk; ++k, ++launched) {
       a.k = (int)launched;
       a.restart = (k0 > 0 && launched == k0) ? 1 : 0;
       a.pin = (launched & 1) ? B.p1 : B.p0;
       a.pout = (launched & 1) ? B.p0 : B.p1;
-      bpcg_spmv_kernel<T><<<a.gs, kBlock, 0, st>>>(a);
+      launch_spmv<T>(a, S->mesh->nchunk, st);
       bpcg_update_kernel<T><<<a.gu, kBlock, 0, st>>>(a);
     }
     SK_HIP_CHECK(hipGetLastError());
-    SK_HIP_CHECK(hipMemcpyAsync(S->st_host, S->st.p, sizeof(PcgState), hipMemcpyDeviceToHost, st));
-    SK_HIP_CHECK(hipStreamSynchronize(st));
-    if (S->st_host->done || S->st_host->iter >= max_iter) break;
-    if (launched > max_iter + kPcgChunk) break;  // device state not advancing
-    if (k0 > 0 && S->st_host->iter < k0) break;   // continuation did not start
+    SK_HIP_CHECK(hipMemcpyAsync(S->st_host + (c & 1), S->st.p, sizeof(PcgState), hipMemcpyDeviceToHost, st));
+    SK_HIP_CHECK(hipEventRecord(S->chunk_ev[c & 1], st));
+    if (c > 0) {
+      SK_HIP_CHECK(hipEventSynchronize(S->chunk_ev[(c - 1) & 1]));
+      const PcgState& h = S->st_host[(c - 1) & 1];
+      if (h.done || h.iter >= max_iter) stop = true;
+      if (launched > max_iter + 2 * kPcgChunk) stop = true;  // device state not advancing
+      if (k0 > 0 && h.iter < k0) stop = true;                 // continuation did not start
+    }
   }
-  *iters = S->st_host->iter;
-  *done = S->st_host->done;
+  SK_HIP_CHECK(hipEventSynchronize(S->chunk_ev[(c - 1) & 1]));
+  *iters = S->st_host[(c - 1) & 1].iter;
+  *done = S->st_host[(c - 1) & 1].done;
   return SK_OK;
 }
 
@@ -818,11 +868,12 @@ int sk_sim_create(sk_mesh* mesh, const sk_sim_params* p, const double* mu, const
   SK_REQUIRE(p->precision == SK_PREC_FP64 || p->precision == SK_PREC_MIXED, "sk_sim_create: bad precision");
   SK_REQUIRE(p->precision == SK_PREC_FP64 || (p->inner_rtol > 0.0 && p->inner_rtol < 1.0 && p->max_refine > 0),
              "sk_sim_create: mixed precision needs 0 < inner_rtol < 1 and max_refine > 0");
-  SK_REQUIRE(p->spmv_blocks >= 0, "sk_sim_create: spmv_blocks must be >= 0");
+  SK_REQUIRE(p->spmv_blocks >= 0 && p->update_blocks >= 0, "sk_sim_create: launch widths must be >= 0");
   sk_sim* S = new sk_sim();
   S->mesh = mesh;
   S->prm = *p;
-  S->spmv_blocks = p->spmv_blocks > 0 ? p->spmv_blocks : kSpmvBlocks;
+  S->spmv_blocks = p->spmv_blocks;
+  S->update_blocks = p->update_blocks;
   const size_t nt = mesh->nt, n3 = 3 * mesh->nv;
   auto run = [&]() -> int {
     SK_TRY(copy_dev(S->mu, mu, nt));
@@ -850,8 +901,9 @@ int sk_sim_create(sk_mesh* mesh, const sk_sim_params* p, const double* mu, const
       SK_TRY(S->binv.alloc(9 * (size_t)mesh->nv));
     }
     SK_TRY(S->st.alloc(1));
-    SK_HIP_CHECK(hipHostMalloc((void**)&S->st_host, sizeof(PcgState), hipHostMallocDefault));
+    SK_HIP_CHECK(hipHostMalloc((void**)&S->st_host, 2 * sizeof(PcgState), hipHostMallocDefault));
     for (auto& e : S->ev) SK_HIP_CHECK(hipEventCreate(&e));
+    for (auto& e : S->chunk_ev) SK_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     S->have_events = true;
     return SK_OK;
   };
@@ -867,8 +919,11 @@ int sk_sim_create(sk_mesh* mesh, const sk_sim_params* p, const double* mu, const
 int sk_sim_destroy(sk_sim* S) {
   if (!S) return SK_OK;
   if (S->st_host) (void)hipHostFree(S->st_host);
-  if (S->have_events)
+  if (S->have_events) {
     for (auto& e : S->ev) (void)hipEventDestroy(e);
+    for (auto& e : S->chunk_ev)
+      if (e) (void)hipEventDestroy(e);
+  }
   delete S;
   return SK_OK;
 }
@@ -1044,7 +1099,7 @@ int sk_sim_spmv_reps(sk_sim* S, int reps, sk_stream_t s) {
     bpcg_init_kernel<float><<<a.gu, kBlock, 0, st>>>(a);
     a.pin = B.p0;
     a.pout = B.p1;
-    for (int k = 0; k < reps; ++k) bpcg_spmv_kernel<float><<<a.gs, kBlock, 0, st>>>(a);
+    for (int k = 0; k < reps; ++k) launch_spmv<float>(a, S->mesh->nchunk, st);
   } else {
     const PcgBufs<double> B = bufs64(S);
     BlockPcgArgs<double> a = block_pcg_args<double>(S, B, S->g.p, 0.0);
@@ -1052,7 +1107,7 @@ int sk_sim_spmv_reps(sk_sim* S, int reps, sk_stream_t s) {
     bpcg_init_kernel<double><<<a.gu, kBlock, 0, st>>>(a);
     a.pin = B.p0;
     a.pout = B.p1;
-    for (int k = 0; k < reps; ++k) bpcg_spmv_kernel<double><<<a.gs, kBlock, 0, st>>>(a);
+    for (int k = 0; k < reps; ++k) launch_spmv<double>(a, S->mesh->nchunk, st);
   }
   SK_HIP_CHECK(hipGetLastError());
   return SK_OK;

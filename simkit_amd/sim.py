@@ -30,7 +30,8 @@ class BackwardEulerSim:
     def __init__(self, X, T, mu, lam=None, rho=1000.0, h=1.0 / 60.0, material="snh", fixed_vertices=(),
                  gravity=0.0, psd=True, newton_rtol=1e-6, newton_max_iter=100, pcg_rtol=1e-8,
                  pcg_max_iter=100000, ls_alpha=0.01, ls_beta=0.5, ls_max_iter=100, ls_threshold=1e-12,
-                 timing=False, precision="fp64", inner_rtol=1e-2, max_refine=50, spmv_blocks=0):
+                 timing=False, precision="fp64", inner_rtol=1e-2, max_refine=50, spmv_blocks=0,
+                 update_blocks=0):
         """precision: "fp64" (every stage FP64) or "mixed" (FP32 element Hessian, PSD projection,
         assembled values and inner PCG to inner_rtol, wrapped in FP64 iterative refinement to
         pcg_rtol; gradient, line-search energies and the polar SVD of ARAP elements with
@@ -79,6 +80,7 @@ class BackwardEulerSim:
         p.inner_rtol = inner_rtol
         p.max_refine = max_refine
         p.spmv_blocks = spmv_blocks
+        p.update_blocks = update_blocks
         self.params = p
         self.precision = precision
         h_ = ctypes.c_void_p()
