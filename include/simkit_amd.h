@@ -297,6 +297,7 @@ typedef struct sk_batch_params {
   double ls_threshold;
   int block;             /* threads per workgroup of the per-problem kernels (128, 256, 512, 1024); 0 = 256 */
   int lanes;             /* lanes per matrix row in the per-problem SpMV (1 or 4); 0 = 1 */
+  int lds;               /* 1: the per-problem PCG keeps p, r and A p in LDS (when the largest problem fits) */
   int use_graph;         /* 1: one Newton iteration of the batch is captured in a hipGraph and replayed */
 } sk_batch_params;
 
@@ -326,14 +327,15 @@ int sk_batch_destroy(sk_batch* batch);
 int sk_batch_step(sk_batch* batch, const double* x_curr, const double* x_prev, const double* x_bc, double* x_next,
                   sk_batch_info* info, int* rounds, sk_stream_t stream);
 /* Sets the per-problem kernel shape (a recorded autotuning choice). */
-int sk_batch_set_shape(sk_batch* batch, int block, int lanes);
+int sk_batch_set_shape(sk_batch* batch, int block, int lanes, int lds);
 /* Times the per-problem PCG kernel on the first Newton system of a step from
- * (x_curr, x_prev, x_bc) for every (block, lanes) in {128, 256, 512, 1024} x
- * {1, 4} (reps launches each, after one warm-up) and keeps the fastest.
- * ms_out (host, 8 entries, may be NULL) receives the mean times in that
- * order. The batch state is reset by the next sk_batch_step. */
+ * (x_curr, x_prev, x_bc) for every (lds, block, lanes) in {0, 1} x {128, 256,
+ * 512, 1024} x {1, 4} (reps launches each, after one warm-up; lds = 1 only
+ * when the largest problem's vectors fit) and keeps the fastest. ms_out
+ * (host, 16 entries, may be NULL) receives the mean times in that order (-1:
+ * not run). The batch state is reset by the next sk_batch_step. */
 int sk_batch_autotune(sk_batch* batch, const double* x_curr, const double* x_prev, const double* x_bc, int reps,
-                      int* block_out, int* lanes_out, double* ms_out, sk_stream_t stream);
+                      int* block_out, int* lanes_out, int* lds_out, double* ms_out, sk_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
 /* Linear modal analysis                                                     */

@@ -113,6 +113,7 @@ class SkBatchParams(ctypes.Structure):
         ("ls_threshold", ctypes.c_double),
         ("block", ctypes.c_int),
         ("lanes", ctypes.c_int),
+        ("lds", ctypes.c_int),
         ("use_graph", ctypes.c_int),
     ]
 
@@ -184,8 +185,8 @@ SIGNATURES = {
     "sk_batch_create": [_P, _INT, _P, _P, _P, ctypes.POINTER(SkBatchParams), _P, _P, _P, _P, _P, _P, _P],
     "sk_batch_destroy": [_P],
     "sk_batch_step": [_P, _P, _P, _P, _P, _P, _P, _P],
-    "sk_batch_set_shape": [_P, _INT, _INT],
-    "sk_batch_autotune": [_P, _P, _P, _P, _INT, _P, _P, _P, _P],
+    "sk_batch_set_shape": [_P, _INT, _INT, _INT],
+    "sk_batch_autotune": [_P, _P, _P, _P, _INT, _P, _P, _P, _P, _P],
     "sk_dense_project": [_I64, _INT, _P, _P, _P, _P, _P, _P],
     "sk_dense_expand": [_I64, _INT, _P, _P, _P, _P, _P],
     "sk_axpby": [_I64, _D, _P, _D, _P, _P, _P],

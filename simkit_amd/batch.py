@@ -39,7 +39,7 @@ class BatchSim:
     def __init__(self, meshes, mu, lam, rho=1000.0, h=1.0 / 60.0, material="snh", fixed_vertices=None, gravity=0.0,
                  psd=True, newton_rtol=1e-6, newton_max_iter=100, pcg_rtol=1e-8, pcg_max_iter=100000,
                  inner_rtol=1e-2, ls_alpha=0.01, ls_beta=0.5, ls_max_iter=100, ls_threshold=1e-12, block=0, lanes=0,
-                 use_graph=True):
+                 lds=0, use_graph=True):
         """meshes: list of (X (n,3), T (t,4)); mu, lam: per problem (scalar each) or per union element;
         fixed_vertices: list (per problem, local vertex ids) or None."""
         require_gpu()
@@ -96,7 +96,7 @@ class BatchSim:
         prm.pcg_max_iter = pcg_max_iter
         prm.inner_rtol = inner_rtol
         prm.ls_alpha, prm.ls_beta, prm.ls_max_iter, prm.ls_threshold = ls_alpha, ls_beta, ls_max_iter, ls_threshold
-        prm.block, prm.lanes, prm.use_graph = block, lanes, 1 if use_graph else 0
+        prm.block, prm.lanes, prm.lds, prm.use_graph = block, lanes, lds, 1 if use_graph else 0
         self.params = prm
         Th = np.ascontiguousarray(T, dtype=np.int32)
         h_ = ctypes.c_void_p()
@@ -135,25 +135,27 @@ class BatchSim:
         return kept, infos, rounds
 
     def autotune(self, reps=3, cache=None):
-        """Pick (block, lanes) of the per-problem kernels by timing them on this batch's first Newton
-        system from rest; with a cache path, reuse / record the choice keyed by the batch's shape."""
+        """Pick (block, lanes, lds) of the per-problem kernels by timing them on this batch's first
+        Newton system from rest (lds: the PCG vectors live in LDS, tried when the largest problem
+        fits); with a cache path, reuse / record the choice keyed by the batch's shape."""
         key = f"P{self.P}_nv{self.nv}_nt{self.nt}_mat{self.material}"
         if cache and os.path.exists(cache):
             with open(cache) as fh:
                 table = json.load(fh)
             if key in table:
-                self.params.block, self.params.lanes = table[key]["block"], table[key]["lanes"]
-                self._apply_choice(self.params.block, self.params.lanes)
-                self.tuned = table[key]
+                t = table[key]
+                self._apply_choice(t["block"], t["lanes"], t.get("lds", 0))
+                self.tuned = t
                 return self.tuned
-        b, l = ctypes.c_int(), ctypes.c_int()
-        ms = (ctypes.c_double * 8)()
+        b, l, d = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        ms = (ctypes.c_double * 16)()
         x = self.X_dev
-        _lib.call("sk_batch_autotune", self._h, ptr(x), ptr(x), None, int(reps), ctypes.byref(b), ctypes.byref(l), ms,
-                  stream())
-        cands = [(nt, ln) for nt in (128, 256, 512, 1024) for ln in (1, 4)]
-        self.tuned = {"block": b.value, "lanes": l.value,
-                      "ms": {f"{nt}x{ln}": ms[i] for i, (nt, ln) in enumerate(cands)}}
+        _lib.call("sk_batch_autotune", self._h, ptr(x), ptr(x), None, int(reps), ctypes.byref(b), ctypes.byref(l),
+                  ctypes.byref(d), ms, stream())
+        cands = [(ld, nt, ln) for ld in (0, 1) for nt in (128, 256, 512, 1024) for ln in (1, 4)]
+        self.tuned = {"block": b.value, "lanes": l.value, "lds": d.value,
+                      "ms": {f"{'lds' if ld else 'global'}_{nt}x{ln}": ms[i]
+                             for i, (ld, nt, ln) in enumerate(cands) if ms[i] >= 0}}
         if cache:
             table = {}
             if os.path.exists(cache):
@@ -165,8 +167,8 @@ class BatchSim:
                 json.dump(table, fh, indent=1)
         return self.tuned
 
-    def _apply_choice(self, block, lanes):
-        _lib.call("sk_batch_set_shape", self._h, int(block), int(lanes))
+    def _apply_choice(self, block, lanes, lds):
+        _lib.call("sk_batch_set_shape", self._h, int(block), int(lanes), int(lds))
 
 
 def cantilever_batch(n=512, seed=0, cell=0.025, L_range=(8, 64), E_range=(1e5, 1e7), nu_range=(0.30, 0.48),

@@ -77,6 +77,7 @@ struct BatchArgs {
   double newton_rtol;
   int newton_max_iter;
   int count_pcg;           // 0: autotuning run, counters untouched
+  int max_rows;            // largest problem (vertices)
 };
 
 // Every thread of an NT-thread block receives the NV totals (fixed order).
@@ -151,13 +152,13 @@ __device__ __forceinline__ void row_product(const BatchArgs& a, int64_t v, const
 }
 
 template <int LANES>
-__device__ __forceinline__ void row_product_f(const BatchArgs& a, int64_t v, const float* __restrict__ src, int j,
-                                              float (&y)[3]) {
+__device__ __forceinline__ void row_product_f(const BatchArgs& a, int64_t v, const float* __restrict__ src,
+                                              int64_t off, int j, float (&y)[3]) {
   y[0] = y[1] = y[2] = 0.f;
   if (v >= 0) {
     const int32_t b1 = a.bptr[v + 1];
     for (int32_t b = a.bptr[v] + j; b < b1; b += LANES) {
-      const int64_t w = 3 * (int64_t)a.bcol[b];
+      const int64_t w = 3 * (int64_t)a.bcol[b] - off;
       const float* A = a.vals + 9 * (int64_t)b;
       const float s0 = src[w], s1 = src[w + 1], s2 = src[w + 2];
 #pragma unroll
@@ -191,9 +192,10 @@ __device__ __forceinline__ int block_read(const int* flag) {
 // Thread rows: row slot i = threadIdx.x / LANES covers vertices v0 + i,
 // v0 + i + NT / LANES, ...
 // ---------------------------------------------------------------------------
-template <int NT, int LANES>
+template <int NT, int LANES, bool KLDS>
 __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
   __shared__ double lds[4 * (NT / kWave)];
+  extern __shared__ float vec_lds[];  // KLDS: p, r, Ap of the problem (3 max_rows floats each)
   constexpr int RS = NT / LANES;  // rows per sweep
   const int j = threadIdx.x % LANES, slot = threadIdx.x / LANES;
   for (int p = blockIdx.x; p < a.P; p += gridDim.x) {
@@ -201,6 +203,11 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
     const int64_t v0 = a.vstart[p], v1 = a.vstart[p + 1];
     const int64_t nrow = v1 - v0;
     const int64_t sweeps = (nrow + RS - 1) / RS;
+    // p, r and Ap: in LDS (indexed from the problem's first DOF) or in the global vectors
+    const int64_t off = KLDS ? 3 * v0 : 0;
+    float* P = KLDS ? vec_lds : a.p32;
+    float* Rv = KLDS ? vec_lds + 3 * a.max_rows : a.r32;
+    float* AP = KLDS ? vec_lds + 6 * a.max_rows : a.Ap32;
     // init: r64 = b = -g (free), dx = 0, r32 = b, x32 = 0, z = B^-1 r, p = z
     double acc[2] = {0.0, 0.0};
     for (int64_t s = 0; s < sweeps; ++s) {
@@ -214,14 +221,14 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
           a.r64[k] = b;
           a.dx[k] = 0.0;
           r[q] = (float)b;
-          a.r32[k] = r[q];
+          Rv[k - off] = r[q];
           a.x32[k] = 0.f;
           acc[1] += b * b;
         }
         apply_binv3(a.binv + 9 * v, r[0], r[1], r[2], z);
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-          a.p32[3 * v + q] = z[q];
+          P[3 * v + q - off] = z[q];
           acc[0] += (double)r[q] * (double)z[q];
         }
       }
@@ -243,14 +250,14 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
       for (int64_t s = 0; s < sweeps; ++s) {
         const int64_t v = v0 + s * RS + slot;
         float y[3];
-        row_product_f<LANES>(a, v < v1 ? v : -1, a.p32, j, y);
+        row_product_f<LANES>(a, v < v1 ? v : -1, P, off, j, y);
         if (v < v1 && j == 0) {
 #pragma unroll
           for (int q = 0; q < 3; ++q) {
             const int64_t k = 3 * v + q;
             const float ap = is_fixed(a.fixed, k) ? 0.f : y[q];
-            a.Ap32[k] = ap;
-            d1[0] += (double)a.p32[k] * (double)ap;
+            AP[k - off] = ap;
+            d1[0] += (double)P[k - off] * (double)ap;
           }
         }
       }
@@ -268,9 +275,9 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
 #pragma unroll
           for (int q = 0; q < 3; ++q) {
             const int64_t k = 3 * v + q;
-            a.x32[k] += alpha * a.p32[k];
-            const float r = a.r32[k] - alpha * a.Ap32[k];
-            a.r32[k] = r;
+            a.x32[k] += alpha * P[k - off];
+            const float r = Rv[k - off] - alpha * AP[k - off];
+            Rv[k - off] = r;
             d2[0] += (double)r * (double)r;
           }
         }
@@ -306,7 +313,7 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
               const int64_t k = 3 * v + q;
               const double r = is_fixed(a.fixed, k) ? 0.0 : -a.g[k] - y[q];
               a.r64[k] = r;
-              a.r32[k] = (float)r;
+              Rv[k - off] = (float)r;
               d3[0] += r * r;
             }
           }
@@ -322,9 +329,10 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
         const int64_t v = v0 + s * RS + slot;
         if (v < v1 && j == 0) {
           float z[3];
-          apply_binv3(a.binv + 9 * v, a.r32[3 * v], a.r32[3 * v + 1], a.r32[3 * v + 2], z);
+          const float* rv = Rv + 3 * v - off;
+          apply_binv3(a.binv + 9 * v, rv[0], rv[1], rv[2], z);
 #pragma unroll
-          for (int q = 0; q < 3; ++q) d4[0] += (double)a.r32[3 * v + q] * (double)z[q];
+          for (int q = 0; q < 3; ++q) d4[0] += (double)rv[q] * (double)z[q];
         }
       }
       all_sum<NT, 1>(d4, lds);
@@ -334,9 +342,10 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
         const int64_t v = v0 + s * RS + slot;
         if (v < v1 && j == 0) {
           float z[3];
-          apply_binv3(a.binv + 9 * v, a.r32[3 * v], a.r32[3 * v + 1], a.r32[3 * v + 2], z);
+          const float* rv = Rv + 3 * v - off;
+          apply_binv3(a.binv + 9 * v, rv[0], rv[1], rv[2], z);
 #pragma unroll
-          for (int q = 0; q < 3; ++q) a.p32[3 * v + q] = z[q] + beta * a.p32[3 * v + q];
+          for (int q = 0; q < 3; ++q) P[3 * v + q - off] = z[q] + beta * P[3 * v + q - off];
         }
       }
       __syncthreads();
@@ -528,8 +537,9 @@ struct sk_batch {
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  int graph_block = 0, graph_lanes = 0;
+  int graph_block = 0, graph_lanes = 0, graph_lds = 0;
   int grid = 0;
+  int max_rows = 0;
   BatchArgs args() const;
 };
 
@@ -581,15 +591,27 @@ BatchArgs sk_batch::args() const {
   a.newton_rtol = prm.newton_rtol;
   a.newton_max_iter = prm.newton_max_iter;
   a.count_pcg = 1;
+  a.max_rows = max_rows;
   return a;
 }
 
 namespace {
 
-int launch_pcg(const sk_batch* B, const BatchArgs& a, int block, int lanes) {
+// Whether the problems' p, r and Ap (3 max_rows floats each) fit the LDS of one workgroup.
+size_t pcg_lds_bytes(const sk_batch* B) { return 9 * (size_t)B->max_rows * sizeof(float); }
+// 60 KiB keeps the total, with the kernel's static reduction scratch, under 64 KiB.
+bool pcg_lds_fits(const sk_batch* B) { return B->max_rows > 0 && pcg_lds_bytes(B) <= 60 * 1024; }
+
+int launch_pcg(const sk_batch* B, const BatchArgs& a, int block, int lanes, int lds) {
   const int g = B->grid;
-#define SK_PCG_CASE(NT, L) \
-  if (block == NT && lanes == L) { batch_pcg_kernel<NT, L><<<g, NT, 0, B->st>>>(a); return SK_OK; }
+  const size_t shm = lds ? pcg_lds_bytes(B) : 0;
+  SK_REQUIRE(!lds || pcg_lds_fits(B), "sk_batch: problems too large for the LDS-resident PCG");
+#define SK_PCG_CASE(NT, L)                                                 \
+  if (block == NT && lanes == L) {                                         \
+    if (lds) batch_pcg_kernel<NT, L, true><<<g, NT, shm, B->st>>>(a);      \
+    else batch_pcg_kernel<NT, L, false><<<g, NT, 0, B->st>>>(a);           \
+    return SK_OK;                                                          \
+  }
   SK_PCG_CASE(128, 1) SK_PCG_CASE(128, 4) SK_PCG_CASE(256, 1) SK_PCG_CASE(256, 4)
   SK_PCG_CASE(512, 1) SK_PCG_CASE(512, 4) SK_PCG_CASE(1024, 1) SK_PCG_CASE(1024, 4)
 #undef SK_PCG_CASE
@@ -632,11 +654,11 @@ int hessian_phase(sk_batch* B) {
 }
 
 // One Newton iteration of every active problem (no host synchronization).
-int newton_iteration(sk_batch* B, int block, int lanes) {
+int newton_iteration(sk_batch* B, int block, int lanes, int lds) {
   const BatchArgs a = B->args();
   const int64_t n3 = 3 * B->mesh->nv;
   SK_TRY(hessian_phase(B));
-  SK_TRY(launch_pcg(B, a, block, lanes));
+  SK_TRY(launch_pcg(B, a, block, lanes, lds));
   SK_TRY(launch_ls(B, a, block));
   batch_update_kernel<<<grid_for(n3), kBlock, 0, B->st>>>(n3, a);
   SK_TRY(gradient_phase(B));
@@ -665,14 +687,14 @@ int read_active(sk_batch* B, int* n) {
   return SK_OK;
 }
 
-int ensure_graph(sk_batch* B, int block, int lanes) {
-  if (B->exec && B->graph_block == block && B->graph_lanes == lanes) return SK_OK;
+int ensure_graph(sk_batch* B, int block, int lanes, int lds) {
+  if (B->exec && B->graph_block == block && B->graph_lanes == lanes && B->graph_lds == lds) return SK_OK;
   if (B->exec) (void)hipGraphExecDestroy(B->exec);
   if (B->graph) (void)hipGraphDestroy(B->graph);
   B->exec = nullptr;
   B->graph = nullptr;
   SK_HIP_CHECK(hipStreamBeginCapture(B->st, hipStreamCaptureModeThreadLocal));
-  const int s = newton_iteration(B, block, lanes);
+  const int s = newton_iteration(B, block, lanes, lds);
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(B->st, &g);
   if (s != SK_OK) {
@@ -684,11 +706,13 @@ int ensure_graph(sk_batch* B, int block, int lanes) {
   SK_HIP_CHECK(hipGraphInstantiate(&B->exec, g, nullptr, nullptr, 0));
   B->graph_block = block;
   B->graph_lanes = lanes;
+  B->graph_lds = lds;
   return SK_OK;
 }
 
 int batch_block(const sk_batch* B) { return B->prm.block > 0 ? B->prm.block : 256; }
 int batch_lanes(const sk_batch* B) { return B->prm.lanes > 0 ? B->prm.lanes : 1; }
+int batch_lds(const sk_batch* B) { return B->prm.lds && pcg_lds_fits(B) ? 1 : 0; }
 
 }  // namespace
 
@@ -710,6 +734,7 @@ int sk_batch_create(sk_mesh* mesh, int nprob, const int64_t* vstart, const int64
   SK_REQUIRE(p->block == 0 || p->block == 128 || p->block == 256 || p->block == 512 || p->block == 1024,
              "sk_batch_create: block must be 0, 128, 256, 512 or 1024");
   SK_REQUIRE(p->lanes == 0 || p->lanes == 1 || p->lanes == 4, "sk_batch_create: lanes must be 0, 1 or 4");
+  SK_REQUIRE(p->lds == 0 || p->lds == 1, "sk_batch_create: lds must be 0 or 1");
   SK_REQUIRE(vstart[0] == 0 && vstart[nprob] == mesh->nv && tstart[0] == 0 && tstart[nprob] == mesh->nt,
              "sk_batch_create: problem ranges must cover the mesh");
   std::vector<int32_t> vs(nprob + 1), ts(nprob + 1), pid(mesh->nv);
@@ -726,11 +751,23 @@ int sk_batch_create(sk_mesh* mesh, int nprob, const int64_t* vstart, const int64
         SK_REQUIRE(v >= vstart[q] && v < vstart[q + 1], "sk_batch_create: a tet references another problem");
       }
   }
+  {
+    // The per-problem SpMV reads only the problem's own rows: every block of
+    // the mesh's pattern must couple two vertices of one problem.
+    std::vector<int32_t> bp(mesh->nv + 1), bc(mesh->nblk);
+    SK_HIP_CHECK(hipMemcpy(bp.data(), mesh->bptr.p, bp.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (mesh->nblk) SK_HIP_CHECK(hipMemcpy(bc.data(), mesh->bcol.p, bc.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    for (int64_t v = 0; v < mesh->nv; ++v)
+      for (int32_t b = bp[v]; b < bp[v + 1]; ++b)
+        SK_REQUIRE(bc[b] >= 0 && bc[b] < mesh->nv && pid[bc[b]] == pid[v],
+                   "sk_batch_create: the mesh couples vertices of different problems");
+  }
   sk_batch* B = new sk_batch();
   B->mesh = mesh;
   B->P = nprob;
   B->prm = *p;
   B->grid = nprob;
+  for (int q = 0; q < nprob; ++q) B->max_rows = std::max<int>(B->max_rows, (int)(vstart[q + 1] - vstart[q]));
   const size_t nt = mesh->nt, n3 = 3 * mesh->nv;
   auto copy = [](DevBuf<double>& d, const double* s, size_t n) -> int {
     SK_TRY(d.alloc(n));
@@ -796,14 +833,14 @@ int sk_batch_step(sk_batch* B, const double* x_curr, const double* x_prev, const
   hipStream_t caller = as_stream(s);
   SK_HIP_CHECK(hipEventRecord(B->ev_in, caller));
   SK_HIP_CHECK(hipStreamWaitEvent(B->st, B->ev_in, 0));
-  const int block = batch_block(B), lanes = batch_lanes(B);
+  const int block = batch_block(B), lanes = batch_lanes(B), lds = batch_lds(B);
   SK_TRY(start_step(B, x_curr, x_prev, x_bc));
   int n_active = 0, r = 0;
   SK_TRY(read_active(B, &n_active));
-  if (B->prm.use_graph) SK_TRY(ensure_graph(B, block, lanes));
+  if (B->prm.use_graph) SK_TRY(ensure_graph(B, block, lanes, lds));
   while (n_active > 0 && r < B->prm.newton_max_iter) {
     if (B->prm.use_graph) SK_HIP_CHECK(hipGraphLaunch(B->exec, B->st));
-    else SK_TRY(newton_iteration(B, block, lanes));
+    else SK_TRY(newton_iteration(B, block, lanes, lds));
     ++r;
     SK_TRY(read_active(B, &n_active));
   }
@@ -842,17 +879,19 @@ int sk_batch_step(sk_batch* B, const double* x_curr, const double* x_prev, const
   return SK_OK;
 }
 
-int sk_batch_set_shape(sk_batch* B, int block, int lanes) {
+int sk_batch_set_shape(sk_batch* B, int block, int lanes, int lds) {
   SK_REQUIRE(B, "sk_batch_set_shape: null batch");
   SK_REQUIRE(block == 128 || block == 256 || block == 512 || block == 1024, "sk_batch_set_shape: bad block");
   SK_REQUIRE(lanes == 1 || lanes == 4, "sk_batch_set_shape: lanes must be 1 or 4");
+  SK_REQUIRE(lds == 0 || (lds == 1 && pcg_lds_fits(B)), "sk_batch_set_shape: lds must be 0, or 1 when it fits");
   B->prm.block = block;
   B->prm.lanes = lanes;
+  B->prm.lds = lds;
   return SK_OK;
 }
 
 int sk_batch_autotune(sk_batch* B, const double* x_curr, const double* x_prev, const double* x_bc, int reps,
-                      int* block_out, int* lanes_out, double* ms_out, sk_stream_t s) {
+                      int* block_out, int* lanes_out, int* lds_out, double* ms_out, sk_stream_t s) {
   SK_REQUIRE(B && x_curr && x_prev && block_out && lanes_out, "sk_batch_autotune: null argument");
   SK_REQUIRE(reps >= 1, "sk_batch_autotune: reps must be >= 1");
   hipStream_t caller = as_stream(s);
@@ -868,30 +907,38 @@ int sk_batch_autotune(sk_batch* B, const double* x_curr, const double* x_prev, c
   SK_HIP_CHECK(hipEventCreate(&e1));
   const int blocks[4] = {128, 256, 512, 1024}, lanes[2] = {1, 4};
   double best = 1e300;
-  int bb = 256, bl = 1, c = 0;
-  for (int bi = 0; bi < 4; ++bi)
-    for (int li = 0; li < 2; ++li, ++c) {
-      SK_TRY(launch_pcg(B, a, blocks[bi], lanes[li]));  // warm-up
-      SK_HIP_CHECK(hipEventRecord(e0, B->st));
-      for (int k = 0; k < reps; ++k) SK_TRY(launch_pcg(B, a, blocks[bi], lanes[li]));
-      SK_HIP_CHECK(hipEventRecord(e1, B->st));
-      SK_HIP_CHECK(hipEventSynchronize(e1));
-      float ms = 0.f;
-      SK_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-      const double t = ms / reps;
-      if (ms_out) ms_out[c] = t;
-      if (t < best) {
-        best = t;
-        bb = blocks[bi];
-        bl = lanes[li];
+  int bb = 256, bl = 1, bd = 0, c = 0;
+  for (int ld = 0; ld < 2; ++ld)
+    for (int bi = 0; bi < 4; ++bi)
+      for (int li = 0; li < 2; ++li, ++c) {
+        if (ld && !pcg_lds_fits(B)) {
+          if (ms_out) ms_out[c] = -1.0;
+          continue;
+        }
+        SK_TRY(launch_pcg(B, a, blocks[bi], lanes[li], ld));  // warm-up
+        SK_HIP_CHECK(hipEventRecord(e0, B->st));
+        for (int k = 0; k < reps; ++k) SK_TRY(launch_pcg(B, a, blocks[bi], lanes[li], ld));
+        SK_HIP_CHECK(hipEventRecord(e1, B->st));
+        SK_HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        SK_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        const double t = ms / reps;
+        if (ms_out) ms_out[c] = t;
+        if (t < best) {
+          best = t;
+          bb = blocks[bi];
+          bl = lanes[li];
+          bd = ld;
+        }
       }
-    }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   B->prm.block = bb;
   B->prm.lanes = bl;
+  B->prm.lds = bd;
   *block_out = bb;
   *lanes_out = bl;
+  if (lds_out) *lds_out = bd;
   SK_HIP_CHECK(hipEventRecord(B->ev_out, B->st));
   SK_HIP_CHECK(hipStreamWaitEvent(caller, B->ev_out, 0));
   return SK_OK;

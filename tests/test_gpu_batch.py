@@ -2,8 +2,9 @@
 beams against the same beam advanced alone by the single-mesh simulator
 (mixed precision and FP64) and by the oracle's FP64 harness, per kept step
 at 1e-6 relative (max |x - x_ref| / max |x_ref| of that beam); every
-per-problem kernel shape, with and without the hipGraph replay; bitwise
-determinism; the autotuner and its cache."""
+per-problem kernel shape (vectors in global memory or in LDS), with and
+without the hipGraph replay; bitwise determinism; the autotuner and its
+cache."""
 
 import json
 
@@ -35,13 +36,15 @@ def _batch(meshes, fixed, mu, lam, **kw):
     return BatchSim(meshes, mu, lam, rho=1000.0, fixed_vertices=fixed, gravity=-9.8, **kw)
 
 
-@pytest.mark.parametrize("block,lanes,graph", [(256, 1, True), (128, 4, True), (1024, 4, False), (512, 1, False)])
-def test_batch_matches_single_mesh_sims(block, lanes, graph):
+@pytest.mark.parametrize("block,lanes,lds,graph", [(256, 1, 0, True), (128, 4, 0, True), (1024, 4, 0, False),
+                                                    (512, 1, 0, False), (256, 1, 1, True), (128, 4, 1, False),
+                                                    (1024, 1, 1, True), (512, 4, 1, True)])
+def test_batch_matches_single_mesh_sims(block, lanes, lds, graph):
     from simkit_amd.sim import BackwardEulerSim
     lengths = [8, 13, 9, 16, 11, 10]
     meshes, fixed, mu, lam = _beams(1, lengths)
     steps = 8
-    b = _batch(meshes, fixed, mu, lam, block=block, lanes=lanes, use_graph=graph)
+    b = _batch(meshes, fixed, mu, lam, block=block, lanes=lanes, lds=lds, use_graph=graph)
     kept, infos, rounds = b.rollout(steps)
     assert all(i["status"] == 0 for step in infos for i in step), infos
     assert max(rounds) == max(i["newton_iters"] for i in infos[-1])
@@ -75,8 +78,8 @@ def test_batch_deterministic_and_autotune(tmp_path):
     b = _batch(meshes, fixed, mu, lam)
     cache = tmp_path / "tune.json"
     tuned = b.autotune(reps=1, cache=str(cache))
-    assert tuned["block"] in (128, 256, 512, 1024) and tuned["lanes"] in (1, 4)
-    assert all(v > 0 for v in tuned["ms"].values())
+    assert tuned["block"] in (128, 256, 512, 1024) and tuned["lanes"] in (1, 4) and tuned["lds"] in (0, 1)
+    assert len(tuned["ms"]) == 16 and all(v > 0 for v in tuned["ms"].values())
     table = json.loads(cache.read_text())
     assert len(table) == 1 and list(table.values())[0]["block"] == tuned["block"]
     k1, i1, r1 = b.rollout(4)
