@@ -208,6 +208,50 @@ def warmstart_eval_leg(cells, episodes, steps):
             **res}
 
 
+def mixed_jittered_leg(cells, steps):
+    """Mixed-precision workload: the prescribed-motion cube with every interior rest vertex jittered
+    by U(-0.2, 0.2) cells per axis (seed 0; the pinned bottom and driven top faces stay planar), for
+    stable neo-Hookean and ARAP. Each energy runs `steps` steps in plain FP64 and in FP32 with FP64
+    refinement; reports the per-step relative position error max|x32 - x64| / max|x64| and the rate
+    of each path (with one state read back per step)."""
+    import torch
+    from simkit_amd.mesh import jitter
+    from simkit_amd.sim import BackwardEulerSim, top_face_sinusoid
+    from simkit_amd.ympr_to_lame import ympr_to_lame
+    X, T, bottom, top = build_workload(cells)
+    X = jitter(X, 1.0 / cells, 0.2, seed=0, keep=np.concatenate([bottom, top]))
+    e = X[T[:, 1:]] - X[T[:, :1]]
+    assert np.all(np.einsum("ij,ij->i", e[:, 0], np.cross(e[:, 1], e[:, 2])) > 0), "jitter inverted a tet"
+    mu, lam = ympr_to_lame(1e6, 0.45)
+    bc = top_face_sinusoid(X, top, 0.1, 1.0)
+    out = {"workload": f"jittered prescribed-motion cube {cells}^3 cells (+-0.2 cell, seed 0), {steps} steps",
+           "steps": steps}
+    for mat in ("snh", "arap"):
+        runs = {}
+        for prec in ("fp64", "mixed"):
+            sim = BackwardEulerSim(X, T, mu, lam, rho=1000.0, h=1.0 / 60.0, material=mat,
+                                   fixed_vertices=np.concatenate([bottom, top]), pcg_rtol=1e-8, newton_rtol=1e-6,
+                                   precision=prec)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            kept, infos = sim.rollout(steps, bc=bc)
+            dt = time.perf_counter() - t0
+            if any(i["status"] != 0 for i in infos):
+                raise RuntimeError(f"{mat}/{prec}: status {[i['status'] for i in infos]}")
+            runs[prec] = (kept, infos, dt)
+            del sim
+        (k64, i64, t64), (k32, i32, t32) = runs["fp64"], runs["mixed"]
+        err = [float(np.abs(k32[s] - k64[s]).max() / np.abs(k64[s]).max()) for s in range(1, steps + 1)]
+        out[mat] = {"per_step_rel_err_vs_fp64": err, "max_rel_err": max(err),
+                    "fp64_steps_per_s": round(steps / t64, 4), "mixed_steps_per_s": round(steps / t32, 4),
+                    "newton_fp64": sum(i["newton_iters"] for i in i64),
+                    "newton_mixed": sum(i["newton_iters"] for i in i32),
+                    "pcg_fp64": sum(i["pcg_iters"] for i in i64), "pcg_mixed": sum(i["pcg_iters"] for i in i32)}
+        log(f"bench[jittered {mat}]: max rel err {max(err):.2e}, fp64 {steps / t64:.3f} / mixed {steps / t32:.3f} "
+            f"steps/s")
+    return out
+
+
 def batch_leg(n, steps):
     """Batched solve of n cantilever beams, one launch per Newton phase."""
     import torch
@@ -275,6 +319,9 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start legs")
     ap.add_argument("--warmstart-episodes", type=int, default=2)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-beams leg")
+    ap.add_argument("--no-jittered", action="store_true",
+                    help="skip the jittered-cube mixed-precision leg (SNH and ARAP vs FP64)")
+    ap.add_argument("--jittered-steps", type=int, default=20)
     ap.add_argument("--batch-beams", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=30)
     ap.add_argument("--pmc-probe", action="store_true",
@@ -355,6 +402,8 @@ def main():
         ws = warmstart_eval_leg(32, args.warmstart_episodes, 100)
         if ws is not None:
             out["learned_warm_start_32"] = ws
+    if not args.no_jittered and head_name == "fp64":
+        out["mixed_jittered"] = mixed_jittered_leg(args.cells, args.jittered_steps)
     if not args.no_batch:
         out["batched_beams"] = batch_leg(args.batch_beams, args.batch_steps)
     if not args.no_cpu_baseline:
