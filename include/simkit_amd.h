@@ -214,6 +214,8 @@ typedef struct sk_sim_params {
   int spmv_blocks;       /* SpMV launch width cap (workgroups walk chunk ranges); 0 = default (FP64: one
                             chunk per wave, FP32: 1024) */
   int update_blocks;     /* launch width cap of the vertex-wise PCG kernels; 0 = one lane per vertex */
+  int pcg_chunk;         /* PCG iterations enqueued between convergence checks (the check of one chunk
+                            waits until the next is enqueued); 0 = default (8 block-Jacobi, 4 two-level) */
 } sk_sim_params;
 
 typedef struct sk_step_info {
@@ -407,6 +409,21 @@ int sk_mlp_backward(sk_mlp* net, const float* X, const float* T, int batch, doub
 int sk_mlp_get_grad(sk_mlp* net, float* grad, sk_stream_t stream);
 /* One Adam step (bias-corrected) with the last gradient. */
 int sk_mlp_adam(sk_mlp* net, float lr, float beta1, float beta2, float eps, sk_stream_t stream);
+
+/* Two-level PCG preconditioner, no counterpart in the reference (which
+ * factorises each Newton system with SuperLU, newton.py:52): block-Jacobi
+ * plus an additive coarse correction W E^-1 W^T over piecewise-rigid
+ * aggregates. X (host, nv x 3) gives the rest positions; the bounding box is
+ * cut into cubic bins of about per_agg vertices, each non-empty bin is an
+ * aggregate with its 6 rigid motions restricted to the free DOFs
+ * (orthonormalised, degenerate ones dropped). E = W^T A W is formed from the
+ * first system the simulator then solves and inverted once on the device;
+ * later solves reuse it (any symmetric positive definite E gives a valid
+ * preconditioner). per_agg <= 0 returns to block-Jacobi. Fails with
+ * SK_ERR_INVALID when the coarse space would exceed 4096 unknowns. */
+int sk_sim_set_coarse(sk_sim* sim, const double* X, int per_agg);
+/* Aggregates, coarse unknowns, whether E^-1 has been formed, and the time it took (ms). */
+int sk_sim_coarse_info(const sk_sim* sim, int* n_agg, int* nc, int* built, double* build_ms);
 
 /* Measurement hook, no counterpart in the reference: re-initialises PCG
  * scratch vectors on the system assembled by the last sk_sim_step (RHS = the

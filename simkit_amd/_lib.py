@@ -71,6 +71,7 @@ class SkSimParams(ctypes.Structure):
         ("max_refine", ctypes.c_int),
         ("spmv_blocks", ctypes.c_int),
         ("update_blocks", ctypes.c_int),
+        ("pcg_chunk", ctypes.c_int),
     ]
 
 
@@ -203,6 +204,8 @@ SIGNATURES = {
     "sk_sim_step_guess": [_P, _P, _P, _P, _P, _P, ctypes.POINTER(SkStepInfo), _P],
     "sk_sim_step_bdf2": [_P, _P, _P, _P, _P, _P, _P, ctypes.POINTER(SkStepInfo), _P],
     "sk_sim_spmv_reps": [_P, _INT, _P],
+    "sk_sim_set_coarse": [_P, _P, _INT],
+    "sk_sim_coarse_info": [_P, _P, _P, _P, _P],
 }
 _RESTYPES = {"sk_last_error": ctypes.c_char_p, "sk_status_string": ctypes.c_char_p, "sk_mlp_num_params": ctypes.c_int64}
 

@@ -1,0 +1,81 @@
+// Two-level PCG preconditioner: block-Jacobi plus an additive coarse
+// correction over piecewise-rigid aggregates,
+//   M^-1 r = B^-1 r + W E^-1 W^T r,   E = W^T A0 W,
+// where the vertices are grouped into spatial aggregates, W holds each
+// aggregate's six rigid motions (three translations, three infinitesimal
+// rotations about the aggregate centroid) restricted to the free DOFs and
+// orthonormalised per aggregate, and A0 is the first system the simulator
+// solves. Any symmetric positive definite E keeps M^-1 symmetric positive
+// definite, so E is formed and inverted once and reused by every later solve.
+//
+// Layout (n_agg aggregates, nc coarse unknowns):
+//   agg_ptr[n_agg + 1], agg_vert[nv]: the vertices of aggregate I are
+//     agg_vert[agg_ptr[I] .. agg_ptr[I + 1]), ascending;
+//   vagg[nv]: aggregate of each vertex;
+//   dv[3 * v]: offset of vertex v from its aggregate's centroid;
+//   dva[3 * t]: the same offset for vertex agg_vert[t] (aggregate order);
+//   tm[36 I + 6 a + j]: column j of aggregate I's orthonormal basis in the
+//     raw rigid-motion coordinates a (0..2 translation, 3..5 rotation);
+//   agg_col[n_agg + 1]: the coarse unknowns of aggregate I are
+//     agg_col[I] .. agg_col[I + 1) (at most 6; fewer where pinned DOFs or a
+//     degenerate vertex set leave fewer independent motions);
+//   einv[nc * ld]: E^-1, symmetric, FP32 (applied with FP64 accumulation),
+//     rows padded with zeros to ld = nc rounded up to a multiple of 4;
+//   c[ld]: the restricted residual W^T r of the current iteration (zero
+//     past nc).
+// A vertex's raw rigid motion (u, w) displaces it by u + w x d.
+#pragma once
+
+#include "sk_internal.h"
+
+namespace sk {
+
+constexpr int kMaxCoarse = 4096;  // coarse unknowns at most (E^-1: 64 MiB in FP32)
+
+template <typename T>
+struct CoarseView {
+  int n_agg;
+  int nc;
+  int ld;
+  const int32_t* agg_ptr;
+  const int32_t* agg_vert;
+  const int32_t* agg_col;
+  const T* dv;
+  const T* dva;
+  const double* tm;
+  const float* einv;
+  double* c;
+};
+
+struct CoarseSpace {
+  int n_agg = 0, nc = 0, ld = 0;
+  bool built = false;
+  double build_ms = 0.0;
+  DevBuf<int32_t> agg_ptr, agg_vert, agg_col, vagg, adj_ptr, adj;
+  DevBuf<double> tm, dv64, dva64, E, pivot, c;
+  DevBuf<float> dv32, dva32, einv;
+  template <typename T>
+  CoarseView<T> view() const;
+};
+
+template <>
+inline CoarseView<double> CoarseSpace::view<double>() const {
+  return {n_agg, nc, ld, agg_ptr.p, agg_vert.p, agg_col.p, dv64.p, dva64.p, tm.p, einv.p, c.p};
+}
+template <>
+inline CoarseView<float> CoarseSpace::view<float>() const {
+  return {n_agg, nc, ld, agg_ptr.p, agg_vert.p, agg_col.p, dv32.p, dva32.p, tm.p, einv.p, c.p};
+}
+
+// Aggregates from the rest positions X (host, nv x 3): each axis of the
+// bounding box is cut evenly into bins of close to the edge s of a cube that
+// holds per_agg vertices at the mesh's mean density; non-empty bins become
+// aggregates. fixed_host: per-DOF 0/1 (may be null).
+int coarse_setup(CoarseSpace& C, sk_mesh* m, const double* X, const uint8_t* fixed_host, int per_agg);
+
+// E = W^T A W from the assembled SELL values (FP32 or FP64), then E^-1 in
+// FP64 by Gauss-Jordan elimination (E is symmetric positive definite: no
+// pivoting), symmetrised and stored in FP32. Synchronises the stream.
+int coarse_build(CoarseSpace& C, sk_mesh* m, const void* vals, bool fp32, const uint8_t* fixed, hipStream_t st);
+
+}  // namespace sk

@@ -14,7 +14,7 @@
 // iteration count, stop flag) in a device struct that only later launches
 // read, so every launch returns at once after convergence and the host reads
 // the struct once per chunk of enqueued iterations.
-#include "sk_internal.h"
+#include "sk_coarse.h"
 
 #include <cmath>
 
@@ -140,6 +140,7 @@ struct BlockPcgArgs {
   int gs;
   int k;          // iteration index of this launch
   int restart;    // 1 on the first launch after a residual replacement
+  CoarseView<T> cv;  // two-level preconditioner (cv.n_agg = 0: block-Jacobi only)
 };
 
 template <typename T>
@@ -149,10 +150,93 @@ __device__ __forceinline__ void apply_binv(const T* __restrict__ binv, int64_t v
   for (int p = 0; p < 3; ++p) z[p] = B[3 * p] * r[0] + B[3 * p + 1] * r[1] + B[3 * p + 2] * r[2];
 }
 
+// Vertex loop of the vertex-wise kernels: grid-stride in index order, or
+// (kAgg, two-level preconditioner) workgroup I over the vertices of
+// aggregate I, which also restricts the residual to that aggregate.
+template <bool kAgg, typename T, typename F>
+__device__ __forceinline__ void for_vertices(const BlockPcgArgs<T>& a, F f) {
+  if (kAgg) {
+    const int32_t t1 = a.cv.agg_ptr[blockIdx.x + 1];
+    for (int32_t t = a.cv.agg_ptr[blockIdx.x] + threadIdx.x; t < t1; t += blockDim.x) f((int64_t)a.cv.agg_vert[t]);
+  } else {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < a.nv; v += (int64_t)gridDim.x * blockDim.x)
+      f(v);
+  }
+}
+
+// raw[0..2] += r, raw[3..5] += d x r: the vertex's term of R_v^T r.
 template <typename T>
+__device__ __forceinline__ void restrict_add(const CoarseView<T>& cv, int64_t v, const T (&r)[3], double (&raw)[6]) {
+  const double d0 = (double)cv.dv[3 * v], d1 = (double)cv.dv[3 * v + 1], d2 = (double)cv.dv[3 * v + 2];
+  const double r0 = (double)r[0], r1 = (double)r[1], r2 = (double)r[2];
+  raw[0] += r0;
+  raw[1] += r1;
+  raw[2] += r2;
+  raw[3] += d1 * r2 - d2 * r1;
+  raw[4] += d2 * r0 - d0 * r2;
+  raw[5] += d0 * r1 - d1 * r0;
+}
+
+// The aggregate's coarse columns and, in threads 0..5, column threadIdx.x of
+// its basis T_I: loaded at kernel entry so that the restriction at the end
+// does not wait for them.
+struct AggPre {
+  double col[6];
+  int c0, k;
+};
+
+template <bool kAgg, typename T>
+__device__ __forceinline__ AggPre agg_prefetch(const BlockPcgArgs<T>& a) {
+  AggPre pre;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) pre.col[q] = 0.0;
+  pre.c0 = pre.k = 0;
+  if (kAgg) {
+    pre.c0 = a.cv.agg_col[blockIdx.x];
+    pre.k = a.cv.agg_col[blockIdx.x + 1] - pre.c0;
+    if (threadIdx.x < 6) {
+      const double* tm = a.cv.tm + 36 * (int64_t)blockIdx.x + threadIdx.x;
+#pragma unroll
+      for (int q = 0; q < 6; ++q) pre.col[q] = tm[6 * q];
+    }
+  }
+  return pre;
+}
+
+// Per-workgroup partials of r.z and r.r; with kAgg also the aggregate's
+// restricted residual c_I = T_I^T raw (thread j < k writes c_I[j]).
+template <bool kAgg, typename T>
+__device__ __forceinline__ void finish_vertex_pass(const BlockPcgArgs<T>& a, double rz, double rr,
+                                                   const double (&raw)[6], const AggPre& pre) {
+  if (!kAgg) {
+    double vals[2] = {rz, rr};
+    store_block_partials<2>(vals, a.upart);
+    return;
+  }
+  __shared__ double lds[8 * kBlock / kWave];
+  __shared__ double s_raw[6];
+  double v[8] = {rz, rr, raw[0], raw[1], raw[2], raw[3], raw[4], raw[5]};
+  block_sum<8>(v, lds);
+  if (threadIdx.x == 0) {
+    a.upart[blockIdx.x] = v[0];
+    a.upart[gridDim.x + blockIdx.x] = v[1];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) s_raw[q] = v[2 + q];
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < pre.k) {
+    double c = 0.0;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) c += pre.col[q] * s_raw[q];
+    a.cv.c[pre.c0 + threadIdx.x] = c;
+  }
+}
+
+template <typename T, bool kAgg>
 __global__ __launch_bounds__(kBlock) void bpcg_init_kernel(BlockPcgArgs<T> a) {
-  double rz = 0.0, rr = 0.0;
-  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < a.nv; v += (int64_t)gridDim.x * blockDim.x) {
+  const AggPre pre = agg_prefetch<kAgg>(a);
+  double rz = 0.0, rr = 0.0, raw[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for_vertices<kAgg>(a, [&](int64_t v) {
     T r[3], z[3];
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
@@ -170,9 +254,9 @@ __global__ __launch_bounds__(kBlock) void bpcg_init_kernel(BlockPcgArgs<T> a) {
       rz += (double)r[p] * (double)z[p];
       rr += (double)r[p] * (double)r[p];
     }
-  }
-  double vals[2] = {rz, rr};
-  store_block_partials<2>(vals, a.upart);
+    if (kAgg) restrict_add(a.cv, v, r, raw);
+  });
+  finish_vertex_pass<kAgg>(a, rz, rr, raw, pre);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     a.st->done = 0;
     a.st->iter = 0;
@@ -181,10 +265,11 @@ __global__ __launch_bounds__(kBlock) void bpcg_init_kernel(BlockPcgArgs<T> a) {
 
 // Residual replacement: r already holds the replaced residual; z = B^-1 r,
 // the correction accumulator x restarts from 0, the search direction is kept.
-template <typename T>
+template <typename T, bool kAgg>
 __global__ __launch_bounds__(kBlock) void bpcg_restart_kernel(BlockPcgArgs<T> a) {
-  double rz = 0.0, rr = 0.0;
-  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < a.nv; v += (int64_t)gridDim.x * blockDim.x) {
+  const AggPre pre = agg_prefetch<kAgg>(a);
+  double rz = 0.0, rr = 0.0, raw[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for_vertices<kAgg>(a, [&](int64_t v) {
     T r[3], z[3];
 #pragma unroll
     for (int p = 0; p < 3; ++p) r[p] = a.r[3 * v + p];
@@ -197,9 +282,9 @@ __global__ __launch_bounds__(kBlock) void bpcg_restart_kernel(BlockPcgArgs<T> a)
       rz += (double)r[p] * (double)z[p];
       rr += (double)r[p] * (double)r[p];
     }
-  }
-  double vals[2] = {rz, rr};
-  store_block_partials<2>(vals, a.upart);
+    if (kAgg) restrict_add(a.cv, v, r, raw);
+  });
+  finish_vertex_pass<kAgg>(a, rz, rr, raw, pre);
   if (blockIdx.x == 0 && threadIdx.x == 0) a.st->done = 0;
 }
 
@@ -304,13 +389,14 @@ __global__ __launch_bounds__(kBlock) void bpcg_spmv_kernel(BlockPcgArgs<T> a) {
   store_block_partials<1>(vals, a.spart);
 }
 
-template <typename T>
+template <typename T, bool kAgg>
 __global__ __launch_bounds__(kBlock) void bpcg_update_kernel(BlockPcgArgs<T> a) {
+  const AggPre pre = agg_prefetch<kAgg>(a);
   double alpha_d;
   if (!update_prologue(a.st, a.spart, a.gs, a.k, alpha_d)) return;
   const T alpha = (T)alpha_d;
-  double rz = 0.0, rr = 0.0;
-  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < a.nv; v += (int64_t)gridDim.x * blockDim.x) {
+  double rz = 0.0, rr = 0.0, raw[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for_vertices<kAgg>(a, [&](int64_t v) {
     T r[3], z[3];
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
@@ -327,9 +413,87 @@ __global__ __launch_bounds__(kBlock) void bpcg_update_kernel(BlockPcgArgs<T> a) 
       rz += (double)r[p] * (double)z[p];
       rr += (double)r[p] * (double)r[p];
     }
+    if (kAgg) restrict_add(a.cv, v, r, raw);
+  });
+  finish_vertex_pass<kAgg>(a, rz, rr, raw, pre);
+}
+
+// Coarse correction of the two-level preconditioner, one workgroup per
+// aggregate I (launched after a kAgg vertex pass of the same grid): y_I =
+// (E^-1 c)_I, the aggregate's rigid motion (u, w) = T_I y_I is added to z on
+// its free DOFs, and c_I . y_I is added to the workgroup's r.z partial, so
+// the next launch's r.z is r . (B^-1 r + W E^-1 W^T r). The stop flag is read
+// at entry and tested after the first barrier; nothing is written once set.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void bpcg_coarse_kernel(BlockPcgArgs<T> a) {
+  __shared__ int s_done;
+  __shared__ double lds[6 * kBlock / kWave];
+  __shared__ double s_y[6], s_uw[6], s_cy[6];
+  const CoarseView<T>& cv = a.cv;
+  const int I = blockIdx.x;
+  if (threadIdx.x == 0) s_done = a.st->done;
+  const int c0 = cv.agg_col[I], k = cv.agg_col[I + 1] - c0;
+  const int32_t t0 = cv.agg_ptr[I], t1 = cv.agg_ptr[I + 1];
+  double trow[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // thread q < 6: row q of T_I
+  if (threadIdx.x < 6) {
+#pragma unroll
+    for (int j = 0; j < 6; ++j) trow[j] = cv.tm[36 * (int64_t)I + 6 * threadIdx.x + j];
   }
-  double vals[2] = {rz, rr};
-  store_block_partials<2>(vals, a.upart);
+  const double cq = ((int)threadIdx.x < k) ? cv.c[c0 + threadIdx.x] : 0.0;
+  const double part0 = threadIdx.x == 0 ? a.upart[I] : 0.0;
+  // y_q = sum_j E^-1[c0 + q][j] c_j, four columns per load, two loads in flight per row
+  const int ld4 = cv.ld >> 2;
+  const float4* __restrict__ E4 = reinterpret_cast<const float4*>(cv.einv);
+  const double2* __restrict__ c2 = reinterpret_cast<const double2*>(cv.c);
+  double y[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int j4 = threadIdx.x; j4 < ld4; j4 += 2 * kBlock) {
+    const int j4b = j4 + kBlock;
+    const bool hb = j4b < ld4;
+    const double2 ca = c2[2 * j4], cb = c2[2 * j4 + 1];
+    double2 da = {0.0, 0.0}, db = {0.0, 0.0};
+    if (hb) {
+      da = c2[2 * j4b];
+      db = c2[2 * j4b + 1];
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      if (q < k) {
+        const float4* row = E4 + (int64_t)(c0 + q) * ld4;
+        const float4 e = row[j4];
+        const float4 f = hb ? row[j4b] : make_float4(0.f, 0.f, 0.f, 0.f);
+        y[q] += (double)e.x * ca.x + (double)e.y * ca.y + (double)e.z * cb.x + (double)e.w * cb.y;
+        y[q] += (double)f.x * da.x + (double)f.y * da.y + (double)f.z * db.x + (double)f.w * db.y;
+      }
+    }
+  }
+  block_sum<6>(y, lds);
+  if (s_done) return;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) s_y[q] = y[q];
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    double u = 0.0;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) u += trow[j] * s_y[j];
+    s_uw[threadIdx.x] = u;
+    s_cy[threadIdx.x] = cq * s_y[threadIdx.x];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) a.upart[I] = part0 + (s_cy[0] + s_cy[1] + s_cy[2] + s_cy[3] + s_cy[4] + s_cy[5]);
+  const double u0 = s_uw[0], u1 = s_uw[1], u2 = s_uw[2], w0 = s_uw[3], w1 = s_uw[4], w2 = s_uw[5];
+  for (int32_t t = t0 + threadIdx.x; t < t1; t += kBlock) {
+    const int64_t v = cv.agg_vert[t];
+    const double d0 = (double)cv.dva[3 * (int64_t)t], d1 = (double)cv.dva[3 * (int64_t)t + 1],
+                 d2 = (double)cv.dva[3 * (int64_t)t + 2];
+    const double corr[3] = {u0 + w1 * d2 - w2 * d1, u1 + w2 * d0 - w0 * d2, u2 + w0 * d1 - w1 * d0};
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+      const int64_t kk = 3 * v + p;
+      if (!(a.fixed && a.fixed[kk])) a.z[kk] = (T)((double)a.z[kk] + corr[p]);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -612,7 +776,9 @@ struct sk_sim {
   hipEvent_t chunk_ev[2] = {};
   hipEvent_t ev[6] = {};
   bool have_events = false;
+  sk::CoarseSpace coarse;  // two-level preconditioner (coarse.n_agg > 0 when enabled)
   bool mixed() const { return prm.precision == SK_PREC_MIXED; }
+  bool two_level() const { return coarse.n_agg > 0; }
 };
 
 namespace {
@@ -625,7 +791,14 @@ int copy_dev(DevBuf<double>& dst, const double* src, size_t n) {
   return SK_OK;
 }
 
-constexpr int kPcgChunk = 16;
+// PCG iterations enqueued per convergence check: past convergence up to about
+// 1.5 chunks of launches return in their prologue, so short solves (the
+// two-level preconditioner's) use shorter chunks.
+int pcg_chunk(const sk_sim* S) {
+  if (S->prm.pcg_chunk > 0) return S->prm.pcg_chunk;
+  return S->two_level() ? 4 : 8;
+}
+constexpr int kPcgChunk = 16;  // sk_pcg_csr
 constexpr int kSpmvBlocks32 = 1024;
 
 int spmv_grid(int64_t rows) {
@@ -667,12 +840,18 @@ BlockPcgArgs<T> block_pcg_args(sk_sim* S, const PcgBufs<T>& B, const T* b, doubl
   // lane per vertex unless update_blocks caps them (they stride over the
   // vertices); the SpMV runs one chunk per wave in FP64 and walks contiguous
   // chunk ranges with 1024 workgroups in FP32, unless spmv_blocks caps it.
-  const int gu = S->update_blocks > 0 ? std::min(grid_for(m->nv), S->update_blocks) : grid_for(m->nv);
+  // With the two-level preconditioner the vertex-wise kernels run one
+  // workgroup per aggregate.
+  const int gu = S->two_level() ? S->coarse.n_agg
+                 : S->update_blocks > 0 ? std::min(grid_for(m->nv), S->update_blocks)
+                                        : grid_for(m->nv);
   const int cap = S->spmv_blocks > 0 ? S->spmv_blocks : (sizeof(T) == 8 ? 0 : kSpmvBlocks32);
   const int gs = cap > 0 ? std::min(sell_grid(m->nchunk), (cap + 7) / 8 * 8) : sell_grid(m->nchunk);
   double* part = m->red.partials.p;
+  CoarseView<T> cv{};
+  if (S->two_level()) cv = S->coarse.view<T>();
   return BlockPcgArgs<T>{m->nv, sell_view(m), B.vals, B.binv, S->fixed.p, b, B.x, B.r, B.z, B.p1, B.p0, B.Ap,
-                         S->st.p, rtol2, part, gu, part + 2 * gu, gs, 0, 0};
+                         S->st.p, rtol2, part, gu, part + 2 * gu, gs, 0, 0, cv};
 }
 
 template <typename T>
@@ -681,7 +860,7 @@ void launch_spmv(const BlockPcgArgs<T>& a, int64_t nchunk, hipStream_t st) {
   else bpcg_spmv_kernel<T, true><<<a.gs, kBlock, 0, st>>>(a);
 }
 
-// Block-Jacobi PCG on the assembled system with right-hand side b (x0 = 0).
+// Block-Jacobi (or two-level) PCG on the assembled system with right-hand side b (x0 = 0).
 // With k0 > 0 it instead continues an earlier run after a residual
 // replacement: B.r holds the replaced residual, iterations continue from
 // index k0 with the kept search direction, and x accumulates the correction
@@ -690,28 +869,41 @@ void launch_spmv(const BlockPcgArgs<T>& a, int64_t nchunk, hipStream_t st) {
 template <typename T>
 int run_block_pcg(sk_sim* S, const PcgBufs<T>& B, const T* b, double rtol, int64_t max_iter, hipStream_t st,
                   int64_t* iters, int* done, int64_t k0 = 0) {
+  // The coarse operator is formed from the first system solved and kept.
+  if (S->two_level() && !S->coarse.built)
+    SK_TRY(coarse_build(S->coarse, S->mesh, B.vals, sizeof(T) == 4, S->fixed.p, st));
   BlockPcgArgs<T> a = block_pcg_args<T>(S, B, b, rtol * rtol);
   SK_REQUIRE(2 * a.gu + a.gs <= 8 * Reducer::kMaxBlocks, "mesh too large for the PCG reductions");
+  const bool tl = S->two_level();
   if (k0 == 0) {
     a.pout = B.p0;  // init zeroes p0, the first "previous p"
-    bpcg_init_kernel<T><<<a.gu, kBlock, 0, st>>>(a);
+    if (tl) bpcg_init_kernel<T, true><<<a.gu, kBlock, 0, st>>>(a);
+    else bpcg_init_kernel<T, false><<<a.gu, kBlock, 0, st>>>(a);
   } else {
-    bpcg_restart_kernel<T><<<a.gu, kBlock, 0, st>>>(a);
+    if (tl) bpcg_restart_kernel<T, true><<<a.gu, kBlock, 0, st>>>(a);
+    else bpcg_restart_kernel<T, false><<<a.gu, kBlock, 0, st>>>(a);
   }
+  if (tl) bpcg_coarse_kernel<T><<<a.gu, kBlock, 0, st>>>(a);
   SK_HIP_CHECK(hipGetLastError());
-  // Chunks of kPcgChunk iterations; the state copied after chunk c is checked
+  const int chunk = pcg_chunk(S);
+  // Chunks of `chunk` iterations; the state copied after chunk c is checked
   // only once chunk c + 1 has been enqueued, so the GPU is not idle while the
   // host reads it. Launches past convergence exit in their prologue.
   int64_t launched = k0;
   int c = 0;
   for (bool stop = false; !stop; ++c) {
-    for (int k = 0; k < kPcgChunk; ++k, ++launched) {
+    for (int k = 0; k < chunk; ++k, ++launched) {
       a.k = (int)launched;
       a.restart = (k0 > 0 && launched == k0) ? 1 : 0;
       a.pin = (launched & 1) ? B.p1 : B.p0;
       a.pout = (launched & 1) ? B.p0 : B.p1;
       launch_spmv<T>(a, S->mesh->nchunk, st);
-      bpcg_update_kernel<T><<<a.gu, kBlock, 0, st>>>(a);
+      if (tl) {
+        bpcg_update_kernel<T, true><<<a.gu, kBlock, 0, st>>>(a);
+        bpcg_coarse_kernel<T><<<a.gu, kBlock, 0, st>>>(a);
+      } else {
+        bpcg_update_kernel<T, false><<<a.gu, kBlock, 0, st>>>(a);
+      }
     }
     SK_HIP_CHECK(hipGetLastError());
     SK_HIP_CHECK(hipMemcpyAsync(S->st_host + (c & 1), S->st.p, sizeof(PcgState), hipMemcpyDeviceToHost, st));
@@ -720,7 +912,7 @@ int run_block_pcg(sk_sim* S, const PcgBufs<T>& B, const T* b, double rtol, int64
       SK_HIP_CHECK(hipEventSynchronize(S->chunk_ev[(c - 1) & 1]));
       const PcgState& h = S->st_host[(c - 1) & 1];
       if (h.done || h.iter >= max_iter) stop = true;
-      if (launched > max_iter + 2 * kPcgChunk) stop = true;  // device state not advancing
+      if (launched > max_iter + 2 * chunk) stop = true;  // device state not advancing
       if (k0 > 0 && h.iter < k0) stop = true;                 // continuation did not start
     }
   }
@@ -869,6 +1061,7 @@ int sk_sim_create(sk_mesh* mesh, const sk_sim_params* p, const double* mu, const
   SK_REQUIRE(p->precision == SK_PREC_FP64 || (p->inner_rtol > 0.0 && p->inner_rtol < 1.0 && p->max_refine > 0),
              "sk_sim_create: mixed precision needs 0 < inner_rtol < 1 and max_refine > 0");
   SK_REQUIRE(p->spmv_blocks >= 0 && p->update_blocks >= 0, "sk_sim_create: launch widths must be >= 0");
+  SK_REQUIRE(p->pcg_chunk >= 0 && p->pcg_chunk <= 1024, "sk_sim_create: pcg_chunk must be in [0, 1024]");
   sk_sim* S = new sk_sim();
   S->mesh = mesh;
   S->prm = *p;
@@ -925,6 +1118,31 @@ int sk_sim_destroy(sk_sim* S) {
       if (e) (void)hipEventDestroy(e);
   }
   delete S;
+  return SK_OK;
+}
+
+int sk_sim_set_coarse(sk_sim* S, const double* X, int per_agg) {
+  SK_REQUIRE(S, "sk_sim_set_coarse: null simulator");
+  S->coarse = CoarseSpace();
+  if (per_agg <= 0) return SK_OK;
+  SK_REQUIRE(X, "sk_sim_set_coarse: null positions");
+  const int64_t n3 = 3 * S->mesh->nv;
+  std::vector<uint8_t> fixed;
+  if (S->fixed.p) {
+    fixed.resize(n3);
+    SK_HIP_CHECK(hipMemcpy(fixed.data(), S->fixed.p, n3, hipMemcpyDeviceToHost));
+  }
+  const int s = coarse_setup(S->coarse, S->mesh, X, S->fixed.p ? fixed.data() : nullptr, per_agg);
+  if (s != SK_OK) S->coarse = CoarseSpace();
+  return s;
+}
+
+int sk_sim_coarse_info(const sk_sim* S, int* n_agg, int* nc, int* built, double* build_ms) {
+  SK_REQUIRE(S, "sk_sim_coarse_info: null simulator");
+  if (n_agg) *n_agg = S->coarse.n_agg;
+  if (nc) *nc = S->coarse.nc;
+  if (built) *built = S->coarse.built ? 1 : 0;
+  if (build_ms) *build_ms = S->coarse.build_ms;
   return SK_OK;
 }
 
@@ -1096,7 +1314,7 @@ int sk_sim_spmv_reps(sk_sim* S, int reps, sk_stream_t s) {
     const PcgBufs<float> B = bufs32(S);
     BlockPcgArgs<float> a = block_pcg_args<float>(S, B, B.r, 0.0);
     a.pout = B.p0;
-    bpcg_init_kernel<float><<<a.gu, kBlock, 0, st>>>(a);
+    bpcg_init_kernel<float, false><<<a.gu, kBlock, 0, st>>>(a);
     a.pin = B.p0;
     a.pout = B.p1;
     for (int k = 0; k < reps; ++k) launch_spmv<float>(a, S->mesh->nchunk, st);
@@ -1104,7 +1322,7 @@ int sk_sim_spmv_reps(sk_sim* S, int reps, sk_stream_t s) {
     const PcgBufs<double> B = bufs64(S);
     BlockPcgArgs<double> a = block_pcg_args<double>(S, B, S->g.p, 0.0);
     a.pout = B.p0;
-    bpcg_init_kernel<double><<<a.gu, kBlock, 0, st>>>(a);
+    bpcg_init_kernel<double, false><<<a.gu, kBlock, 0, st>>>(a);
     a.pin = B.p0;
     a.pout = B.p1;
     for (int k = 0; k < reps; ++k) launch_spmv<double>(a, S->mesh->nchunk, st);

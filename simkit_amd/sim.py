@@ -31,11 +31,17 @@ class BackwardEulerSim:
                  gravity=0.0, psd=True, newton_rtol=1e-6, newton_max_iter=100, pcg_rtol=1e-8,
                  pcg_max_iter=100000, ls_alpha=0.01, ls_beta=0.5, ls_max_iter=100, ls_threshold=1e-12,
                  timing=False, precision="fp64", inner_rtol=1e-2, max_refine=50, spmv_blocks=0,
-                 update_blocks=0):
+                 update_blocks=0, preconditioner="bjacobi", coarse_vertices=216, pcg_chunk=0):
         """precision: "fp64" (every stage FP64) or "mixed" (FP32 element Hessian, PSD projection,
         assembled values and inner PCG to inner_rtol, wrapped in FP64 iterative refinement to
         pcg_rtol; gradient, line-search energies and the polar SVD of ARAP elements with
-        det F < 1e-3 stay FP64)."""
+        det F < 1e-3 stay FP64).
+
+        preconditioner: "bjacobi" (3x3 block-Jacobi) or "two_level" (block-Jacobi plus an additive
+        coarse correction over piecewise-rigid aggregates of about coarse_vertices rest vertices;
+        the coarse operator is formed from the first system solved). Both solve to pcg_rtol."""
+        if preconditioner not in ("bjacobi", "two_level"):
+            raise ValueError("preconditioner must be 'bjacobi' or 'two_level'")
         X = np.asarray(X, dtype=np.float64)
         T = np.asarray(T)
         self.X = X
@@ -81,6 +87,7 @@ class BackwardEulerSim:
         p.max_refine = max_refine
         p.spmv_blocks = spmv_blocks
         p.update_blocks = update_blocks
+        p.pcg_chunk = pcg_chunk
         self.params = p
         self.precision = precision
         h_ = ctypes.c_void_p()
@@ -89,6 +96,18 @@ class BackwardEulerSim:
         self._h = h_
         self._fin = weakref.finalize(self, _destroy, h_.value)
         self.X_dev = dev(X.reshape(-1))
+        self.preconditioner = preconditioner
+        if preconditioner == "two_level":
+            Xc = np.ascontiguousarray(X, dtype=np.float64)
+            _lib.call("sk_sim_set_coarse", self._h, Xc.ctypes.data_as(ctypes.c_void_p), int(coarse_vertices))
+
+    def coarse_info(self):
+        """Aggregates, coarse unknowns, whether the coarse inverse has been formed and its build time (ms)."""
+        n, nc, built, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _lib.call("sk_sim_coarse_info", self._h, ctypes.byref(n), ctypes.byref(nc), ctypes.byref(built),
+                  ctypes.byref(ms))
+        return {"aggregates": n.value, "coarse_unknowns": nc.value, "built": bool(built.value),
+                "build_ms": ms.value}
 
     def step(self, x_curr, x_prev, x_bc=None, out=None):
         """One BE step on device tensors (3nv,). Returns (x_next, info dict)."""
