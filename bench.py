@@ -105,18 +105,24 @@ def cpu_baseline(X, T, bottom, top, mu, lam, gpu_infos, steps, pcg_sample):
     }
 
 
-def run_path(args, X, T, bottom, top, mu, lam, precision):
+# solver paths: name -> (precision, PCG preconditioner)
+PATHS = {"fp64": ("fp64", "bjacobi"), "mixed": ("mixed", "bjacobi"),
+         "fp64_two_level": ("fp64", "two_level"), "mixed_two_level": ("mixed", "two_level")}
+
+
+def run_path(args, X, T, bottom, top, mu, lam, name):
     """Warm up, then time the first --steps steps of one solver path; returns a result dict."""
     import torch
     from simkit_amd import _lib
     from simkit_amd.sim import BackwardEulerSim, top_face_sinusoid
+    precision, pc = PATHS[name]
     t_setup = time.perf_counter()
     sim = BackwardEulerSim(X, T, mu, lam, rho=1000.0, h=1.0 / 60.0, material="snh",
                            fixed_vertices=np.concatenate([bottom, top]), pcg_rtol=1e-8, newton_rtol=1e-6,
-                           timing=True, precision=precision, inner_rtol=args.inner_rtol)
+                           timing=True, precision=precision, inner_rtol=args.inner_rtol, preconditioner=pc)
     bc = top_face_sinusoid(X, top, 0.1, 1.0)
     torch.cuda.synchronize()
-    log(f"bench[{precision}]: setup {time.perf_counter() - t_setup:.1f}s; warmup {args.warmup} steps")
+    log(f"bench[{name}]: setup {time.perf_counter() - t_setup:.1f}s; warmup {args.warmup} steps")
 
     def run(nsteps):
         x_prev = sim.X_dev.clone()
@@ -126,7 +132,7 @@ def run_path(args, X, T, bottom, top, mu, lam, precision):
             x_next, info = sim.step(x_curr, x_prev, bc(n, (n + 1) * sim.h))
             infos.append(info)
             if info["status"] != 0:
-                raise RuntimeError(f"{precision} step {n}: status {info['status']} ({info})")
+                raise RuntimeError(f"{name} step {n}: status {info['status']} ({info})")
             x_prev, x_curr = x_curr, x_next
         return x_curr, infos
 
@@ -152,24 +158,25 @@ def run_path(args, X, T, bottom, top, mu, lam, precision):
         "phase_ms_total": {k: round(sum(i[k] for i in infos), 2)
                            for k in ("ms_gradient", "ms_hessian", "ms_pcg", "ms_line_search")},
         "spmv_ms": spmv_ms, "spmv_bytes": traffic, "infos": infos, "x_final": x_final.cpu().numpy(),
+        "coarse": sim.coarse_info() if pc == "two_level" else None,
     }
-    log(f"bench[{precision}]: {elapsed:.2f}s, {res['steps_per_s']:.3f} steps/s, newton {res['newton_iters']}, "
+    log(f"bench[{name}]: {elapsed:.2f}s, {res['steps_per_s']:.3f} steps/s, newton {res['newton_iters']}, "
         f"pcg {res['pcg_iters']}, spmv {spmv_ms * 1e3:.1f} us")
     del sim
     return res
 
 
 def learned_leg(cells, steps, warmup):
-    """Learned warm start + FP32/FP64-refinement path on the same trajectory as the headline, with the
-    committed model for this mesh (trained on random top-face sinusoids, so this trajectory is held
-    out). None when no model exists for `cells`."""
+    """Learned warm start + FP32/FP64-refinement path (two-level PCG) on the same trajectory as the
+    headline, with the committed model for this mesh (trained on random top-face sinusoids, so this
+    trajectory is held out). None when no model exists for `cells`."""
     import torch
     from simkit_amd import learned
     model = learned.load_model(cells)
     if model is None:
         return None
     t0 = time.perf_counter()
-    sim, coords = learned.cube_problem(cells, precision="mixed", k=model.k)
+    sim, coords = learned.cube_problem(cells, precision="mixed", k=model.k, preconditioner="two_level")
     setup = time.perf_counter() - t0
     A, f = np.array([0.1, 0.0, 0.0]), np.array([1.0, 1.0, 1.0])   # d(t) = (0.1 sin 2 pi t, 0, 0)
     ro = learned.Rollout(sim, coords, model)
@@ -200,12 +207,12 @@ def warmstart_eval_leg(cells, episodes, steps):
     model = learned.load_model(cells)
     if model is None:
         return None
-    sim, coords = learned.cube_problem(cells, precision="mixed", k=model.k)
+    sim, coords = learned.cube_problem(cells, precision="mixed", k=model.k, preconditioner="two_level")
     res = learned.evaluate(sim, coords, model, episodes, steps, seed=2)
     log(f"bench[warmstart{cells}]: speedup {res['speedup']:.3f}, newton/step {res['plain']['newton_per_step']:.2f}"
         f" -> {res['learned']['newton_per_step']:.2f}")
-    return {"workload": f"driven cube {cells}^3 cells, random top-face sinusoids (held out), stable neo-Hookean",
-            **res}
+    return {"workload": f"driven cube {cells}^3 cells, random top-face sinusoids (held out), stable neo-Hookean, "
+                        f"FP32 + FP64 refinement, two-level PCG", **res}
 
 
 def mixed_jittered_leg(cells, steps):
@@ -293,7 +300,7 @@ def pmc_probe(args):
         dst.copy_(src)
     torch.cuda.synchronize()
     info = {"copy_bytes_read": nbytes, "copy_bytes_written": nbytes}
-    for prec in args.paths.split(","):
+    for prec in [p for p in args.paths.split(",") if p in ("fp64", "mixed")]:
         sim = BackwardEulerSim(X, T, mu, lam, rho=1000.0, fixed_vertices=np.concatenate([bottom, top]),
                                precision=prec)
         x = sim.X_dev.clone()
@@ -315,7 +322,8 @@ def main():
     ap.add_argument("--cpu-pcg-sample", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--inner-rtol", type=float, default=1e-2)
-    ap.add_argument("--paths", default="fp64,mixed", help="comma list of fp64, mixed")
+    ap.add_argument("--paths", default="fp64,mixed,fp64_two_level,mixed_two_level",
+                    help="comma list of " + ", ".join(PATHS))
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start legs")
     ap.add_argument("--warmstart-episodes", type=int, default=2)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-beams leg")
@@ -358,7 +366,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if head_name == "fp64" else "f32+f64",
+        "dtype": "f64" if PATHS[head_name][0] == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {
             "workload": f"prescribed-motion cube {args.cells}^3 cells, stable neo-Hookean, "
@@ -367,10 +375,10 @@ def main():
             "newton_iters": head["newton_iters"], "pcg_iters": head["pcg_iters"], "ls_trials": head["ls_trials"],
             "phase_ms_total": head["phase_ms_total"],
         },
-        "roofline": {"bound": "hbm", "kernel": SPMV_KERNEL[head_name], "achieved": round(achieved, 1),
+        "roofline": {"bound": "hbm", "kernel": SPMV_KERNEL[PATHS[head_name][0]], "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": pmc_traffic(f"spmv_{head_name}" if head_name == "fp64" else "spmv_fp32")
+                     "traffic": pmc_traffic("spmv_fp64" if PATHS[head_name][0] == "fp64" else "spmv_fp32")
                      if args.cells == 48 else None,
                      "bytes_per_launch": int(head["spmv_bytes"]), "avg_launch_ms": round(head["spmv_ms"], 5)},
     }
@@ -389,15 +397,32 @@ def main():
                      "frac": round(ach / HBM_PEAK_GBS, 4),
                      "traffic": pmc_traffic("spmv_fp32") if args.cells == 48 else None},
         }
+    for name in ("fp64_two_level", "mixed_two_level"):
+        if name in results and head_name == "fp64":
+            r = results[name]
+            dev = float(np.abs(r["x_final"] - head["x_final"]).max() / np.abs(head["x_final"]).max())
+            out.setdefault("two_level", {})[PATHS[name][0]] = {
+                "value": round(r["steps_per_s"], 4), "ms_per_step": round(r["ms_per_step"], 3),
+                "speedup_vs_fp64": round(r["steps_per_s"] / head["steps_per_s"], 3),
+                "final_position_rel_dev_vs_fp64": dev, "newton_iters": r["newton_iters"],
+                "pcg_iters": r["pcg_iters"], "refine_steps": r["refine_steps"], "phase_ms_total": r["phase_ms_total"],
+                "coarse": r["coarse"]}
+    if "two_level" in out:
+        out["two_level"]["preconditioner"] = ("block-Jacobi + additive coarse correction over piecewise-rigid "
+                                              "aggregates (6 rigid motions each), coarse operator from the first "
+                                              "Newton system")
     if not args.no_learned and head_name == "fp64":
         lr = learned_leg(args.cells, args.steps, args.warmup)
         if lr is not None:
             dev = float(np.abs(lr["x_final"] - head["x_final"]).max() / np.abs(head["x_final"]).max())
             lr.pop("x_final")
-            out["learned_fp32"] = {"value": round(lr["steps_per_s"], 4),
+            out["learned_fp32"] = {"value": round(lr["steps_per_s"], 4), "preconditioner": "two_level",
                                    "speedup_vs_fp64": round(lr["steps_per_s"] / head["steps_per_s"], 3),
                                    "speedup_vs_fp32_refined": round(lr["steps_per_s"] / results["mixed"]["steps_per_s"], 3)
                                    if "mixed" in results else None,
+                                   "speedup_vs_fp32_two_level": round(
+                                       lr["steps_per_s"] / results["mixed_two_level"]["steps_per_s"], 3)
+                                   if "mixed_two_level" in results else None,
                                    "final_position_rel_dev_vs_fp64": dev, **lr}
         ws = warmstart_eval_leg(32, args.warmstart_episodes, 100)
         if ws is not None:
