@@ -259,11 +259,12 @@ def mixed_jittered_leg(cells, steps):
     return out
 
 
-def batch_leg(n, steps):
-    """Batched solve of n cantilever beams, one launch per Newton phase."""
+def batch_leg(n, steps, coarse_rows=64):
+    """Batched solve of n cantilever beams, one launch per Newton phase; coarse_rows > 0: two-level
+    per-problem PCG (rigid-motion aggregates of that many consecutive vertices, re-formed every step)."""
     import torch
     from simkit_amd.batch import cantilever_batch
-    sim, desc = cantilever_batch(n)
+    sim, desc = cantilever_batch(n, coarse_rows=coarse_rows)
     tuned = sim.autotune()
     x = sim.X_dev.clone()
     sim.step(x, x)
@@ -275,11 +276,14 @@ def batch_leg(n, steps):
     flat = [i for st in infos for i in st]
     res = {"workload": f"{n} cantilever beams (4x4 cells x U{{8..64}}, cell {desc['cell']} m), stable neo-Hookean,"
                        f" FP32 + FP64 refinement, gravity", "beams": n, "vertices": desc["vertices"],
+           "preconditioner": f"two-level, aggregates of {coarse_rows} consecutive vertices" if coarse_rows
+           else "block-Jacobi",
            "tets": desc["tets"], "steps": steps, "batch_steps_per_s": round(steps / dt, 4),
            "beam_steps_per_s": round(steps * n / dt, 1), "newton_rounds_per_step": float(np.mean(rounds)),
            "newton_per_beam_step": float(np.mean([i["newton_iters"] for i in flat])),
+           "pcg_per_beam_step": float(np.mean([i["pcg_iters"] for i in flat])),
            "statuses": sorted(set(int(i["status"]) for i in flat)), "autotune": tuned, "graph": True}
-    log(f"bench[batch]: {dt:.2f}s, {res['beam_steps_per_s']} beam-steps/s")
+    log(f"bench[batch{coarse_rows}]: {dt:.2f}s, {res['beam_steps_per_s']} beam-steps/s")
     return res
 
 
@@ -430,7 +434,10 @@ def main():
     if not args.no_jittered and head_name == "fp64":
         out["mixed_jittered"] = mixed_jittered_leg(args.cells, args.jittered_steps)
     if not args.no_batch:
-        out["batched_beams"] = batch_leg(args.batch_beams, args.batch_steps)
+        out["batched_beams"] = batch_leg(args.batch_beams, args.batch_steps, coarse_rows=64)
+        out["batched_beams"]["block_jacobi"] = {
+            k: v for k, v in batch_leg(args.batch_beams, args.batch_steps, coarse_rows=0).items()
+            if k in ("beam_steps_per_s", "batch_steps_per_s", "pcg_per_beam_step", "newton_rounds_per_step")}
     if not args.no_cpu_baseline:
         log("bench: CPU baseline sample")
         out["cpu_baseline"] = cpu_baseline(X, T, bottom, top, mu, lam, head["infos"], args.steps,

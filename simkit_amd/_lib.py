@@ -187,6 +187,7 @@ SIGNATURES = {
     "sk_batch_destroy": [_P],
     "sk_batch_step": [_P, _P, _P, _P, _P, _P, _P, _P],
     "sk_batch_set_shape": [_P, _INT, _INT, _INT],
+    "sk_batch_set_coarse": [_P, _P, _INT, _INT],
     "sk_batch_autotune": [_P, _P, _P, _P, _INT, _P, _P, _P, _P, _P],
     "sk_dense_project": [_I64, _INT, _P, _P, _P, _P, _P, _P],
     "sk_dense_expand": [_I64, _INT, _P, _P, _P, _P, _P],

@@ -39,9 +39,12 @@ class BatchSim:
     def __init__(self, meshes, mu, lam, rho=1000.0, h=1.0 / 60.0, material="snh", fixed_vertices=None, gravity=0.0,
                  psd=True, newton_rtol=1e-6, newton_max_iter=100, pcg_rtol=1e-8, pcg_max_iter=100000,
                  inner_rtol=1e-2, ls_alpha=0.01, ls_beta=0.5, ls_max_iter=100, ls_threshold=1e-12, block=0, lanes=0,
-                 lds=0, use_graph=True):
+                 lds=0, use_graph=True, coarse_rows=0, coarse_every_step=True):
         """meshes: list of (X (n,3), T (t,4)); mu, lam: per problem (scalar each) or per union element;
-        fixed_vertices: list (per problem, local vertex ids) or None."""
+        fixed_vertices: list (per problem, local vertex ids) or None.
+        coarse_rows > 0: two-level per-problem PCG with rigid-motion aggregates of that many
+        consecutive vertices (a multiple of 64); 0: block-Jacobi. coarse_every_step: re-form the
+        coarse operators at the start of every step (else once, at the first step)."""
         require_gpu()
         P = len(meshes)
         Xs = [np.asarray(X, np.float64) for X, _ in meshes]
@@ -109,6 +112,11 @@ class BatchSim:
         self.X_dev = dev(X.reshape(-1))
         self._info = (_lib.SkBatchInfo * P)()
         self.tuned = None
+        self.coarse_rows = int(coarse_rows)
+        if self.coarse_rows > 0:
+            Xc = np.ascontiguousarray(X, dtype=np.float64)
+            _lib.call("sk_batch_set_coarse", self._h, Xc.ctypes.data_as(ctypes.c_void_p), self.coarse_rows,
+                      1 if coarse_every_step else 0)
 
     def problem_slice(self, p):
         return slice(3 * int(self.vstart[p]), 3 * int(self.vstart[p + 1]))
@@ -138,7 +146,8 @@ class BatchSim:
         """Pick (block, lanes, lds) of the per-problem kernels by timing them on this batch's first
         Newton system from rest (lds: the PCG vectors live in LDS, tried when the largest problem
         fits); with a cache path, reuse / record the choice keyed by the batch's shape."""
-        key = f"P{self.P}_nv{self.nv}_nt{self.nt}_mat{self.material}"
+        key = f"P{self.P}_nv{self.nv}_nt{self.nt}_mat{self.material}" + (
+            f"_coarse{self.coarse_rows}" if self.coarse_rows else "")
         if cache and os.path.exists(cache):
             with open(cache) as fh:
                 table = json.load(fh)
@@ -201,10 +210,12 @@ def main(argv=None):
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--compare-serial", type=int, default=0,
                     help="also advance the first K beams one at a time with the single-mesh simulator")
+    ap.add_argument("--coarse-rows", type=int, default=0,
+                    help="two-level per-problem PCG with aggregates of this many consecutive vertices (0: off)")
     ap.add_argument("--json", default=None)
     a = ap.parse_args(argv)
     t0 = time.perf_counter()
-    sim, desc = cantilever_batch(a.n, use_graph=not a.no_graph)
+    sim, desc = cantilever_batch(a.n, use_graph=not a.no_graph, coarse_rows=a.coarse_rows)
     meshes_desc = desc
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t0
@@ -226,6 +237,7 @@ def main(argv=None):
     res = {"workload": "512 cantilever beams" if a.n == 512 else f"{a.n} cantilever beams", **desc,
            "steps": a.steps, "seconds": dt, "batch_steps_per_s": a.steps / dt,
            "beam_steps_per_s": a.steps * a.n / dt, "graph": not a.no_graph, "autotune": tuned,
+           "coarse_rows": a.coarse_rows,
            "newton_rounds_per_step": float(np.mean(rounds)),
            "newton_per_beam_step": float(np.mean([i["newton_iters"] for i in flat])),
            "pcg_per_beam_step": float(np.mean([i["pcg_iters"] for i in flat])),

@@ -9,7 +9,10 @@
 // problems when there are more problems than workgroups):
 //   * PCG: block-Jacobi CG on the problem's free DOFs in FP32 with FP64
 //     residual replacement (the mixed-precision scheme of the single-mesh
-//     simulator), all scalars reduced inside the workgroup;
+//     simulator), all scalars reduced inside the workgroup; optionally with a
+//     two-level preconditioner (block-Jacobi + an additive coarse correction
+//     over runs of consecutive vertices, rigid motions per run, one coarse
+//     inverse per problem);
 //   * line search: the Armijo backtracking of backtracking_line_search.py:8-67
 //     on the problem's own incremental potential, in FP64 difference form;
 //   * gradient norm and Newton stopping test (newton.py:46-69 with the
@@ -20,6 +23,7 @@
 // reads the active flags on the device; it is captured once in a hipGraph
 // and replayed. Workgroup size and rows-per-lane layout of the per-problem
 // kernels are chosen by sk_batch_autotune (timed on the batch itself).
+#include "sk_coarse.h"
 #include "sk_elem.h"
 #include "sk_internal.h"
 #include "sk_materials.h"
@@ -78,6 +82,16 @@ struct BatchArgs {
   int newton_max_iter;
   int count_pcg;           // 0: autotuning run, counters untouched
   int max_rows;            // largest problem (vertices)
+  // two-level preconditioner (used by the KC kernel variants)
+  const float* cdv;         // [3 nv] offset of each vertex from its aggregate's centroid
+  const float* ctm;         // [36 n_agg] aggregate g's orthonormal basis: T[a][j] at 36 g + 6 a + j
+  const int32_t* cagg0;     // [P + 1] first aggregate of each problem
+  const int32_t* ccol;      // [n_agg + 1] first coarse unknown of each aggregate (batch numbering)
+  const int64_t* einv_off;  // [P] offset of each problem's E^-1 (n_q x n_q, row-major, FP32)
+  const float* einv;
+  int agg_rows;             // vertices per aggregate (a multiple of 64)
+  int max_agg, max_nc;      // per problem
+  int vec_floats;           // dynamic-LDS floats before the coarse scratch
 };
 
 // Every thread of an NT-thread block receives the NV totals (fixed order).
@@ -186,32 +200,171 @@ __device__ __forceinline__ int block_read(const int* flag) {
 }
 
 // ---------------------------------------------------------------------------
+// Two-level preconditioner of the per-problem PCG:
+//   z = B^-1 r + W_q E_q^-1 W_q^T r,
+// W_q: the rigid motions (3 translations, 3 rotations about the centroid,
+// restricted to the free DOFs and orthonormalised) of each run of agg_rows
+// consecutive vertices of problem q; E_q = W_q^T A_q W_q from the first Newton
+// system, inverted once. A wave's rows always lie in one aggregate (agg_rows
+// is a multiple of the 64 / LANES rows of a wave), so the restriction is a
+// wave sum per (aggregate, wave slot) followed by a fixed-order sum of the
+// slots: deterministic and atomic-free.
+// ---------------------------------------------------------------------------
+struct CoarseLds {
+  float* raw;    // [max_agg][slots][6] wave partials of (r, d x r)
+  double* c;     // [max_nc] restricted residual
+  double* y;     // [max_nc] E^-1 c
+  double* uw;    // [max_agg][6] rigid motion (u, w) of each aggregate
+};
+
+__device__ __host__ inline int coarse_raw_floats(int max_agg, int agg_rows) {
+  const int n = max_agg * (agg_rows / 16) * 6;  // slots per aggregate <= agg_rows / 16 (LANES = 4)
+  return (n + 1) & ~1;                          // keep the doubles after it 8-byte aligned
+}
+
+__device__ inline CoarseLds coarse_lds(float* dyn, const BatchArgs& a) {
+  CoarseLds c;
+  c.raw = dyn + a.vec_floats;
+  double* d = reinterpret_cast<double*>(c.raw + coarse_raw_floats(a.max_agg, a.agg_rows));
+  c.c = d;
+  c.y = d + a.max_nc;
+  c.uw = d + 2 * a.max_nc;
+  return c;
+}
+
+// Wave partial of the restriction: every lane passes its row's (r, d x r)
+// (zeros for lanes without a row); lane 0 stores the wave sum in the slot of
+// the wave's aggregate. row0: the wave's first row relative to the problem.
+template <int LANES>
+__device__ __forceinline__ void restrict_wave(const BatchArgs& a, const CoarseLds& cl, int64_t row0, int nagg,
+                                              const float (&v)[6]) {
+  constexpr int rpw = kWave / LANES;  // rows per wave
+  float s[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    float t = v[q];
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) t += __shfl_xor(t, off, kWave);
+    s[q] = t;
+  }
+  const int l = (int)(row0 / a.agg_rows);
+  if (threadIdx.x % kWave == 0 && l < nagg) {
+    const int slots = a.agg_rows / rpw;
+    const int ws = (int)((row0 % a.agg_rows) / rpw);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) cl.raw[(l * slots + ws) * 6 + q] = s[q];
+  }
+}
+
+__device__ __forceinline__ void restrict_terms(const BatchArgs& a, int64_t v, const float (&r)[3], float (&t)[6]) {
+  const float d0 = a.cdv[3 * v], d1 = a.cdv[3 * v + 1], d2 = a.cdv[3 * v + 2];
+  t[0] = r[0];
+  t[1] = r[1];
+  t[2] = r[2];
+  t[3] = d1 * r[2] - d2 * r[1];
+  t[4] = d2 * r[0] - d0 * r[2];
+  t[5] = d0 * r[1] - d1 * r[0];
+}
+
+// From the restriction slots (written before the preceding barrier): c = W^T r,
+// y = E^-1 c, the aggregates' rigid motions; returns c . y in every thread.
+template <int NT, int LANES>
+__device__ double coarse_apply(const BatchArgs& a, const CoarseLds& cl, int p, double* lds) {
+  constexpr int rpw = kWave / LANES;
+  const int slots = a.agg_rows / rpw;
+  const int g0 = a.cagg0[p], nagg = a.cagg0[p + 1] - g0;
+  const int cb = a.ccol[g0], nc = a.ccol[g0 + nagg] - cb;
+  for (int t = threadIdx.x; t < nagg * 6; t += NT) {
+    const int l = t / 6, j = t % 6, g = g0 + l;
+    const int c0 = a.ccol[g] - cb, k = a.ccol[g + 1] - a.ccol[g];
+    if (j < k) {
+      double s = 0.0;
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        float rq = 0.f;
+        for (int w = 0; w < slots; ++w) rq += cl.raw[(l * slots + w) * 6 + q];
+        s += (double)a.ctm[36 * (int64_t)g + 6 * q + j] * (double)rq;
+      }
+      cl.c[c0 + j] = s;
+    }
+  }
+  __syncthreads();
+  const float* E = a.einv + a.einv_off[p];
+  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+  for (int i = wid; i < nc; i += NT / kWave) {
+    double s = 0.0;
+    for (int j = lane; j < nc; j += kWave) s += (double)E[(int64_t)i * nc + j] * cl.c[j];
+    s = wave_sum(s);
+    if (lane == 0) cl.y[i] = s;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < nagg * 6; t += NT) {
+    const int l = t / 6, q = t % 6, g = g0 + l;
+    const int c0 = a.ccol[g] - cb, k = a.ccol[g + 1] - a.ccol[g];
+    double s = 0.0;
+    for (int j = 0; j < k; ++j) s += (double)a.ctm[36 * (int64_t)g + 6 * q + j] * cl.y[c0 + j];
+    cl.uw[t] = s;
+  }
+  double cy[1] = {0.0};
+  for (int t = threadIdx.x; t < nc; t += NT) cy[0] += cl.c[t] * cl.y[t];
+  all_sum<NT, 1>(cy, lds);  // its barriers also publish uw
+  return cy[0];
+}
+
+// Coarse part of z at row v (problem-relative row `row`), zero on fixed DOFs.
+__device__ __forceinline__ void coarse_term(const BatchArgs& a, const CoarseLds& cl, int64_t v, int64_t row,
+                                            float (&zc)[3]) {
+  const double* uw = cl.uw + 6 * (row / a.agg_rows);
+  const double d0 = a.cdv[3 * v], d1 = a.cdv[3 * v + 1], d2 = a.cdv[3 * v + 2];
+  const double t[3] = {uw[0] + uw[4] * d2 - uw[5] * d1, uw[1] + uw[5] * d0 - uw[3] * d2,
+                       uw[2] + uw[3] * d1 - uw[4] * d0};
+#pragma unroll
+  for (int q = 0; q < 3; ++q) zc[q] = is_fixed(a.fixed, 3 * v + q) ? 0.f : (float)t[q];
+}
+
+// ---------------------------------------------------------------------------
 // Per-problem PCG: H dx = -g on the free DOFs, FP32 CG with FP64 residual
 // replacement whenever the FP32 recurrence residual has dropped by
 // inner_rtol; stops at ||b - H dx|| <= pcg_rtol ||b|| (FP64).
 // Thread rows: row slot i = threadIdx.x / LANES covers vertices v0 + i,
 // v0 + i + NT / LANES, ...
 // ---------------------------------------------------------------------------
-template <int NT, int LANES, bool KLDS>
+template <int NT, int LANES, bool KLDS, bool KC>
 __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
   __shared__ double lds[4 * (NT / kWave)];
-  extern __shared__ float vec_lds[];  // KLDS: p, r, Ap of the problem (3 max_rows floats each)
+  // KLDS: p, r (and A p without KC) of the problem; KC: the coarse scratch after them (declared
+  // as double so that the base is 8-byte aligned for the scratch's FP64 part)
+  extern __shared__ double dyn_d[];
+  float* dyn = reinterpret_cast<float*>(dyn_d);
   constexpr int RS = NT / LANES;  // rows per sweep
+  constexpr int rpw = kWave / LANES;
   const int j = threadIdx.x % LANES, slot = threadIdx.x / LANES;
+  const int64_t wrow = (int64_t)(threadIdx.x / kWave) * rpw;  // this wave's first row within a sweep
+  CoarseLds cl{};
+  if (KC) cl = coarse_lds(dyn, a);
   for (int p = blockIdx.x; p < a.P; p += gridDim.x) {
+    int nagg = 0;
+    if (KC) {
+      nagg = a.cagg0[p + 1] - a.cagg0[p];
+      const int nraw = nagg * (a.agg_rows / rpw) * 6;
+      for (int t = threadIdx.x; t < nraw; t += NT) cl.raw[t] = 0.f;
+    }
     if (!block_read(a.active + p)) continue;
     const int64_t v0 = a.vstart[p], v1 = a.vstart[p + 1];
     const int64_t nrow = v1 - v0;
     const int64_t sweeps = (nrow + RS - 1) / RS;
-    // p, r and Ap: in LDS (indexed from the problem's first DOF) or in the global vectors
+    // p and r: in LDS (indexed from the problem's first DOF) or in the global
+    // vectors; A p in LDS only without the coarse scratch
     const int64_t off = KLDS ? 3 * v0 : 0;
-    float* P = KLDS ? vec_lds : a.p32;
-    float* Rv = KLDS ? vec_lds + 3 * a.max_rows : a.r32;
-    float* AP = KLDS ? vec_lds + 6 * a.max_rows : a.Ap32;
-    // init: r64 = b = -g (free), dx = 0, r32 = b, x32 = 0, z = B^-1 r, p = z
+    const int64_t offA = (KLDS && !KC) ? 3 * v0 : 0;
+    float* P = KLDS ? dyn : a.p32;
+    float* Rv = KLDS ? dyn + 3 * a.max_rows : a.r32;
+    float* AP = (KLDS && !KC) ? dyn + 6 * a.max_rows : a.Ap32;
+    // init: r64 = b = -g (free), dx = 0, r32 = b, x32 = 0, z = M^-1 r, p = z
     double acc[2] = {0.0, 0.0};
     for (int64_t s = 0; s < sweeps; ++s) {
       const int64_t v = v0 + s * RS + slot;
+      float tr[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
       if (v < v1 && j == 0) {
         float r[3], z[3];
 #pragma unroll
@@ -228,14 +381,31 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
         apply_binv3(a.binv + 9 * v, r[0], r[1], r[2], z);
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-          P[3 * v + q - off] = z[q];
+          if (!KC) P[3 * v + q - off] = z[q];
           acc[0] += (double)r[q] * (double)z[q];
         }
+        if (KC) restrict_terms(a, v, r, tr);
       }
+      if (KC) restrict_wave<LANES>(a, cl, s * RS + wrow, nagg, tr);
     }
     all_sum<NT, 2>(acc, lds);
     double rz = acc[0];
     const double bb = acc[1];
+    if (KC) {
+      rz += coarse_apply<NT, LANES>(a, cl, p, lds);
+      for (int64_t s = 0; s < sweeps; ++s) {
+        const int64_t v = v0 + s * RS + slot;
+        if (v < v1 && j == 0) {
+          float z[3], zc[3];
+          const float* rv = Rv + 3 * v - off;
+          apply_binv3(a.binv + 9 * v, rv[0], rv[1], rv[2], z);
+          coarse_term(a, cl, v, v - v0, zc);
+#pragma unroll
+          for (int q = 0; q < 3; ++q) P[3 * v + q - off] = z[q] + zc[q];
+        }
+      }
+      __syncthreads();
+    }
     const double target = a.pcg_rtol2 * bb;
     double seg_rr = bb;
     int it = 0, refines = 0, st = SK_OK;
@@ -245,7 +415,11 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
       done = true;
     }
     while (!done && it < a.pcg_max_iter) {
-      // Ap = A p (fixed rows 0), p.Ap
+      // Ap = A p (fixed rows 0), p.Ap; the restriction slots are cleared for this iteration
+      if (KC) {
+        const int nraw = nagg * (a.agg_rows / rpw) * 6;
+        for (int t = threadIdx.x; t < nraw; t += NT) cl.raw[t] = 0.f;
+      }
       double d1[1] = {0.0};
       for (int64_t s = 0; s < sweeps; ++s) {
         const int64_t v = v0 + s * RS + slot;
@@ -256,7 +430,7 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
           for (int q = 0; q < 3; ++q) {
             const int64_t k = 3 * v + q;
             const float ap = is_fixed(a.fixed, k) ? 0.f : y[q];
-            AP[k - off] = ap;
+            AP[k - offA] = ap;
             d1[0] += (double)P[k - off] * (double)ap;
           }
         }
@@ -268,21 +442,38 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
         break;
       }
       const float alpha = (float)(rz / pAp);
-      double d2[1] = {0.0};
+      // x += alpha p, r -= alpha A p, r.r; with KC also r.(B^-1 r) and the restriction of r
+      double d2[2] = {0.0, 0.0};
       for (int64_t s = 0; s < sweeps; ++s) {
         const int64_t v = v0 + s * RS + slot;
+        float tr[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         if (v < v1 && j == 0) {
+          float r[3];
 #pragma unroll
           for (int q = 0; q < 3; ++q) {
             const int64_t k = 3 * v + q;
             a.x32[k] += alpha * P[k - off];
-            const float r = Rv[k - off] - alpha * AP[k - off];
-            Rv[k - off] = r;
-            d2[0] += (double)r * (double)r;
+            r[q] = Rv[k - off] - alpha * AP[k - offA];
+            Rv[k - off] = r[q];
+            d2[0] += (double)r[q] * (double)r[q];
+          }
+          if (KC) {
+            float z[3];
+            apply_binv3(a.binv + 9 * v, r[0], r[1], r[2], z);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) d2[1] += (double)r[q] * (double)z[q];
+            restrict_terms(a, v, r, tr);
           }
         }
+        if (KC) restrict_wave<LANES>(a, cl, s * RS + wrow, nagg, tr);
       }
-      all_sum<NT, 1>(d2, lds);
+      if (KC) {
+        all_sum<NT, 2>(d2, lds);
+      } else {
+        double t1[1] = {d2[0]};
+        all_sum<NT, 1>(t1, lds);
+        d2[0] = t1[0];
+      }
       ++it;
       if (!isfinite(d2[0])) {
         st = SK_ERR_NONFINITE;
@@ -302,26 +493,63 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
           }
         }
         __syncthreads();
-        double d3[1] = {0.0};
+        double d3[2] = {0.0, 0.0};
         for (int64_t s = 0; s < sweeps; ++s) {
           const int64_t v = v0 + s * RS + slot;
           double y[3];
+          float tr[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
           row_product<LANES, double>(a, v < v1 ? v : -1, a.dx, j, y);
           if (v < v1 && j == 0) {
+            float rf[3];
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
               const int64_t k = 3 * v + q;
               const double r = is_fixed(a.fixed, k) ? 0.0 : -a.g[k] - y[q];
               a.r64[k] = r;
-              Rv[k - off] = (float)r;
+              rf[q] = (float)r;
+              Rv[k - off] = rf[q];
               d3[0] += r * r;
             }
+            if (KC) {
+              float z[3];
+              apply_binv3(a.binv + 9 * v, rf[0], rf[1], rf[2], z);
+#pragma unroll
+              for (int q = 0; q < 3; ++q) d3[1] += (double)rf[q] * (double)z[q];
+              restrict_terms(a, v, rf, tr);
+            }
           }
+          if (KC) restrict_wave<LANES>(a, cl, s * RS + wrow, nagg, tr);
         }
-        all_sum<NT, 1>(d3, lds);
+        if (KC) {
+          all_sum<NT, 2>(d3, lds);
+        } else {
+          double t1[1] = {d3[0]};
+          all_sum<NT, 1>(t1, lds);
+          d3[0] = t1[0];
+        }
         ++refines;
         if (d3[0] <= target) break;
         seg_rr = d3[0];
+        d2[1] = d3[1];
+      }
+      if (KC) {
+        // z = B^-1 r + W E^-1 W^T r; r.z = r.B^-1 r + c.y; p = z + beta p
+        const double rz_new = d2[1] + coarse_apply<NT, LANES>(a, cl, p, lds);
+        const float beta = (float)(rz_new / rz);
+        rz = rz_new;
+        for (int64_t s = 0; s < sweeps; ++s) {
+          const int64_t v = v0 + s * RS + slot;
+          if (v < v1 && j == 0) {
+            float z[3], zc[3];
+            const float* rv = Rv + 3 * v - off;
+            apply_binv3(a.binv + 9 * v, rv[0], rv[1], rv[2], z);
+            coarse_term(a, cl, v, v - v0, zc);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) P[3 * v + q - off] = z[q] + zc[q] + beta * P[3 * v + q - off];
+          }
+        }
+        __syncthreads();
+        continue;
       }
       // z = B^-1 r, r.z, then p = z + beta p
       double d4[1] = {0.0};
@@ -540,6 +768,15 @@ struct sk_batch {
   int graph_block = 0, graph_lanes = 0, graph_lds = 0;
   int grid = 0;
   int max_rows = 0;
+  // two-level preconditioner (agg_rows > 0 when enabled)
+  int agg_rows = 0, n_agg = 0, max_agg = 0, max_nc = 0;
+  bool coarse_built = false, coarse_every_step = false;
+  DevBuf<float> cdv, ctm, einv;
+  DevBuf<int32_t> cagg0, ccol, cagg_prob, cadj_ptr, cadj, cnq;
+  DevBuf<int64_t> einv_off;
+  DevBuf<double> cE;
+  DevBuf<int> cbad;
+  bool coarse() const { return agg_rows > 0; }
   BatchArgs args() const;
 };
 
@@ -592,25 +829,53 @@ BatchArgs sk_batch::args() const {
   a.newton_max_iter = prm.newton_max_iter;
   a.count_pcg = 1;
   a.max_rows = max_rows;
+  a.cdv = cdv.p;
+  a.ctm = ctm.p;
+  a.cagg0 = cagg0.p;
+  a.ccol = ccol.p;
+  a.einv_off = einv_off.p;
+  a.einv = einv.p;
+  a.agg_rows = agg_rows;
+  a.max_agg = max_agg;
+  a.max_nc = max_nc;
   return a;
 }
 
 namespace {
 
 // Whether the problems' p, r and Ap (3 max_rows floats each) fit the LDS of one workgroup.
-size_t pcg_lds_bytes(const sk_batch* B) { return 9 * (size_t)B->max_rows * sizeof(float); }
+// Dynamic LDS of the PCG kernel: the LDS-resident vectors (p, r and, without
+// the coarse scratch, A p) followed by the coarse scratch.
+int pcg_vec_floats(const sk_batch* B, int lds) { return lds ? (B->coarse() ? 6 : 9) * B->max_rows : 0; }
+size_t pcg_shm_bytes(const sk_batch* B, int lds) {
+  size_t b = (size_t)pcg_vec_floats(B, lds) * sizeof(float);
+  if (B->coarse())
+    b += (size_t)coarse_raw_floats(B->max_agg, B->agg_rows) * sizeof(float) +
+         (2 * (size_t)B->max_nc + 6 * (size_t)B->max_agg) * sizeof(double);
+  return b;
+}
+size_t pcg_lds_bytes(const sk_batch* B) { return pcg_shm_bytes(B, 1); }
 // 60 KiB keeps the total, with the kernel's static reduction scratch, under 64 KiB.
 bool pcg_lds_fits(const sk_batch* B) { return B->max_rows > 0 && pcg_lds_bytes(B) <= 60 * 1024; }
 
 int launch_pcg(const sk_batch* B, const BatchArgs& a, int block, int lanes, int lds) {
   const int g = B->grid;
-  const size_t shm = lds ? pcg_lds_bytes(B) : 0;
+  const size_t shm = pcg_shm_bytes(B, lds);
   SK_REQUIRE(!lds || pcg_lds_fits(B), "sk_batch: problems too large for the LDS-resident PCG");
-#define SK_PCG_CASE(NT, L)                                                 \
-  if (block == NT && lanes == L) {                                         \
-    if (lds) batch_pcg_kernel<NT, L, true><<<g, NT, shm, B->st>>>(a);      \
-    else batch_pcg_kernel<NT, L, false><<<g, NT, 0, B->st>>>(a);           \
-    return SK_OK;                                                          \
+  SK_REQUIRE(!B->coarse() || B->coarse_built, "sk_batch: coarse operator not built");
+  BatchArgs a2 = a;
+  a2.vec_floats = pcg_vec_floats(B, lds);
+  const bool kc = B->coarse();
+#define SK_PCG_CASE(NT, L)                                                         \
+  if (block == NT && lanes == L) {                                                 \
+    if (kc) {                                                                      \
+      if (lds) batch_pcg_kernel<NT, L, true, true><<<g, NT, shm, B->st>>>(a2);     \
+      else batch_pcg_kernel<NT, L, false, true><<<g, NT, shm, B->st>>>(a2);        \
+    } else {                                                                       \
+      if (lds) batch_pcg_kernel<NT, L, true, false><<<g, NT, shm, B->st>>>(a2);    \
+      else batch_pcg_kernel<NT, L, false, false><<<g, NT, 0, B->st>>>(a2);         \
+    }                                                                              \
+    return SK_OK;                                                                  \
   }
   SK_PCG_CASE(128, 1) SK_PCG_CASE(128, 4) SK_PCG_CASE(256, 1) SK_PCG_CASE(256, 4)
   SK_PCG_CASE(512, 1) SK_PCG_CASE(512, 4) SK_PCG_CASE(1024, 1) SK_PCG_CASE(1024, 4)
@@ -635,6 +900,166 @@ int launch_ls(const sk_batch* B, const BatchArgs& a, int block) {
   if (block == 1024) return launch_ls_nt<1024>(B, a);
   set_error("sk_batch: unsupported block");
   return SK_ERR_INVALID;
+}
+
+// ---------------------------------------------------------------------------
+// Set-up of the per-problem coarse operators (once, from an assembled system).
+// ---------------------------------------------------------------------------
+struct GalerkinBatchArgs {
+  const int32_t* vstart;
+  const int32_t* bptr;
+  const int32_t* bcol;
+  const float* vals;       // BSR FP32
+  const uint8_t* fixed;
+  const float* cdv;
+  const float* ctm;
+  const int32_t* cagg0;
+  const int32_t* ccol;
+  const int32_t* cagg_prob;
+  const int32_t* cadj_ptr;
+  const int32_t* cadj;
+  const int32_t* cnq;      // [P] coarse unknowns per problem
+  const int64_t* einv_off;
+  double* E;               // per problem, n_q x n_q, at einv_off[q]
+  int agg_rows;
+};
+
+// Row p of R_v = [I | e_0 x d, e_1 x d, e_2 x d] (3 x 6).
+__device__ __forceinline__ void rigid_row_f(const float* d, int p, double (&out)[6]) {
+  const double dx = d[0], dy = d[1], dz = d[2];
+  out[0] = p == 0;
+  out[1] = p == 1;
+  out[2] = p == 2;
+  const double c0[3] = {0.0, -dz, dy}, c1[3] = {dz, 0.0, -dx}, c2[3] = {-dy, dx, 0.0};
+  out[3] = c0[p];
+  out[4] = c1[p];
+  out[5] = c2[p];
+}
+
+// Workgroup g (aggregate): E_q blocks (g, J) = T_g^T (sum over blocks (v, w),
+// v in g, w in J, of R_v^T M_v A_vw M_w R_w) T_J for every adjacent J, with
+// fixed-order block sums.
+__global__ __launch_bounds__(kBlock) void batch_galerkin_kernel(GalerkinBatchArgs a) {
+  __shared__ double lds[36 * kBlock / kWave];
+  const int g = blockIdx.x, q = a.cagg_prob[g];
+  const int64_t v0 = a.vstart[q], v1 = a.vstart[q + 1];
+  const int g0 = a.cagg0[q], l = g - g0;
+  const int64_t r0 = v0 + (int64_t)l * a.agg_rows, r1 = std::min<int64_t>(v1, r0 + a.agg_rows);
+  const int cb = a.ccol[g0], nq = a.cnq[q];
+  const int cI = a.ccol[g] - cb, kI = a.ccol[g + 1] - a.ccol[g];
+  for (int jj = a.cadj_ptr[g]; jj < a.cadj_ptr[g + 1]; ++jj) {
+    const int J = a.cadj[jj];
+    const int cJ = a.ccol[J] - cb, kJ = a.ccol[J + 1] - a.ccol[J];
+    double acc[36];
+#pragma unroll
+    for (int k = 0; k < 36; ++k) acc[k] = 0.0;
+    for (int64_t v = r0 + threadIdx.x; v < r1; v += blockDim.x) {
+      for (int32_t b = a.bptr[v]; b < a.bptr[v + 1]; ++b) {
+        const int64_t w = a.bcol[b];
+        if (g0 + (int)((w - v0) / a.agg_rows) != J) continue;
+        double A[9];
+#pragma unroll
+        for (int m = 0; m < 9; ++m) {
+          const bool fr = !(a.fixed && (a.fixed[3 * v + m / 3] || a.fixed[3 * w + m % 3]));
+          A[m] = fr ? (double)a.vals[9 * (int64_t)b + m] : 0.0;
+        }
+        double Rw[3][6], AR[3][6];
+#pragma unroll
+        for (int p = 0; p < 3; ++p) rigid_row_f(a.cdv + 3 * w, p, Rw[p]);
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+#pragma unroll
+          for (int c = 0; c < 6; ++c) AR[p][c] = A[3 * p] * Rw[0][c] + A[3 * p + 1] * Rw[1][c] + A[3 * p + 2] * Rw[2][c];
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+          double Rv[6];
+          rigid_row_f(a.cdv + 3 * v, p, Rv);
+#pragma unroll
+          for (int r = 0; r < 6; ++r)
+#pragma unroll
+            for (int c = 0; c < 6; ++c) acc[6 * r + c] += Rv[r] * AR[p][c];
+        }
+      }
+    }
+    block_sum<36>(acc, lds);
+    if (threadIdx.x == 0) {
+      const float* TI = a.ctm + 36 * (int64_t)g;
+      const float* TJ = a.ctm + 36 * (int64_t)J;
+      double* E = a.E + a.einv_off[q];
+      for (int j = 0; j < kI; ++j)
+        for (int m = 0; m < kJ; ++m) {
+          double s = 0.0;
+          for (int r = 0; r < 6; ++r)
+            for (int c = 0; c < 6; ++c) s += (double)TI[6 * r + j] * acc[6 * r + c] * (double)TJ[6 * c + m];
+          E[(int64_t)(cI + j) * nq + cJ + m] = s;
+        }
+    }
+    __syncthreads();
+  }
+}
+
+// Workgroup q: Gauss-Jordan inversion of E_q in place (symmetric positive
+// definite, no pivoting), then E_q^-1 symmetrised into FP32. A non-positive
+// or non-finite pivot sets bad[q] and stops that problem's elimination.
+constexpr int kGjBlock = 1024;
+
+__global__ __launch_bounds__(kGjBlock) void batch_gj_kernel(double* E, const int64_t* off, const int32_t* nq_arr,
+                                                            float* einv, int* bad) {
+  __shared__ double s_piv;
+  const int q = blockIdx.x, n = nq_arr[q];
+  double* A = E + off[q];
+  for (int k = 0; k < n; ++k) {
+    if (threadIdx.x == 0) s_piv = A[(int64_t)k * n + k];
+    __syncthreads();
+    const double piv = s_piv;
+    if (!(piv > 0.0) || !isfinite(piv)) {
+      if (threadIdx.x == 0) bad[q] = 1;
+      return;  // uniform: every thread read the same pivot
+    }
+    for (int64_t idx = threadIdx.x; idx < (int64_t)n * n; idx += blockDim.x) {
+      const int i = (int)(idx / n), j = (int)(idx % n);
+      if (i != k && j != k) A[idx] -= A[(int64_t)i * n + k] * A[(int64_t)k * n + j] / piv;
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < 2 * n; idx += blockDim.x) {
+      if (idx < n) {
+        A[(int64_t)k * n + idx] = (idx == k) ? 1.0 / piv : A[(int64_t)k * n + idx] / piv;
+      } else if (idx - n != k) {
+        A[(int64_t)(idx - n) * n + k] = -A[(int64_t)(idx - n) * n + k] / piv;
+      }
+    }
+    __syncthreads();
+  }
+  for (int64_t idx = threadIdx.x; idx < (int64_t)n * n; idx += blockDim.x) {
+    const int i = (int)(idx / n), j = (int)(idx % n);
+    einv[off[q] + idx] = (float)(0.5 * (A[idx] + A[(int64_t)j * n + i]));
+  }
+}
+
+int hessian_phase(sk_batch* B);
+
+// Forms and inverts every problem's coarse operator from the system assembled
+// at the batch's current positions (synchronises the batch stream).
+int build_coarse(sk_batch* B) {
+  SK_TRY(hessian_phase(B));
+  GalerkinBatchArgs g{B->vstart.p, B->mesh->bptr.p, B->mesh->bcol.p, B->vals32.p, B->fixed.p, B->cdv.p, B->ctm.p,
+                      B->cagg0.p, B->ccol.p, B->cagg_prob.p, B->cadj_ptr.p, B->cadj.p, B->cnq.p, B->einv_off.p,
+                      B->cE.p, B->agg_rows};
+  SK_HIP_CHECK(hipMemsetAsync(B->cE.p, 0, B->cE.n * sizeof(double), B->st));
+  SK_HIP_CHECK(hipMemsetAsync(B->cbad.p, 0, B->P * sizeof(int), B->st));
+  batch_galerkin_kernel<<<B->n_agg, kBlock, 0, B->st>>>(g);
+  batch_gj_kernel<<<B->P, kGjBlock, 0, B->st>>>(B->cE.p, B->einv_off.p, B->cnq.p, B->einv.p, B->cbad.p);
+  SK_HIP_CHECK(hipGetLastError());
+  std::vector<int> bad(B->P);
+  SK_HIP_CHECK(hipMemcpyAsync(bad.data(), B->cbad.p, B->P * sizeof(int), hipMemcpyDeviceToHost, B->st));
+  SK_HIP_CHECK(hipStreamSynchronize(B->st));
+  for (int q = 0; q < B->P; ++q)
+    if (bad[q]) {
+      set_error("sk_batch: coarse operator of problem " + std::to_string(q) + " is not positive definite");
+      return SK_ERR_PCG_BREAKDOWN;
+    }
+  B->coarse_built = true;
+  return SK_OK;
 }
 
 int gradient_phase(sk_batch* B) {
@@ -835,6 +1260,7 @@ int sk_batch_step(sk_batch* B, const double* x_curr, const double* x_prev, const
   SK_HIP_CHECK(hipStreamWaitEvent(B->st, B->ev_in, 0));
   const int block = batch_block(B), lanes = batch_lanes(B), lds = batch_lds(B);
   SK_TRY(start_step(B, x_curr, x_prev, x_bc));
+  if (B->coarse() && (!B->coarse_built || B->coarse_every_step)) SK_TRY(build_coarse(B));
   int n_active = 0, r = 0;
   SK_TRY(read_active(B, &n_active));
   if (B->prm.use_graph) SK_TRY(ensure_graph(B, block, lanes, lds));
@@ -879,6 +1305,103 @@ int sk_batch_step(sk_batch* B, const double* x_curr, const double* x_prev, const
   return SK_OK;
 }
 
+int sk_batch_set_coarse(sk_batch* B, const double* X, int agg_rows, int every_step) {
+  SK_REQUIRE(B, "sk_batch_set_coarse: null batch");
+  if (B->exec) (void)hipGraphExecDestroy(B->exec);
+  if (B->graph) (void)hipGraphDestroy(B->graph);
+  B->exec = nullptr;
+  B->graph = nullptr;
+  B->agg_rows = B->n_agg = B->max_agg = B->max_nc = 0;
+  B->coarse_built = false;
+  for (DevBuf<float>* b : {&B->cdv, &B->ctm, &B->einv}) b->release();
+  for (DevBuf<int32_t>* b : {&B->cagg0, &B->ccol, &B->cagg_prob, &B->cadj_ptr, &B->cadj, &B->cnq}) b->release();
+  B->einv_off.release();
+  B->cE.release();
+  B->cbad.release();
+  if (agg_rows <= 0) return SK_OK;
+  SK_REQUIRE(X, "sk_batch_set_coarse: null positions");
+  SK_REQUIRE(agg_rows % kWave == 0 && agg_rows <= 512, "sk_batch_set_coarse: agg_rows must be 64, 128, ..., 512");
+  sk_mesh* m = B->mesh;
+  const int64_t nv = m->nv;
+  const int P = B->P;
+  std::vector<int32_t> vs(P + 1), bptr(nv + 1), bcol(m->nblk);
+  SK_HIP_CHECK(hipMemcpy(vs.data(), B->vstart.p, (P + 1) * sizeof(int32_t), hipMemcpyDeviceToHost));
+  SK_HIP_CHECK(hipMemcpy(bptr.data(), m->bptr.p, (nv + 1) * sizeof(int32_t), hipMemcpyDeviceToHost));
+  if (m->nblk) SK_HIP_CHECK(hipMemcpy(bcol.data(), m->bcol.p, m->nblk * sizeof(int32_t), hipMemcpyDeviceToHost));
+  std::vector<uint8_t> fixed;
+  if (B->fixed.p) {
+    fixed.resize(3 * nv);
+    SK_HIP_CHECK(hipMemcpy(fixed.data(), B->fixed.p, 3 * nv, hipMemcpyDeviceToHost));
+  }
+  // aggregates: runs of agg_rows consecutive vertices of each problem
+  std::vector<int32_t> cagg0(P + 1, 0), ccol(1, 0), cagg_prob, cnq(P), vagg(nv);
+  std::vector<int64_t> einv_off(P);
+  std::vector<double> dv(3 * nv), tm;
+  int64_t etot = 0;
+  int max_agg = 0, max_nc = 0;
+  for (int q = 0; q < P; ++q) {
+    const int64_t v0 = vs[q], v1 = vs[q + 1];
+    const int nagg = (int)((v1 - v0 + agg_rows - 1) / agg_rows);
+    cagg0[q + 1] = cagg0[q] + nagg;
+    const int c_start = ccol.back();
+    for (int l = 0; l < nagg; ++l) {
+      const int64_t r0 = v0 + (int64_t)l * agg_rows, r1 = std::min<int64_t>(v1, r0 + agg_rows);
+      double cen[3] = {0.0, 0.0, 0.0};
+      for (int64_t v = r0; v < r1; ++v)
+        for (int a = 0; a < 3; ++a) cen[a] += X[3 * v + a];
+      for (int a = 0; a < 3; ++a) cen[a] /= (double)(r1 - r0);
+      for (int64_t v = r0; v < r1; ++v) {
+        vagg[v] = cagg0[q] + l;
+        for (int a = 0; a < 3; ++a) dv[3 * v + a] = X[3 * v + a] - cen[a];
+      }
+      double t[36];
+      const int k = rigid_basis(dv.data(), fixed.empty() ? nullptr : fixed.data(), nullptr, (int)(r1 - r0), r0, t);
+      tm.insert(tm.end(), t, t + 36);
+      ccol.push_back(ccol.back() + k);
+      cagg_prob.push_back(q);
+    }
+    cnq[q] = ccol.back() - c_start;
+    einv_off[q] = etot;
+    etot += (int64_t)cnq[q] * cnq[q];
+    max_agg = std::max(max_agg, nagg);
+    max_nc = std::max(max_nc, cnq[q]);
+  }
+  const int n_agg = cagg0[P];
+  SK_REQUIRE(max_nc <= 1024, "sk_batch_set_coarse: a problem has more than 1024 coarse unknowns");
+  std::vector<std::vector<int32_t>> nbr(n_agg);
+  for (int64_t v = 0; v < nv; ++v)
+    for (int32_t b = bptr[v]; b < bptr[v + 1]; ++b) nbr[vagg[v]].push_back(vagg[bcol[b]]);
+  std::vector<int32_t> adj_ptr(n_agg + 1, 0), adj;
+  for (int g = 0; g < n_agg; ++g) {
+    auto& l = nbr[g];
+    std::sort(l.begin(), l.end());
+    l.erase(std::unique(l.begin(), l.end()), l.end());
+    adj.insert(adj.end(), l.begin(), l.end());
+    adj_ptr[g + 1] = (int32_t)adj.size();
+  }
+  B->agg_rows = agg_rows;
+  B->coarse_every_step = every_step != 0;
+  B->n_agg = n_agg;
+  B->max_agg = max_agg;
+  B->max_nc = max_nc;
+  auto fail = [&](int st) {
+    B->agg_rows = 0;
+    return st;
+  };
+  if (pcg_shm_bytes(B, 0) > 60 * 1024) {
+    set_error("sk_batch_set_coarse: coarse scratch exceeds the workgroup's LDS");
+    return fail(SK_ERR_INVALID);
+  }
+  std::vector<float> dv32(dv.begin(), dv.end()), tm32(tm.begin(), tm.end());
+  int s = SK_OK;
+  if ((s = B->cdv.upload(dv32)) || (s = B->ctm.upload(tm32)) || (s = B->cagg0.upload(cagg0)) ||
+      (s = B->ccol.upload(ccol)) || (s = B->cagg_prob.upload(cagg_prob)) || (s = B->cadj_ptr.upload(adj_ptr)) ||
+      (s = B->cadj.upload(adj)) || (s = B->cnq.upload(cnq)) || (s = B->einv_off.upload(einv_off)) ||
+      (s = B->cE.alloc(etot)) || (s = B->einv.alloc(etot)) || (s = B->cbad.alloc(P)))
+    return fail(s);
+  return SK_OK;
+}
+
 int sk_batch_set_shape(sk_batch* B, int block, int lanes, int lds) {
   SK_REQUIRE(B, "sk_batch_set_shape: null batch");
   SK_REQUIRE(block == 128 || block == 256 || block == 512 || block == 1024, "sk_batch_set_shape: bad block");
@@ -899,6 +1422,7 @@ int sk_batch_autotune(sk_batch* B, const double* x_curr, const double* x_prev, c
   SK_HIP_CHECK(hipStreamWaitEvent(B->st, B->ev_in, 0));
   // The first Newton system of a step: gradient, Hessian, gather.
   SK_TRY(start_step(B, x_curr, x_prev, x_bc));
+  if (B->coarse() && !B->coarse_built) SK_TRY(build_coarse(B));
   SK_TRY(hessian_phase(B));
   BatchArgs a = B->args();
   a.count_pcg = 0;

@@ -67,6 +67,14 @@ inline CoarseView<float> CoarseSpace::view<float>() const {
   return {n_agg, nc, ld, agg_ptr.p, agg_vert.p, agg_col.p, dv32.p, dva32.p, tm.p, einv.p, c.p};
 }
 
+// Orthonormal basis of an aggregate's rigid motions restricted to its free
+// DOFs (host): the Gram matrix of the 6 raw motions (offsets dv[3 v], v in
+// verts[0..nverts) or vbase + 0..nverts when verts is null; fixed: per-DOF
+// 0/1 or null) is eigen-decomposed, eigenvalues above 1e-9 of the largest
+// are kept. tm[6 a + j]: column j in raw coordinates a; returns the count.
+int rigid_basis(const double* dv, const uint8_t* fixed, const int32_t* verts, int nverts, int64_t vbase,
+                double tm[36]);
+
 // Aggregates from the rest positions X (host, nv x 3): each axis of the
 // bounding box is cut evenly into bins of close to the edge s of a cube that
 // holds per_agg vertices at the mesh's mean density; non-empty bins become

@@ -179,6 +179,33 @@ __global__ __launch_bounds__(kBlock) void symmetrize_f32_kernel(const double* A,
 
 }  // namespace
 
+int rigid_basis(const double* dv, const uint8_t* fixed, const int32_t* verts, int nverts, int64_t vbase,
+                double tm[36]) {
+  double G[6][6] = {};
+  for (int t = 0; t < nverts; ++t) {
+    const int64_t v = verts ? verts[t] : vbase + t;
+    for (int p = 0; p < 3; ++p) {
+      if (fixed && fixed[3 * v + p]) continue;
+      double R[6];
+      rigid_row(&dv[3 * v], p, R);
+      for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 6; ++c) G[r][c] += R[r] * R[c];
+    }
+  }
+  double w[6], V[36];
+  jacobi6(G, w, V);
+  const double wmax = *std::max_element(w, w + 6);
+  for (int i = 0; i < 36; ++i) tm[i] = 0.0;
+  int k = 0;
+  for (int j = 0; j < 6; ++j) {
+    if (!(wmax > 0.0) || !(w[j] > 1e-9 * wmax)) continue;
+    const double sc = 1.0 / std::sqrt(w[j]);
+    for (int a = 0; a < 6; ++a) tm[6 * a + k] = V[6 * j + a] * sc;
+    ++k;
+  }
+  return k;
+}
+
 int coarse_setup(CoarseSpace& C, sk_mesh* m, const double* X, const uint8_t* fixed_host, int per_agg) {
   SK_REQUIRE(X, "coarse set-up: null positions");
   SK_REQUIRE(per_agg >= 1, "coarse set-up: vertices per aggregate must be >= 1");
@@ -236,27 +263,8 @@ int coarse_setup(CoarseSpace& C, sk_mesh* m, const double* X, const uint8_t* fix
     for (int a = 0; a < 3; ++a) dv[3 * v + a] = X[3 * v + a] - cen[3 * vagg[v] + a];
   std::vector<int32_t> agg_col(n_agg + 1, 0);
   for (int I = 0; I < n_agg; ++I) {
-    double G[6][6] = {};
-    for (int t = agg_ptr[I]; t < agg_ptr[I + 1]; ++t) {
-      const int64_t v = agg_vert[t];
-      for (int p = 0; p < 3; ++p) {
-        if (fixed_host && fixed_host[3 * v + p]) continue;
-        double R[6];
-        rigid_row(&dv[3 * v], p, R);
-        for (int r = 0; r < 6; ++r)
-          for (int c = 0; c < 6; ++c) G[r][c] += R[r] * R[c];
-      }
-    }
-    double w[6], V[36];
-    jacobi6(G, w, V);
-    const double wmax = *std::max_element(w, w + 6);
-    int k = 0;
-    for (int j = 0; j < 6; ++j) {
-      if (!(wmax > 0.0) || !(w[j] > 1e-9 * wmax)) continue;
-      const double sc = 1.0 / std::sqrt(w[j]);
-      for (int a = 0; a < 6; ++a) tm[36 * (size_t)I + 6 * a + k] = V[6 * j + a] * sc;
-      ++k;
-    }
+    const int k = rigid_basis(dv.data(), fixed_host, agg_vert.data() + agg_ptr[I], agg_ptr[I + 1] - agg_ptr[I], 0,
+                              &tm[36 * (size_t)I]);
     agg_col[I + 1] = agg_col[I] + k;
   }
   const int nc = agg_col[n_agg];

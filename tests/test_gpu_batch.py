@@ -111,3 +111,46 @@ def test_damped_stiff_beam_stops_at_position_resolution():
     for n in range(1, steps + 1):
         for x in (kept[n], bk[n][b.problem_slice(0)], bk[n][b.problem_slice(1)]):
             assert np.abs(x - ref[n - 1]).max() <= TOL * np.abs(ref[n - 1]).max()
+
+
+@pytest.mark.parametrize("block,lanes,lds,graph,rows", [(256, 1, 0, True, 64), (1024, 4, 1, True, 64),
+                                                         (128, 4, 0, False, 128), (512, 1, 1, False, 64),
+                                                         (1024, 1, 0, True, 192)])
+def test_batch_two_level_matches_single_mesh_sims(block, lanes, lds, graph, rows):
+    """Per-problem two-level PCG (rigid-motion aggregates of consecutive vertices): every beam still
+    matches the same beam advanced alone by the single-mesh simulator, per step at 1e-6."""
+    from simkit_amd.sim import BackwardEulerSim
+    lengths = [8, 13, 9, 16, 11, 10]
+    meshes, fixed, mu, lam = _beams(1, lengths)
+    steps = 6
+    b = _batch(meshes, fixed, mu, lam, block=block, lanes=lanes, lds=lds, use_graph=graph, coarse_rows=rows)
+    kept, infos, rounds = b.rollout(steps)
+    assert all(i["status"] == 0 for step in infos for i in step), infos
+    for p, (X, T) in enumerate(meshes):
+        s = BackwardEulerSim(X, T, mu[p], lam[p], rho=1000.0, fixed_vertices=fixed[p], gravity=-9.8,
+                             precision="mixed")
+        ref, _ = s.rollout(steps)
+        sl = b.problem_slice(p)
+        for n in range(1, steps + 1):
+            err = np.abs(kept[n][sl] - ref[n]).max() / np.abs(ref[n]).max()
+            assert err < TOL, (p, n, err)
+
+
+def test_batch_two_level_fewer_iterations_deterministic():
+    meshes, fixed, mu, lam = _beams(4, [40, 64, 25])
+    plain = _batch(meshes, fixed, mu, lam)
+    _, pinfo, _ = plain.rollout(4)
+    two = _batch(meshes, fixed, mu, lam, coarse_rows=64)
+    k1, i1, r1 = two.rollout(4)
+    k2, i2, r2 = two.rollout(4)
+    assert all(np.array_equal(k1[n], k2[n]) for n in k1) and i1 == i2 and r1 == r2
+    it_plain = sum(i["pcg_iters"] for st in pinfo for i in st)
+    it_two = sum(i["pcg_iters"] for st in i1 for i in st)
+    assert it_two < 0.5 * it_plain, (it_two, it_plain)
+
+
+def test_batch_two_level_bad_rows():
+    from simkit_amd import _lib
+    meshes, fixed, mu, lam = _beams(5, [8, 9])
+    with pytest.raises(_lib.SimkitAmdError, match="agg_rows"):
+        _batch(meshes, fixed, mu, lam, coarse_rows=100)
