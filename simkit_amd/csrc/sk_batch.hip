@@ -1036,12 +1036,12 @@ __global__ __launch_bounds__(kGjBlock) void batch_gj_kernel(double* E, const int
   }
 }
 
-int hessian_phase(sk_batch* B);
+int hessian_phase(sk_batch* B, bool only_active);
 
 // Forms and inverts every problem's coarse operator from the system assembled
 // at the batch's current positions (synchronises the batch stream).
 int build_coarse(sk_batch* B) {
-  SK_TRY(hessian_phase(B));
+  SK_TRY(hessian_phase(B, false));
   GalerkinBatchArgs g{B->vstart.p, B->mesh->bptr.p, B->mesh->bcol.p, B->vals32.p, B->fixed.p, B->cdv.p, B->ctm.p,
                       B->cagg0.p, B->ccol.p, B->cagg_prob.p, B->cadj_ptr.p, B->cadj.p, B->cnq.p, B->einv_off.p,
                       B->cE.p, B->agg_rows};
@@ -1069,20 +1069,24 @@ int gradient_phase(sk_batch* B) {
                             1.0 / (B->prm.h * B->prm.h), B->st);
 }
 
-// Element Hessian + BSR FP32 gather (every problem; inactive ones are not read).
-int hessian_phase(sk_batch* B) {
+// Element Hessian + BSR FP32 gather. only_active: skip the elements and
+// blocks of problems whose active flag (read on the device) is 0 (their
+// values are not read until they are assembled again); else every problem.
+int hessian_phase(sk_batch* B, bool only_active) {
   sk_mesh* m = B->mesh;
+  const int* act = only_active ? B->active.p : nullptr;
+  const int32_t* pid = only_active ? B->pid.p : nullptr;
   SK_TRY(launch_elem_hess(m, B->prm.material, B->x.p, nullptr, B->mu.p, B->lam.p, B->vol.p, B->prm.psd, true,
-                          B->st));
+                          B->st, act, pid));
   return launch_gather_hess(m, B->vals32.p, B->mass.p, 1.0 / (B->prm.h * B->prm.h), B->fixed.p, B->binv32.p,
-                            kLayoutBSR, true, B->st);
+                            kLayoutBSR, true, B->st, act, pid);
 }
 
 // One Newton iteration of every active problem (no host synchronization).
 int newton_iteration(sk_batch* B, int block, int lanes, int lds) {
   const BatchArgs a = B->args();
   const int64_t n3 = 3 * B->mesh->nv;
-  SK_TRY(hessian_phase(B));
+  SK_TRY(hessian_phase(B, true));
   SK_TRY(launch_pcg(B, a, block, lanes, lds));
   SK_TRY(launch_ls(B, a, block));
   batch_update_kernel<<<grid_for(n3), kBlock, 0, B->st>>>(n3, a);
@@ -1423,7 +1427,7 @@ int sk_batch_autotune(sk_batch* B, const double* x_curr, const double* x_prev, c
   // The first Newton system of a step: gradient, Hessian, gather.
   SK_TRY(start_step(B, x_curr, x_prev, x_bc));
   if (B->coarse() && !B->coarse_built) SK_TRY(build_coarse(B));
-  SK_TRY(hessian_phase(B));
+  SK_TRY(hessian_phase(B, false));
   BatchArgs a = B->args();
   a.count_pcg = 0;
   hipEvent_t e0, e1;

@@ -121,8 +121,11 @@ int launch_elem_grad(sk_mesh* m, int material, const double* x, const double* F0
                      const double* lam, const double* vol, hipStream_t st);
 // fp32: element Hessian, PSD projection and staged blocks in FP32 (staging
 // buffer reinterpreted), otherwise FP64.
+// active / pid (optional, together): skip elements whose problem (pid of the
+// element's first vertex) has active == 0 (batched problems).
 int launch_elem_hess(sk_mesh* m, int material, const double* x, const double* F0, const double* mu,
-                     const double* lam, const double* vol, int psd, bool fp32, hipStream_t st);
+                     const double* lam, const double* vol, int psd, bool fp32, hipStream_t st,
+                     const int* active = nullptr, const int32_t* pid = nullptr);
 // Gathers staged element forces into g; with mass != nullptr adds
 // m (x - xt) * inv_h2 - fext, zeroes fixed DOFs and reduces ||g||^2 into red.result[0].
 int launch_gather_grad(sk_mesh* m, double* g, const double* mass, const double* x, const double* xt,
@@ -136,7 +139,8 @@ int launch_gather_grad(sk_mesh* m, double* g, const double* mass, const double* 
 // double.
 enum ValueLayout : int { kLayoutCSR = 0, kLayoutSELL = 1, kLayoutBSR = 2 };
 int launch_gather_hess(sk_mesh* m, void* values, const double* mass, double inv_h2, const uint8_t* fixed,
-                       void* binv, int layout, bool fp32, hipStream_t st);
+                       void* binv, int layout, bool fp32, hipStream_t st, const int* active = nullptr,
+                       const int32_t* pid = nullptr);
 // Copies n doubles from device to the mesh's pinned buffer and waits.
 int read_scalars(sk_mesh* m, const double* dev, int n, hipStream_t st);
 

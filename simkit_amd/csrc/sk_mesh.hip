@@ -93,6 +93,8 @@ struct ElemArgs {
   double* fst;
   double* Kst;
   int psd;
+  const int* active;    // optional (batched problems): skip elements whose problem is inactive
+  const int32_t* pid;   // problem of each vertex, with `active`
 };
 
 template <class M>
@@ -143,6 +145,7 @@ template <template <typename> class M, int PSD, typename R>
 __global__ __launch_bounds__(kBlock) void elem_hess_kernel(ElemArgs a) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= a.m.nt) return;
+  if (a.active && !a.active[a.pid[a.m.T[e]]]) return;
   int32_t v[4];
   double Dd[12], xe[12], F[9];
   load_elem(a.m, e, v, Dd);
@@ -244,6 +247,8 @@ struct GatherHArgs {
   void* binv;
   const int32_t* sell_pos;  // non-null: SELL layout
   int bsr;                  // 1: BSR layout (block b at values[9 b + 3 p + q])
+  const int* active;        // optional (batched problems): skip blocks whose problem is inactive
+  const int32_t* pid;
 };
 
 // One lane per block: sums the staged contributions in list order (precision
@@ -254,6 +259,7 @@ template <typename R>
 __global__ __launch_bounds__(kBlock) void gather_hess_kernel(GatherHArgs a) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= a.nblk) return;
+  if (a.active && !a.active[a.pid[a.brow[b]]]) return;
   const R* __restrict__ Kst = reinterpret_cast<const R*>(a.Kst);
   R acc[9];
 #pragma unroll
@@ -393,7 +399,7 @@ int launch_elem_energy(sk_mesh* m, int material, const double* x, const double* 
 
 int launch_elem_grad(sk_mesh* m, int material, const double* x, const double* F0, const double* mu,
                      const double* lam, const double* vol, hipStream_t st) {
-  ElemArgs a{view(m), x, F0, mu, lam, vol, m->fst.p, m->Kst.p, 0};
+  ElemArgs a{view(m), x, F0, mu, lam, vol, m->fst.p, m->Kst.p, 0, nullptr, nullptr};
   const int g = grid_for(m->nt);
   SK_REQUIRE(dispatch_material(material, [&](auto tag) {
                elem_grad_kernel<typename decltype(tag)::template type<double>><<<g, kBlock, 0, st>>>(a);
@@ -417,9 +423,11 @@ static void hess_dispatch_prec(bool fp32, int psd, int g, hipStream_t st, const 
 }
 
 int launch_elem_hess(sk_mesh* m, int material, const double* x, const double* F0, const double* mu,
-                     const double* lam, const double* vol, int psd, bool fp32, hipStream_t st) {
+                     const double* lam, const double* vol, int psd, bool fp32, hipStream_t st, const int* active,
+                     const int32_t* pid) {
   SK_REQUIRE(psd == SK_PSD_NONE || psd == SK_PSD_PROJ || psd == SK_PSD_ABS, "invalid psd method");
-  ElemArgs a{view(m), x, F0, mu, lam, vol, m->fst.p, m->Kst.p, psd};
+  SK_REQUIRE(!active == !pid, "launch_elem_hess: active and pid go together");
+  ElemArgs a{view(m), x, F0, mu, lam, vol, m->fst.p, m->Kst.p, psd, active, pid};
   const int g = grid_for(m->nt);
   bool known = true;
   switch (material) {
@@ -447,11 +455,12 @@ int launch_gather_grad(sk_mesh* m, double* g, const double* mass, const double* 
 }
 
 int launch_gather_hess(sk_mesh* m, void* values, const double* mass, double inv_h2, const uint8_t* fixed,
-                       void* binv, int layout, bool fp32, hipStream_t st) {
+                       void* binv, int layout, bool fp32, hipStream_t st, const int* active, const int32_t* pid) {
   SK_REQUIRE(layout == kLayoutCSR || layout == kLayoutSELL || layout == kLayoutBSR, "invalid value layout");
+  SK_REQUIRE(!active == !pid, "launch_gather_hess: active and pid go together");
   GatherHArgs a{m->nblk, m->bptr.p, m->bcol.p, m->brow.p, m->cptr.p, m->clist.p, m->Kst.p,
                 values, mass, inv_h2, fixed, binv, layout == kLayoutSELL ? m->sell_pos.p : nullptr,
-                layout == kLayoutBSR ? 1 : 0};
+                layout == kLayoutBSR ? 1 : 0, active, pid};
   if (fp32) gather_hess_kernel<float><<<grid_for(m->nblk), kBlock, 0, st>>>(a);
   else gather_hess_kernel<double><<<grid_for(m->nblk), kBlock, 0, st>>>(a);
   SK_HIP_CHECK(hipGetLastError());
