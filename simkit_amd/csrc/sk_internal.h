@@ -103,6 +103,13 @@ struct sk_mesh {
   sk::DevBuf<int32_t> scol;       // [nslot]
   sk::DevBuf<int32_t> sell_pos;   // [nblk] slot of each CSR block
   int64_t nchunk = 0, nslot = 0;
+  // Chunk order of the SpMV kernels, balanced over the 8 XCDs: XCD j (the
+  // hardware deals workgroup b to XCD b % 8) processes corder[j*lpad ..
+  // (j+1)*lpad), the j-th eighth of every run of equal-degree chunks, so each
+  // XCD gets the same share of heavy and light rows from one spatial region;
+  // -1 pads the lists to the common length lpad (a multiple of 4).
+  sk::DevBuf<int32_t> corder;
+  int64_t lpad = 0;
   sk::DevBuf<double> Kst;     // [nt][90] staged element stiffness blocks
   sk::DevBuf<double> fst;     // [nt][12] staged element forces
   sk::Reducer red;

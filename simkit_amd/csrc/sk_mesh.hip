@@ -552,6 +552,20 @@ static int build_mesh(const int32_t* T, int64_t nv, int64_t nt, sk_mesh* m) {
     chunk_off[c + 1] = chunk_off[c] + kSellC * d;
   }
   const int64_t nslot = chunk_off[nchunk];
+  // XCD-balanced chunk order (see sk_mesh::corder)
+  std::vector<std::vector<int32_t>> xl(8);
+  for (int64_t c0 = 0; c0 < nchunk;) {
+    int64_t c1 = c0;
+    while (c1 < nchunk && chunk_deg[c1] == chunk_deg[c0]) ++c1;
+    for (int j = 0; j < 8; ++j)
+      for (int64_t c = c0 + (c1 - c0) * j / 8; c < c0 + (c1 - c0) * (j + 1) / 8; ++c) xl[j].push_back((int32_t)c);
+    c0 = c1;
+  }
+  size_t lmax = 1;
+  for (auto& l : xl) lmax = std::max(lmax, l.size());
+  const int64_t lpad = (int64_t)((lmax + 3) / 4 * 4);
+  std::vector<int32_t> corder(8 * lpad, -1);
+  for (int j = 0; j < 8; ++j) std::copy(xl[j].begin(), xl[j].end(), corder.begin() + j * lpad);
   std::vector<int32_t> scol(nslot, 0), sell_pos(nblk);
   for (int64_t c = 0; c < nchunk; ++c)
     for (int i = 0; i < kSellC; ++i) {
@@ -587,6 +601,8 @@ static int build_mesh(const int32_t* T, int64_t nv, int64_t nt, sk_mesh* m) {
   SK_TRY(m->chunk_off.upload(chunk_off));
   SK_TRY(m->scol.upload(scol));
   SK_TRY(m->sell_pos.upload(sell_pos));
+  SK_TRY(m->corder.upload(corder));
+  m->lpad = lpad;
   m->nchunk = nchunk;
   m->nslot = nslot;
   SK_TRY(m->D.alloc(12 * (size_t)nt));

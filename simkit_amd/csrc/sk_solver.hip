@@ -29,15 +29,6 @@ struct PcgState {
 constexpr int kRowLanes = 8;      // lanes cooperating on one (block) row in SpMV
 constexpr int kSpmvBlocks = 1024; // SpMV grid cap; blocks walk contiguous row chunks
 
-// Logical block index with consecutive logical blocks on the same XCD: the
-// hardware deals workgroups round-robin over the 8 XCDs, so XCD j receives
-// hardware blocks j, j+8, ... and is given the contiguous logical range
-// [j*G/8, (j+1)*G/8). Row chunks processed by one XCD then share its L2.
-__device__ __forceinline__ int xcd_block(int b, int G) {
-  if (G % 8) return b;
-  return (b % 8) * (G / 8) + b / 8;
-}
-
 // Shared prologue of the SpMV launches: reduce the update partials, decide
 // convergence, compute beta. Returns false if the iteration must not run.
 struct PcgCtl {
@@ -116,6 +107,8 @@ struct SellView {
   const int32_t* chunk_deg;
   const int32_t* chunk_off;
   const int32_t* scol;
+  const int32_t* corder;  // XCD-balanced chunk order, 8 lists of lpad (-1: none)
+  int64_t lpad;
 };
 
 template <typename T>
@@ -288,25 +281,33 @@ __global__ __launch_bounds__(kBlock) void bpcg_restart_kernel(BlockPcgArgs<T> a)
   if (blockIdx.x == 0 && threadIdx.x == 0) a.st->done = 0;
 }
 
-// SpMV workgroups walk a contiguous range of `per` chunk groups (kWaves
-// chunks each, one per wave); consecutive logical workgroups share an XCD.
-__device__ __forceinline__ int64_t chunks_per_block(int64_t nchunk) {
+// The SpMV kernels take their chunks from the XCD-balanced order: workgroup
+// b runs on XCD b % 8 and is that XCD's (b / 8)-th workgroup (the grid is a
+// multiple of 8). One chunk of kSellC rows per wave, kSellLanes lanes per row
+// (lane = j * kSellC + i: row i of the chunk, slots s = j, j + kSellLanes, ...).
+//
+// Range form: each workgroup walks `per` consecutive groups of kWaves chunks
+// of its XCD's list.
+__device__ __forceinline__ int64_t chunks_per_block(const SellView& sv) {
   constexpr int kWaves = kBlock / kWave;
-  const int64_t groups = (nchunk + kWaves - 1) / kWaves;
-  return (groups + gridDim.x - 1) / gridDim.x;
+  const int64_t per_xcd = gridDim.x / 8;
+  return (sv.lpad + per_xcd * kWaves - 1) / (per_xcd * kWaves);
 }
 
-__device__ __forceinline__ int64_t block_chunk(int wid, int64_t q, int64_t per) {
+// Position q of this wave in its XCD's list; -2 past the end, -1 padding.
+__device__ __forceinline__ int64_t block_chunk(const SellView& sv, int wid, int64_t q, int64_t per) {
   constexpr int kWaves = kBlock / kWave;
-  return ((int64_t)xcd_block(blockIdx.x, gridDim.x) * per + q) * kWaves + wid;
+  const int64_t pos = ((int64_t)(blockIdx.x / 8) * per + q) * kWaves + wid;
+  if (pos >= sv.lpad) return -2;
+  return sv.corder[(blockIdx.x % 8) * sv.lpad + pos];
 }
 
-// SELL chunk of this wave: one chunk of kSellC rows per wave, kSellLanes
-// lanes per row (lane = j * kSellC + i: row i of the chunk, slots s = j,
-// j + kSellLanes, ...). Consecutive chunks go to the same XCD.
-__device__ __forceinline__ int64_t sell_chunk(int wid) {
+// One-chunk-per-wave form (the grid covers 8 * lpad waves); -1: no chunk.
+__device__ __forceinline__ int64_t sell_chunk(const SellView& sv, int wid) {
   constexpr int kWaves = kBlock / kWave;
-  return (int64_t)xcd_block(blockIdx.x, gridDim.x) * kWaves + wid;
+  const int64_t pos = (int64_t)(blockIdx.x / 8) * kWaves + wid;
+  if (pos >= sv.lpad) return -1;
+  return sv.corder[(blockIdx.x % 8) * sv.lpad + pos];
 }
 
 // y[p] = sum over the row's slots of A_s[p][:] . xv(col_s), where xv(w) gives
@@ -375,15 +376,15 @@ __global__ __launch_bounds__(kBlock) void bpcg_spmv_kernel(BlockPcgArgs<T> a) {
   const int lane = threadIdx.x % kWave;
   double dot = 0.0;
   if (kRange) {
-    const int64_t per = chunks_per_block(a.sv.nchunk);
+    const int64_t per = chunks_per_block(a.sv);
     for (int64_t q = 0; q < per; ++q) {
-      const int64_t ch = block_chunk(threadIdx.x / kWave, q, per);
-      if (ch >= a.sv.nchunk) break;
-      spmv_chunk(a, beta, ch, lane, dot);
+      const int64_t ch = block_chunk(a.sv, threadIdx.x / kWave, q, per);
+      if (ch == -2) break;
+      if (ch >= 0) spmv_chunk(a, beta, ch, lane, dot);
     }
   } else {
-    const int64_t ch = sell_chunk(threadIdx.x / kWave);
-    if (ch < a.sv.nchunk) spmv_chunk(a, beta, ch, lane, dot);
+    const int64_t ch = sell_chunk(a.sv, threadIdx.x / kWave);
+    if (ch >= 0) spmv_chunk(a, beta, ch, lane, dot);
   }
   double vals[1] = {dot};
   store_block_partials<1>(vals, a.spart);
@@ -515,9 +516,9 @@ struct ResidualArgs {
 
 __global__ __launch_bounds__(kBlock) void ir_residual_kernel(ResidualArgs a) {
   const int lane = threadIdx.x % kWave;
-  const int64_t ch = sell_chunk(threadIdx.x / kWave);
+  const int64_t ch = sell_chunk(a.sv, threadIdx.x / kWave);
   double rr = 0.0;
-  if (ch < a.sv.nchunk) {
+  if (ch >= 0) {
     const int i = lane % kSellC, j = lane / kSellC;
     double y[3] = {0.0, 0.0, 0.0};
     if (a.x) {
@@ -808,14 +809,10 @@ int spmv_grid(int64_t rows) {
 
 // One SELL chunk per wave; the grid is rounded up to a multiple of 8 so the
 // XCD-aware block order is a permutation (surplus waves do no work).
-int sell_grid(int64_t nchunk) {
-  int64_t g = (nchunk + (kBlock / kWave) - 1) / (kBlock / kWave);
-  g = (g + 7) / 8 * 8;
-  return (int)(g < 8 ? 8 : g);
-}
+int sell_grid(const sk_mesh* m) { return (int)(8 * (m->lpad / (kBlock / kWave))); }
 
 SellView sell_view(sk_mesh* m) {
-  return SellView{m->nchunk, m->sell_row.p, m->chunk_deg.p, m->chunk_off.p, m->scol.p};
+  return SellView{m->nchunk, m->sell_row.p, m->chunk_deg.p, m->chunk_off.p, m->scol.p, m->corder.p, m->lpad};
 }
 
 // Buffers of one precision's PCG.
@@ -846,7 +843,7 @@ BlockPcgArgs<T> block_pcg_args(sk_sim* S, const PcgBufs<T>& B, const T* b, doubl
                  : S->update_blocks > 0 ? std::min(grid_for(m->nv), S->update_blocks)
                                         : grid_for(m->nv);
   const int cap = S->spmv_blocks > 0 ? S->spmv_blocks : (sizeof(T) == 8 ? 0 : kSpmvBlocks32);
-  const int gs = cap > 0 ? std::min(sell_grid(m->nchunk), (cap + 7) / 8 * 8) : sell_grid(m->nchunk);
+  const int gs = cap > 0 ? std::min(sell_grid(m), (cap + 7) / 8 * 8) : sell_grid(m);
   double* part = m->red.partials.p;
   CoarseView<T> cv{};
   if (S->two_level()) cv = S->coarse.view<T>();
@@ -855,8 +852,8 @@ BlockPcgArgs<T> block_pcg_args(sk_sim* S, const PcgBufs<T>& B, const T* b, doubl
 }
 
 template <typename T>
-void launch_spmv(const BlockPcgArgs<T>& a, int64_t nchunk, hipStream_t st) {
-  if (a.gs >= sell_grid(nchunk)) bpcg_spmv_kernel<T, false><<<a.gs, kBlock, 0, st>>>(a);
+void launch_spmv(const BlockPcgArgs<T>& a, const sk_mesh* m, hipStream_t st) {
+  if (a.gs >= sell_grid(m)) bpcg_spmv_kernel<T, false><<<a.gs, kBlock, 0, st>>>(a);
   else bpcg_spmv_kernel<T, true><<<a.gs, kBlock, 0, st>>>(a);
 }
 
@@ -897,7 +894,7 @@ int run_block_pcg(sk_sim* S, const PcgBufs<T>& B, const T* b, double rtol, int64
       a.restart = (k0 > 0 && launched == k0) ? 1 : 0;
       a.pin = (launched & 1) ? B.p1 : B.p0;
       a.pout = (launched & 1) ? B.p0 : B.p1;
-      launch_spmv<T>(a, S->mesh->nchunk, st);
+      launch_spmv<T>(a, S->mesh, st);
       if (tl) {
         bpcg_update_kernel<T, true><<<a.gu, kBlock, 0, st>>>(a);
         bpcg_coarse_kernel<T><<<a.gu, kBlock, 0, st>>>(a);
@@ -934,7 +931,7 @@ int run_mixed_solve(sk_sim* S, hipStream_t st, int64_t* iters, int* refines, int
   const int64_t n3 = 3 * m->nv;
   ResidualArgs ra{m->nv, sell_view(m), S->vals32.p, S->fixed.p, S->g.p, nullptr, S->r.p, S->r32.p,
                   m->red.partials.p + 4 * (size_t)Reducer::kMaxBlocks};
-  const int gr = sell_grid(m->nchunk);
+  const int gr = sell_grid(m);
   double* rr_dev = m->red.result.p + 1;
   ir_residual_kernel<<<gr, kBlock, 0, st>>>(ra);
   finalize_sum_kernel<1><<<1, kBlock, 0, st>>>(ra.partials, gr, rr_dev);
@@ -1317,7 +1314,7 @@ int sk_sim_spmv_reps(sk_sim* S, int reps, sk_stream_t s) {
     bpcg_init_kernel<float, false><<<a.gu, kBlock, 0, st>>>(a);
     a.pin = B.p0;
     a.pout = B.p1;
-    for (int k = 0; k < reps; ++k) launch_spmv<float>(a, S->mesh->nchunk, st);
+    for (int k = 0; k < reps; ++k) launch_spmv<float>(a, S->mesh, st);
   } else {
     const PcgBufs<double> B = bufs64(S);
     BlockPcgArgs<double> a = block_pcg_args<double>(S, B, S->g.p, 0.0);
@@ -1325,7 +1322,7 @@ int sk_sim_spmv_reps(sk_sim* S, int reps, sk_stream_t s) {
     bpcg_init_kernel<double, false><<<a.gu, kBlock, 0, st>>>(a);
     a.pin = B.p0;
     a.pout = B.p1;
-    for (int k = 0; k < reps; ++k) launch_spmv<double>(a, S->mesh->nchunk, st);
+    for (int k = 0; k < reps; ++k) launch_spmv<double>(a, S->mesh, st);
   }
   SK_HIP_CHECK(hipGetLastError());
   return SK_OK;
