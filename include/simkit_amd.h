@@ -435,6 +435,12 @@ int sk_mlp_adam(sk_mlp* net, float lr, float beta1, float beta2, float eps, sk_s
  * preconditioner). per_agg <= 0 returns to block-Jacobi. Fails with
  * SK_ERR_INVALID when the coarse space would exceed 4096 unknowns. */
 int sk_sim_set_coarse(sk_sim* sim, const double* X, int per_agg);
+/* Host-only, no counterpart in the reference: the vertices of X (host,
+ * nv x 3) grouped by the aggregates sk_sim_set_coarse would form with
+ * per_agg, ascending within each aggregate (order[nv]). A mesh renumbered in
+ * this order gives every aggregate a contiguous vertex range, which the
+ * two-level kernels then read with unit stride. */
+int sk_coarse_order(const double* X, int64_t nv, int per_agg, int32_t* order);
 /* Aggregates, coarse unknowns, whether E^-1 has been formed, and the time it took (ms). */
 int sk_sim_coarse_info(const sk_sim* sim, int* n_agg, int* nc, int* built, double* build_ms);
 

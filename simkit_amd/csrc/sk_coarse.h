@@ -75,6 +75,11 @@ inline CoarseView<float> CoarseSpace::view<float>() const {
 int rigid_basis(const double* dv, const uint8_t* fixed, const int32_t* verts, int nverts, int64_t vbase,
                 double tm[36]);
 
+// Spatial bins of the rest positions (host): vagg[v], and the vertices of
+// aggregate I at agg_vert[agg_ptr[I] .. agg_ptr[I + 1]), ascending.
+int coarse_bins(const double* X, int64_t nv, int per_agg, std::vector<int32_t>& vagg,
+                std::vector<int32_t>& agg_ptr, std::vector<int32_t>& agg_vert);
+
 // Aggregates from the rest positions X (host, nv x 3): each axis of the
 // bounding box is cut evenly into bins of close to the edge s of a cube that
 // holds per_agg vertices at the mesh's mean density; non-empty bins become

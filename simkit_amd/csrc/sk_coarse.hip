@@ -206,10 +206,10 @@ int rigid_basis(const double* dv, const uint8_t* fixed, const int32_t* verts, in
   return k;
 }
 
-int coarse_setup(CoarseSpace& C, sk_mesh* m, const double* X, const uint8_t* fixed_host, int per_agg) {
+int coarse_bins(const double* X, int64_t nv, int per_agg, std::vector<int32_t>& vagg,
+                std::vector<int32_t>& agg_ptr, std::vector<int32_t>& agg_vert) {
   SK_REQUIRE(X, "coarse set-up: null positions");
   SK_REQUIRE(per_agg >= 1, "coarse set-up: vertices per aggregate must be >= 1");
-  const int64_t nv = m->nv;
   SK_REQUIRE(nv > 0, "coarse set-up: empty mesh");
   double lo[3], hi[3];
   for (int a = 0; a < 3; ++a) lo[a] = hi[a] = X[a];
@@ -243,16 +243,24 @@ int coarse_setup(CoarseSpace& C, sk_mesh* m, const double* X, const uint8_t* fix
   std::sort(bins.begin(), bins.end());
   bins.erase(std::unique(bins.begin(), bins.end()), bins.end());
   const int n_agg = (int)bins.size();
-  std::vector<int32_t> vagg(nv), agg_ptr(n_agg + 1, 0), agg_vert(nv);
+  vagg.assign(nv, 0);
+  agg_ptr.assign(n_agg + 1, 0);
+  agg_vert.assign(nv, 0);
   for (int64_t v = 0; v < nv; ++v) {
     vagg[v] = (int32_t)(std::lower_bound(bins.begin(), bins.end(), key[v]) - bins.begin());
     ++agg_ptr[vagg[v] + 1];
   }
   for (int I = 0; I < n_agg; ++I) agg_ptr[I + 1] += agg_ptr[I];
-  {
-    std::vector<int32_t> fill(agg_ptr.begin(), agg_ptr.end() - 1);
-    for (int64_t v = 0; v < nv; ++v) agg_vert[fill[vagg[v]]++] = (int32_t)v;
-  }
+  std::vector<int32_t> fill(agg_ptr.begin(), agg_ptr.end() - 1);
+  for (int64_t v = 0; v < nv; ++v) agg_vert[fill[vagg[v]]++] = (int32_t)v;
+  return SK_OK;
+}
+
+int coarse_setup(CoarseSpace& C, sk_mesh* m, const double* X, const uint8_t* fixed_host, int per_agg) {
+  const int64_t nv = m->nv;
+  std::vector<int32_t> vagg, agg_ptr, agg_vert;
+  SK_TRY(coarse_bins(X, nv, per_agg, vagg, agg_ptr, agg_vert));
+  const int n_agg = (int)agg_ptr.size() - 1;
   // centroids, offsets, per-aggregate Gram matrices of the masked rigid motions
   std::vector<double> cen(3 * (size_t)n_agg, 0.0), dv(3 * nv), tm(36 * (size_t)n_agg, 0.0);
   for (int64_t v = 0; v < nv; ++v)
@@ -357,3 +365,11 @@ int coarse_build(CoarseSpace& C, sk_mesh* m, const void* vals, bool fp32, const 
 }
 
 }  // namespace sk
+
+extern "C" int sk_coarse_order(const double* X, int64_t nv, int per_agg, int32_t* order) {
+  SK_REQUIRE(order, "sk_coarse_order: null output");
+  std::vector<int32_t> vagg, agg_ptr, agg_vert;
+  SK_TRY(sk::coarse_bins(X, nv, per_agg, vagg, agg_ptr, agg_vert));
+  std::copy(agg_vert.begin(), agg_vert.end(), order);
+  return SK_OK;
+}

@@ -31,7 +31,7 @@ class BackwardEulerSim:
                  gravity=0.0, psd=True, newton_rtol=1e-6, newton_max_iter=100, pcg_rtol=1e-8,
                  pcg_max_iter=100000, ls_alpha=0.01, ls_beta=0.5, ls_max_iter=100, ls_threshold=1e-12,
                  timing=False, precision="fp64", inner_rtol=1e-2, max_refine=50, spmv_blocks=0,
-                 update_blocks=0, preconditioner="bjacobi", coarse_vertices=216, pcg_chunk=0):
+                 update_blocks=0, preconditioner="bjacobi", coarse_vertices=216, pcg_chunk=0, renumber=True):
         """precision: "fp64" (every stage FP64) or "mixed" (FP32 element Hessian, PSD projection,
         assembled values and inner PCG to inner_rtol, wrapped in FP64 iterative refinement to
         pcg_rtol; gradient, line-search energies and the polar SVD of ARAP elements with
@@ -39,7 +39,10 @@ class BackwardEulerSim:
 
         preconditioner: "bjacobi" (3x3 block-Jacobi) or "two_level" (block-Jacobi plus an additive
         coarse correction over piecewise-rigid aggregates of about coarse_vertices rest vertices;
-        the coarse operator is formed from the first system solved). Both solve to pcg_rtol."""
+        the coarse operator is formed from the first system solved). Both solve to pcg_rtol.
+        renumber (two-level only): the simulator numbers the vertices internally aggregate by
+        aggregate, so that each aggregate is a contiguous range for the device kernels; states
+        passed in and returned keep the caller's vertex order."""
         if preconditioner not in ("bjacobi", "two_level"):
             raise ValueError("preconditioner must be 'bjacobi' or 'two_level'")
         X = np.asarray(X, dtype=np.float64)
@@ -47,25 +50,42 @@ class BackwardEulerSim:
         self.X = X
         self.T = T
         self.nv, self.nt = X.shape[0], T.shape[0]
+        # internal vertex order: order[t] = caller's vertex at internal position t
+        self._order = self._pos = None
+        if preconditioner == "two_level" and renumber:
+            order = np.empty(self.nv, np.int32)
+            Xc = np.ascontiguousarray(X, dtype=np.float64)
+            _lib.call("sk_coarse_order", Xc.ctypes.data_as(ctypes.c_void_p), int(self.nv), int(coarse_vertices),
+                      order.ctypes.data_as(ctypes.c_void_p))
+            if not np.array_equal(order, np.arange(self.nv)):
+                pos = np.empty_like(order)
+                pos[order] = np.arange(self.nv, dtype=np.int32)
+                self._order, self._pos = order, pos
+                self._order_dev = torch.as_tensor(order.astype(np.int64), device="cuda")
+                self._pos_dev = torch.as_tensor(pos.astype(np.int64), device="cuda")
+        Xi = X if self._order is None else np.ascontiguousarray(X[self._order])
+        Ti = T if self._order is None else self._pos[T]
         self.h = float(h)
         if material not in _lib.MATERIALS:
             raise ValueError("Unknown material type: " + str(material))
         self.material = _lib.MATERIALS[material]
         if self.material != _lib.SK_ARAP and lam is None:
             raise ValueError(f"material {material!r} needs lam")
-        self.mesh, vol = mesh_for_rest(X, T)
+        self.mesh, vol = mesh_for_rest(Xi, Ti)
         self.vol = vol
         m_v = self.mesh.lumped_mass(vol, rho)
         self.mass = m_v.repeat_interleave(3).contiguous()
         fixed = np.zeros(3 * self.nv, np.uint8)
         fv = np.asarray(fixed_vertices, dtype=np.int64).reshape(-1)
+        if self._order is not None:
+            fv = self._pos[fv]
         for c in range(3):
             fixed[3 * fv + c] = 1
         self.fixed_np = fixed.astype(bool)
         self.fixed = dev(fixed, torch.uint8)
         self.f_ext = None
         if gravity:
-            self.f_ext = dev(gravity_force(X, T, gravity, rho).reshape(-1))
+            self.f_ext = dev(gravity_force(Xi, Ti, gravity, rho).reshape(-1))
         self.mu = dev(np.broadcast_to(np.asarray(mu, np.float64).reshape(-1), (self.nt,)).copy())
         self.lam = None if lam is None else dev(np.broadcast_to(np.asarray(lam, np.float64).reshape(-1),
                                                                 (self.nt,)).copy())
@@ -98,8 +118,28 @@ class BackwardEulerSim:
         self.X_dev = dev(X.reshape(-1))
         self.preconditioner = preconditioner
         if preconditioner == "two_level":
-            Xc = np.ascontiguousarray(X, dtype=np.float64)
+            Xc = np.ascontiguousarray(Xi, dtype=np.float64)
             _lib.call("sk_sim_set_coarse", self._h, Xc.ctypes.data_as(ctypes.c_void_p), int(coarse_vertices))
+
+    def _to_internal(self, x):
+        if x is None or self._order is None:
+            return x
+        return x.view(-1, 3).index_select(0, self._order_dev).reshape(-1)
+
+    def _out(self, like, out):
+        """Device buffer for the library's result (internal order)."""
+        if self._order is None and out is not None:
+            return out
+        return torch.empty_like(like)
+
+    def _to_caller(self, xi, out):
+        if self._order is None:
+            return xi
+        x = xi.view(-1, 3).index_select(0, self._pos_dev).reshape(-1)
+        if out is not None:
+            out.copy_(x)
+            return out
+        return x
 
     def coarse_info(self):
         """Aggregates, coarse unknowns, whether the coarse inverse has been formed and its build time (ms)."""
@@ -112,27 +152,29 @@ class BackwardEulerSim:
     def step(self, x_curr, x_prev, x_bc=None, out=None):
         """One BE step on device tensors (3nv,). Returns (x_next, info dict)."""
         info = _lib.SkStepInfo()
-        x_next = torch.empty_like(x_curr) if out is None else out
-        _lib.call("sk_sim_step", self._h, ptr(x_curr), ptr(x_prev), ptr(x_bc), ptr(x_next),
-                  ctypes.byref(info), stream())
-        return x_next, info.as_dict()
+        xc, xp, xb = self._to_internal(x_curr), self._to_internal(x_prev), self._to_internal(x_bc)
+        x_next = self._out(xc, out)
+        _lib.call("sk_sim_step", self._h, ptr(xc), ptr(xp), ptr(xb), ptr(x_next), ctypes.byref(info), stream())
+        return self._to_caller(x_next, out), info.as_dict()
 
     def step_guess(self, x_curr, x_prev, x_guess, x_bc=None, out=None):
         """BE step that starts Newton from x_guess when its incremental potential is lower than at
         x~ (info["guess_used"]), with the stopping test still relative to ||g(x~)||."""
         info = _lib.SkStepInfo()
-        x_next = torch.empty_like(x_curr) if out is None else out
-        _lib.call("sk_sim_step_guess", self._h, ptr(x_curr), ptr(x_prev), ptr(x_bc), ptr(x_guess), ptr(x_next),
+        xc, xp, xb = self._to_internal(x_curr), self._to_internal(x_prev), self._to_internal(x_bc)
+        xg = self._to_internal(x_guess)
+        x_next = self._out(xc, out)
+        _lib.call("sk_sim_step_guess", self._h, ptr(xc), ptr(xp), ptr(xb), ptr(xg), ptr(x_next),
                   ctypes.byref(info), stream())
-        return x_next, info.as_dict()
+        return self._to_caller(x_next, out), info.as_dict()
 
     def step_bdf2(self, x_curr, x_prev, x_prev2, x_prev3, x_bc=None, out=None):
         """One BDF2 step (c = 9/4, BDF2 target) on device tensors."""
         info = _lib.SkStepInfo()
-        x_next = torch.empty_like(x_curr) if out is None else out
-        _lib.call("sk_sim_step_bdf2", self._h, ptr(x_curr), ptr(x_prev), ptr(x_prev2), ptr(x_prev3), ptr(x_bc),
-                  ptr(x_next), ctypes.byref(info), stream())
-        return x_next, info.as_dict()
+        h = [self._to_internal(v) for v in (x_curr, x_prev, x_prev2, x_prev3, x_bc)]
+        x_next = self._out(h[0], out)
+        _lib.call("sk_sim_step_bdf2", self._h, *[ptr(v) for v in h], ptr(x_next), ctypes.byref(info), stream())
+        return self._to_caller(x_next, out), info.as_dict()
 
     def rollout(self, steps, bc=None, x0=None, keep=None, integrator="be"):
         """`steps` steps from rest (x_prev = x_curr = X, or x0). bc(n, t) -> device x_bc for step n
