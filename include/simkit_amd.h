@@ -435,6 +435,12 @@ int sk_mlp_adam(sk_mlp* net, float lr, float beta1, float beta2, float eps, sk_s
  * preconditioner). per_agg <= 0 returns to block-Jacobi. Fails with
  * SK_ERR_INVALID when the coarse space would exceed 4096 unknowns. */
 int sk_sim_set_coarse(sk_sim* sim, const double* X, int per_agg);
+/* Host-only, no counterpart in the reference: the vertices of the tet mesh
+ * T (host, nt x 4) in the row order of the simulator's sliced-ELL matrix
+ * (stable, by descending number of neighbour vertices including itself;
+ * order[nv]). A mesh renumbered in this order has the identity row order,
+ * so the SpMV's per-row stores are contiguous. */
+int sk_sell_order(const int32_t* T, int64_t nv, int64_t nt, int32_t* order);
 /* Host-only, no counterpart in the reference: the vertices of X (host,
  * nv x 3) grouped by the aggregates sk_sim_set_coarse would form with
  * per_agg, ascending within each aggregate (order[nv]). A mesh renumbered in

@@ -207,6 +207,7 @@ SIGNATURES = {
     "sk_sim_spmv_reps": [_P, _INT, _P],
     "sk_sim_set_coarse": [_P, _P, _INT],
     "sk_coarse_order": [_P, _I64, _INT, _P],
+    "sk_sell_order": [_P, _I64, _I64, _P],
     "sk_sim_coarse_info": [_P, _P, _P, _P, _P],
 }
 _RESTYPES = {"sk_last_error": ctypes.c_char_p, "sk_status_string": ctypes.c_char_p, "sk_mlp_num_params": ctypes.c_int64}

@@ -645,6 +645,32 @@ int sk_mesh_create(const int32_t* T_host, const double* D, int64_t nv, int64_t n
   return SK_OK;
 }
 
+int sk_sell_order(const int32_t* T, int64_t nv, int64_t nt, int32_t* order) {
+  SK_REQUIRE(T && order, "sk_sell_order: null argument");
+  SK_REQUIRE(nv > 0 && nt > 0, "sk_sell_order: empty mesh");
+  for (int64_t k = 0; k < 4 * nt; ++k) SK_REQUIRE(T[k] >= 0 && T[k] < nv, "sk_sell_order: tet index out of range");
+  std::vector<int32_t> vptr(nv + 1, 0), vlist(4 * nt);
+  for (int64_t k = 0; k < 4 * nt; ++k) vptr[T[k] + 1]++;
+  std::partial_sum(vptr.begin(), vptr.end(), vptr.begin());
+  {
+    std::vector<int32_t> fill(vptr.begin(), vptr.end() - 1);
+    for (int64_t e = 0; e < nt; ++e)
+      for (int i = 0; i < 4; ++i) vlist[fill[T[4 * e + i]]++] = (int32_t)e;
+  }
+  std::vector<int32_t> deg(nv), nb;
+  for (int64_t v = 0; v < nv; ++v) {
+    nb.clear();
+    nb.push_back((int32_t)v);
+    for (int32_t k = vptr[v]; k < vptr[v + 1]; ++k)
+      for (int j = 0; j < 4; ++j) nb.push_back(T[4 * (int64_t)vlist[k] + j]);
+    std::sort(nb.begin(), nb.end());
+    deg[v] = (int32_t)(std::unique(nb.begin(), nb.end()) - nb.begin());
+  }
+  std::iota(order, order + nv, 0);
+  std::stable_sort(order, order + nv, [&](int32_t a, int32_t b) { return deg[a] > deg[b]; });
+  return SK_OK;
+}
+
 int sk_mesh_destroy(sk_mesh* m) {
   if (!m) return SK_OK;
   if (m->pinned) (void)hipHostFree(m->pinned);

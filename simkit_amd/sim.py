@@ -40,9 +40,10 @@ class BackwardEulerSim:
         preconditioner: "bjacobi" (3x3 block-Jacobi) or "two_level" (block-Jacobi plus an additive
         coarse correction over piecewise-rigid aggregates of about coarse_vertices rest vertices;
         the coarse operator is formed from the first system solved). Both solve to pcg_rtol.
-        renumber (two-level only): the simulator numbers the vertices internally aggregate by
-        aggregate, so that each aggregate is a contiguous range for the device kernels; states
-        passed in and returned keep the caller's vertex order."""
+        renumber: the simulator numbers the vertices internally in the order its kernels read
+        them (two-level: aggregate by aggregate, so that each aggregate is a contiguous range;
+        block-Jacobi: the sliced-ELL row order, so that the SpMV's row stores are contiguous);
+        states passed in and returned keep the caller's vertex order."""
         if preconditioner not in ("bjacobi", "two_level"):
             raise ValueError("preconditioner must be 'bjacobi' or 'two_level'")
         X = np.asarray(X, dtype=np.float64)
@@ -52,11 +53,16 @@ class BackwardEulerSim:
         self.nv, self.nt = X.shape[0], T.shape[0]
         # internal vertex order: order[t] = caller's vertex at internal position t
         self._order = self._pos = None
-        if preconditioner == "two_level" and renumber:
+        if renumber:
             order = np.empty(self.nv, np.int32)
-            Xc = np.ascontiguousarray(X, dtype=np.float64)
-            _lib.call("sk_coarse_order", Xc.ctypes.data_as(ctypes.c_void_p), int(self.nv), int(coarse_vertices),
-                      order.ctypes.data_as(ctypes.c_void_p))
+            if preconditioner == "two_level":
+                Xc = np.ascontiguousarray(X, dtype=np.float64)
+                _lib.call("sk_coarse_order", Xc.ctypes.data_as(ctypes.c_void_p), int(self.nv),
+                          int(coarse_vertices), order.ctypes.data_as(ctypes.c_void_p))
+            else:
+                Tc = np.ascontiguousarray(T, dtype=np.int32)
+                _lib.call("sk_sell_order", Tc.ctypes.data_as(ctypes.c_void_p), int(self.nv), int(self.nt),
+                          order.ctypes.data_as(ctypes.c_void_p))
             if not np.array_equal(order, np.arange(self.nv)):
                 pos = np.empty_like(order)
                 pos[order] = np.arange(self.nv, dtype=np.int32)
