@@ -390,32 +390,78 @@ __global__ __launch_bounds__(kBlock) void bpcg_spmv_kernel(BlockPcgArgs<T> a) {
   store_block_partials<1>(vals, a.spart);
 }
 
+// The per-vertex data of the update (x, p, r, A p, B^-1), loaded before the
+// prologue's reductions so that the loads overlap them.
+template <typename T>
+struct UpdateIn {
+  T x[3], p[3], r[3], ap[3], B[9];
+};
+
+template <typename T>
+__device__ __forceinline__ void load_update(const BlockPcgArgs<T>& a, int64_t v, UpdateIn<T>& u) {
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const int64_t k = 3 * v + q;
+    u.x[q] = a.x[k];
+    u.p[q] = a.pout[k];
+    u.r[q] = a.r[k];
+    u.ap[q] = a.Ap[k];
+  }
+#pragma unroll
+  for (int m = 0; m < 9; ++m) u.B[m] = a.binv[9 * v + m];
+}
+
+template <typename T, bool kAgg>
+__device__ __forceinline__ void apply_update(const BlockPcgArgs<T>& a, int64_t v, const UpdateIn<T>& u, T alpha,
+                                             double& rz, double& rr, double (&raw)[6]) {
+  T r[3], z[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    a.x[3 * v + q] = u.x[q] + alpha * u.p[q];
+    r[q] = u.r[q] - alpha * u.ap[q];
+  }
+#pragma unroll
+  for (int q = 0; q < 3; ++q) z[q] = u.B[3 * q] * r[0] + u.B[3 * q + 1] * r[1] + u.B[3 * q + 2] * r[2];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const int64_t k = 3 * v + q;
+    a.r[k] = r[q];
+    a.z[k] = z[q];
+    rz += (double)r[q] * (double)z[q];
+    rr += (double)r[q] * (double)r[q];
+  }
+  if (kAgg) restrict_add(a.cv, v, r, raw);
+}
+
 template <typename T, bool kAgg>
 __global__ __launch_bounds__(kBlock) void bpcg_update_kernel(BlockPcgArgs<T> a) {
   const AggPre pre = agg_prefetch<kAgg>(a);
+  // this thread's first vertex and the stride to its next ones (for_vertices' order)
+  int64_t v = -1, t = 0, t_end = 0, stride = 0;
+  if (kAgg) {
+    t = a.cv.agg_ptr[blockIdx.x] + threadIdx.x;
+    t_end = a.cv.agg_ptr[blockIdx.x + 1];
+    stride = blockDim.x;
+    if (t < t_end) v = a.cv.agg_vert[t];
+  } else {
+    t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    t_end = a.nv;
+    stride = (int64_t)gridDim.x * blockDim.x;
+    if (t < t_end) v = t;
+  }
+  UpdateIn<T> u;
+  if (v >= 0) load_update(a, v, u);
   double alpha_d;
   if (!update_prologue(a.st, a.spart, a.gs, a.k, alpha_d)) return;
   const T alpha = (T)alpha_d;
   double rz = 0.0, rr = 0.0, raw[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  for_vertices<kAgg>(a, [&](int64_t v) {
-    T r[3], z[3];
-#pragma unroll
-    for (int p = 0; p < 3; ++p) {
-      const int64_t k = 3 * v + p;
-      a.x[k] += alpha * a.pout[k];
-      r[p] = a.r[k] - alpha * a.Ap[k];
-    }
-    apply_binv(a.binv, v, r, z);
-#pragma unroll
-    for (int p = 0; p < 3; ++p) {
-      const int64_t k = 3 * v + p;
-      a.r[k] = r[p];
-      a.z[k] = z[p];
-      rz += (double)r[p] * (double)z[p];
-      rr += (double)r[p] * (double)r[p];
-    }
-    if (kAgg) restrict_add(a.cv, v, r, raw);
-  });
+  if (v >= 0) apply_update<T, kAgg>(a, v, u, alpha, rz, rr, raw);
+  for (t += stride; t < t_end; t += stride) {
+    const int64_t w = kAgg ? (int64_t)a.cv.agg_vert[t] : t;
+    UpdateIn<T> uw;
+    load_update(a, w, uw);
+    apply_update<T, kAgg>(a, w, uw, alpha, rz, rr, raw);
+  }
   finish_vertex_pass<kAgg>(a, rz, rr, raw, pre);
 }
 
@@ -442,28 +488,52 @@ __global__ __launch_bounds__(kBlock) void bpcg_coarse_kernel(BlockPcgArgs<T> a) 
   }
   const double cq = ((int)threadIdx.x < k) ? cv.c[c0 + threadIdx.x] : 0.0;
   const double part0 = threadIdx.x == 0 ? a.upart[I] : 0.0;
-  // y_q = sum_j E^-1[c0 + q][j] c_j, four columns per load, two loads in flight per row
+  // The first vertex this thread prolongs into, loaded now: z is final (the
+  // previous launch wrote it) and only this thread writes it below.
+  const int32_t tp = t0 + (int32_t)threadIdx.x;
+  const bool has_p = tp < t1;
+  int64_t pv = 0;
+  double pd[3] = {0.0, 0.0, 0.0};
+  T pz[3] = {T(0), T(0), T(0)};
+  bool pfree[3] = {false, false, false};
+  if (has_p) {
+    pv = cv.agg_vert[tp];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      pd[q] = (double)cv.dva[3 * (int64_t)tp + q];
+      pz[q] = a.z[3 * pv + q];
+      pfree[q] = !(a.fixed && a.fixed[3 * pv + q]);
+    }
+  }
+  // y_q = sum_j E^-1[c0 + q][j] c_j: four columns per float4 load, up to kU loads per row in
+  // flight per thread
+  constexpr int kU = 4;
   const int ld4 = cv.ld >> 2;
   const float4* __restrict__ E4 = reinterpret_cast<const float4*>(cv.einv);
   const double2* __restrict__ c2 = reinterpret_cast<const double2*>(cv.c);
   double y[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  for (int j4 = threadIdx.x; j4 < ld4; j4 += 2 * kBlock) {
-    const int j4b = j4 + kBlock;
-    const bool hb = j4b < ld4;
-    const double2 ca = c2[2 * j4], cb = c2[2 * j4 + 1];
-    double2 da = {0.0, 0.0}, db = {0.0, 0.0};
-    if (hb) {
-      da = c2[2 * j4b];
-      db = c2[2 * j4b + 1];
+  for (int base = threadIdx.x; base < ld4; base += kU * kBlock) {
+    double2 ca[kU], cb[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int j4 = base + u * kBlock;
+      ca[u] = j4 < ld4 ? c2[2 * j4] : make_double2(0.0, 0.0);
+      cb[u] = j4 < ld4 ? c2[2 * j4 + 1] : make_double2(0.0, 0.0);
     }
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
       if (q < k) {
         const float4* row = E4 + (int64_t)(c0 + q) * ld4;
-        const float4 e = row[j4];
-        const float4 f = hb ? row[j4b] : make_float4(0.f, 0.f, 0.f, 0.f);
-        y[q] += (double)e.x * ca.x + (double)e.y * ca.y + (double)e.z * cb.x + (double)e.w * cb.y;
-        y[q] += (double)f.x * da.x + (double)f.y * da.y + (double)f.z * db.x + (double)f.w * db.y;
+        float4 e[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+          const int j4 = base + u * kBlock;
+          e[u] = j4 < ld4 ? row[j4] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+          y[q] += (double)e[u].x * ca[u].x + (double)e[u].y * ca[u].y + (double)e[u].z * cb[u].x +
+                  (double)e[u].w * cb[u].y;
       }
     }
   }
@@ -484,15 +554,22 @@ __global__ __launch_bounds__(kBlock) void bpcg_coarse_kernel(BlockPcgArgs<T> a) 
   __syncthreads();
   if (threadIdx.x == 0) a.upart[I] = part0 + (s_cy[0] + s_cy[1] + s_cy[2] + s_cy[3] + s_cy[4] + s_cy[5]);
   const double u0 = s_uw[0], u1 = s_uw[1], u2 = s_uw[2], w0 = s_uw[3], w1 = s_uw[4], w2 = s_uw[5];
-  for (int32_t t = t0 + threadIdx.x; t < t1; t += kBlock) {
+  if (has_p) {
+    const double corr[3] = {u0 + w1 * pd[2] - w2 * pd[1], u1 + w2 * pd[0] - w0 * pd[2],
+                            u2 + w0 * pd[1] - w1 * pd[0]};
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+      if (pfree[q]) a.z[3 * pv + q] = (T)((double)pz[q] + corr[q]);
+  }
+  for (int32_t t = tp + kBlock; t < t1; t += kBlock) {  // aggregates larger than the workgroup
     const int64_t v = cv.agg_vert[t];
     const double d0 = (double)cv.dva[3 * (int64_t)t], d1 = (double)cv.dva[3 * (int64_t)t + 1],
                  d2 = (double)cv.dva[3 * (int64_t)t + 2];
     const double corr[3] = {u0 + w1 * d2 - w2 * d1, u1 + w2 * d0 - w0 * d2, u2 + w0 * d1 - w1 * d0};
 #pragma unroll
-    for (int p = 0; p < 3; ++p) {
-      const int64_t kk = 3 * v + p;
-      if (!(a.fixed && a.fixed[kk])) a.z[kk] = (T)((double)a.z[kk] + corr[p]);
+    for (int q = 0; q < 3; ++q) {
+      const int64_t kk = 3 * v + q;
+      if (!(a.fixed && a.fixed[kk])) a.z[kk] = (T)((double)a.z[kk] + corr[q]);
     }
   }
 }
