@@ -545,7 +545,7 @@ int sk_modal_basis(sk_mesh* m, int material, const double* x, const double* mu, 
   double worst = 0.0;
   for (;;) {
     // Relative residual; eigenvalues below 1e-6 of the spectrum bound (rigid
-    // motions of an unconstrained mesh) are judged against that floor.
+    // motions of an unconstrained mesh) are measured against that floor.
     worst = 0.0;
     for (int j = 0; j < k; ++j) worst = std::max(worst, res[j] / std::max(std::fabs(th[j]), 1e-6 * bmax));
     if (worst <= tol || sweeps >= max_sweeps) break;

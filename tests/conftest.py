@@ -42,7 +42,7 @@ def rel_err(a, b):
 
 
 def per_item_rel_err(a, b, axis_items=0):
-    """max over items of (max |a_i - b_i| / max |b_i|): each element's block judged on its own scale."""
+    """max over items of (max |a_i - b_i| / max |b_i|): each element's block measured on its own scale."""
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
     a2 = a.reshape(a.shape[0], -1)

@@ -77,7 +77,7 @@ def test_x_tier_matches_reference_golden(ref_assembly, mat):
         ref = sps.csr_matrix((d[f"{mat}_{name}_data"], d[f"{mat}_{name}_indices"], d[f"{mat}_{name}_indptr"]),
                              shape=Q.shape)
         assert rel_err(Q.toarray(), ref.toarray()) < TOL
-        # At rest the gradient is zero up to roundoff on both sides; judge it on the
+        # At rest the gradient is zero up to roundoff on both sides; measure it on the
         # force scale of the same mesh under the deformed state.
         gscale = max(np.abs(d[f"{mat}_{name}_g"]).max(), np.abs(d[f"{mat}_def_g"]).max())
         assert np.abs(g - d[f"{mat}_{name}_g"]).max() <= TOL * gscale

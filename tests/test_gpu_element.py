@@ -1,7 +1,7 @@
 """GPU parity: element tier, PSD projection, SVD / rotation gradient and rest
 measures against the oracle and the reference's golden vectors.
 
-Tolerance: 1e-10 relative, judged per element on that element's own scale
+Tolerance: 1e-10 relative, measured per element on that element's own scale
 (max |gpu - ref| / max |ref| over each element's block), in FP64.
 """
 
