@@ -27,6 +27,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+HESS_KERNEL = {"fp64": "elem_hess_kernel<SNH, 1, double>", "mixed": "elem_hess_kernel<SNH, 1, float>"}
 # dominant kernel of each path (the FP32 SpMV walks chunk ranges with 1024 workgroups)
 SPMV_KERNEL = {"fp64": "bpcg_spmv_kernel<double, false>", "mixed": "bpcg_spmv_kernel<float, true>"}
 METRIC = "BE steps/s, 48^3-cell neo-Hookean (353k DOF), learned+FP32 vs plain FP64"
@@ -64,6 +66,21 @@ def pmc_traffic(path_key):
     try:
         block = text.split("```json", 1)[1].split("```", 1)[0]
         return int(json.loads(block)[path_key]["traffic_bytes"])
+    except (IndexError, KeyError, ValueError):
+        return None
+
+
+def pmc_flops(key):
+    """FP64 FLOPs per element-pass launch ("elem_hess_fp64") from the committed hardware-counter
+    summary (profiles/pmc_elem_hess_cube48.md: the FP64 VALU instruction counters of the kernel), or
+    None when absent."""
+    path = os.path.join(ROOT, "profiles", "pmc_elem_hess_cube48.md")
+    if not os.path.exists(path):
+        return None
+    text = open(path).read()
+    try:
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        return float(json.loads(block)[key]["flops_per_launch"])
     except (IndexError, KeyError, ValueError):
         return None
 
@@ -150,6 +167,12 @@ def run_path(args, X, T, bottom, top, mu, lam, name):
     e1.record(stream)
     torch.cuda.synchronize()
     spmv_ms = e0.elapsed_time(e1) / args.spmv_reps
+    # the element pass (F, 9x9 Hessian, PSD projection, staged blocks), same stream
+    e0.record(stream)
+    _lib.call("sk_sim_hess_reps", sim._h, args.hess_reps, stream.cuda_stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    hess_ms = e0.elapsed_time(e1) / args.hess_reps
     traffic = spmv_bytes(sim.nv, sim.mesh.nblocks, 8 if precision == "fp64" else 4)
     res = {
         "steps_per_s": args.steps / elapsed, "ms_per_step": 1e3 * elapsed / args.steps,
@@ -157,7 +180,8 @@ def run_path(args, X, T, bottom, top, mu, lam, name):
         "ls_trials": sum(i["ls_trials"] for i in infos), "refine_steps": sum(i["refine_steps"] for i in infos),
         "phase_ms_total": {k: round(sum(i[k] for i in infos), 2)
                            for k in ("ms_gradient", "ms_hessian", "ms_pcg", "ms_line_search")},
-        "spmv_ms": spmv_ms, "spmv_bytes": traffic, "infos": infos, "x_final": x_final.cpu().numpy(),
+        "spmv_ms": spmv_ms, "spmv_bytes": traffic, "hess_ms": hess_ms, "infos": infos,
+        "x_final": x_final.cpu().numpy(),
         "coarse": sim.coarse_info() if pc == "two_level" else None,
     }
     log(f"bench[{name}]: {elapsed:.2f}s, {res['steps_per_s']:.3f} steps/s, newton {res['newton_iters']}, "
@@ -323,6 +347,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cells", type=int, default=48)
     ap.add_argument("--spmv-reps", type=int, default=200)
+    ap.add_argument("--hess-reps", type=int, default=20)
     ap.add_argument("--cpu-pcg-sample", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--inner-rtol", type=float, default=1e-2)
@@ -386,6 +411,15 @@ def main():
                      if args.cells == 48 else None,
                      "bytes_per_launch": int(head["spmv_bytes"]), "avg_launch_ms": round(head["spmv_ms"], 5)},
     }
+    if PATHS[head_name][0] == "fp64" and args.cells == 48:
+        # the FP64-ALU-bound element pass against the FP64 vector peak: FLOPs per launch from the
+        # committed FP64 VALU instruction counters, launch time measured live above
+        fl = pmc_flops("elem_hess_fp64")
+        ach = fl / (head["hess_ms"] * 1e-3) / 1e12 if fl else None
+        out["element_roofline"] = {
+            "bound": "fp64", "kernel": HESS_KERNEL["fp64"], "achieved": round(ach, 2) if ach else None,
+            "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK_TFLOPS, 4) if ach else None,
+            "flops_per_launch": fl, "elements": int(T.shape[0]), "avg_launch_ms": round(head["hess_ms"], 4)}
     if "mixed" in results and head_name == "fp64":
         mx = results["mixed"]
         dev = float(np.abs(mx["x_final"] - head["x_final"]).max() / np.abs(head["x_final"]).max())

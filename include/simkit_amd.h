@@ -457,6 +457,12 @@ int sk_sim_coarse_info(const sk_sim* sim, int* n_agg, int* nc, int* built, doubl
  * (positions, gradient, assembled values) is not modified. */
 int sk_sim_spmv_reps(sk_sim* sim, int reps, sk_stream_t stream);
 
+/* Measurement hook, no counterpart in the reference: enqueues `reps` launches
+ * of the simulator's element pass (F, 9x9 Hessian, vol weight, PSD
+ * projection, staged blocks) at the current positions on `stream`. Only the
+ * element staging buffer is overwritten; the next step re-assembles. */
+int sk_sim_hess_reps(sk_sim* sim, int reps, sk_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

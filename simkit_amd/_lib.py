@@ -205,6 +205,7 @@ SIGNATURES = {
     "sk_sim_step_guess": [_P, _P, _P, _P, _P, _P, ctypes.POINTER(SkStepInfo), _P],
     "sk_sim_step_bdf2": [_P, _P, _P, _P, _P, _P, _P, ctypes.POINTER(SkStepInfo), _P],
     "sk_sim_spmv_reps": [_P, _INT, _P],
+    "sk_sim_hess_reps": [_P, _INT, _P],
     "sk_sim_set_coarse": [_P, _P, _INT],
     "sk_coarse_order": [_P, _I64, _INT, _P],
     "sk_sell_order": [_P, _I64, _I64, _P],

@@ -1405,4 +1405,14 @@ int sk_sim_spmv_reps(sk_sim* S, int reps, sk_stream_t s) {
   return SK_OK;
 }
 
+int sk_sim_hess_reps(sk_sim* S, int reps, sk_stream_t s) {
+  SK_REQUIRE(S && reps >= 0, "sk_sim_hess_reps: invalid argument");
+  hipStream_t st = as_stream(s);
+  const double* lam = S->lam.p;
+  for (int k = 0; k < reps; ++k)
+    SK_TRY(launch_elem_hess(S->mesh, S->prm.material, S->x.p, nullptr, S->mu.p, lam, S->vol.p, S->prm.psd,
+                            S->mixed(), st));
+  return SK_OK;
+}
+
 }  // extern "C"
