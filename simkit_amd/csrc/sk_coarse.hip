@@ -366,10 +366,11 @@ int coarse_build(CoarseSpace& C, sk_mesh* m, const void* vals, bool fp32, const 
 
 }  // namespace sk
 
-extern "C" int sk_coarse_order(const double* X, int64_t nv, int per_agg, int32_t* order) {
+extern "C" int sk_coarse_order(const double* X, int64_t nv, int per_agg, int32_t* order, int* n_agg) {
   SK_REQUIRE(order, "sk_coarse_order: null output");
   std::vector<int32_t> vagg, agg_ptr, agg_vert;
   SK_TRY(sk::coarse_bins(X, nv, per_agg, vagg, agg_ptr, agg_vert));
   std::copy(agg_vert.begin(), agg_vert.end(), order);
+  if (n_agg) *n_agg = (int)agg_ptr.size() - 1;
   return SK_OK;
 }

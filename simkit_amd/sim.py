@@ -26,6 +26,9 @@ from ._mesh import mesh_for_rest
 from .gravity_force import gravity_force
 
 
+_COARSE_MAX = 4096  # coarse unknowns the library accepts (sk_sim_set_coarse)
+
+
 class BackwardEulerSim:
     def __init__(self, X, T, mu, lam=None, rho=1000.0, h=1.0 / 60.0, material="snh", fixed_vertices=(),
                  gravity=0.0, psd=True, newton_rtol=1e-6, newton_max_iter=100, pcg_rtol=1e-8,
@@ -38,8 +41,9 @@ class BackwardEulerSim:
         det F < 1e-3 stay FP64).
 
         preconditioner: "bjacobi" (3x3 block-Jacobi) or "two_level" (block-Jacobi plus an additive
-        coarse correction over piecewise-rigid aggregates of about coarse_vertices rest vertices;
-        the coarse operator is formed from the first system solved). Both solve to pcg_rtol.
+        coarse correction over piecewise-rigid aggregates of about coarse_vertices rest vertices,
+        raised as needed to keep the coarse space within the library's 4,096 unknowns; the coarse
+        operator is formed from the first system solved). Both solve to pcg_rtol.
         renumber: the simulator numbers the vertices internally in the order its kernels read
         them (two-level: aggregate by aggregate, so that each aggregate is a contiguous range;
         block-Jacobi: the sliced-ELL row order, so that the SpMV's row stores are contiguous);
@@ -51,14 +55,28 @@ class BackwardEulerSim:
         self.X = X
         self.T = T
         self.nv, self.nt = X.shape[0], T.shape[0]
+        # aggregates of at least coarse_vertices vertices, grown until their 6 motions each fit the
+        # coarse-space cap
+        self.coarse_vertices = int(coarse_vertices)
+        agg_order = None
+        if preconditioner == "two_level":
+            Xc = np.ascontiguousarray(X, dtype=np.float64)
+            agg_order = np.empty(self.nv, np.int32)
+            n_agg = ctypes.c_int()
+            self.coarse_vertices = max(self.coarse_vertices, int(np.ceil(6 * self.nv / _COARSE_MAX)))
+            while True:
+                _lib.call("sk_coarse_order", Xc.ctypes.data_as(ctypes.c_void_p), int(self.nv),
+                          self.coarse_vertices, agg_order.ctypes.data_as(ctypes.c_void_p), ctypes.byref(n_agg))
+                if 6 * n_agg.value <= _COARSE_MAX:
+                    break
+                self.coarse_vertices = int(np.ceil(self.coarse_vertices * 1.25))
+        coarse_vertices = self.coarse_vertices
         # internal vertex order: order[t] = caller's vertex at internal position t
         self._order = self._pos = None
         if renumber:
             order = np.empty(self.nv, np.int32)
             if preconditioner == "two_level":
-                Xc = np.ascontiguousarray(X, dtype=np.float64)
-                _lib.call("sk_coarse_order", Xc.ctypes.data_as(ctypes.c_void_p), int(self.nv),
-                          int(coarse_vertices), order.ctypes.data_as(ctypes.c_void_p))
+                order[:] = agg_order
             else:
                 Tc = np.ascontiguousarray(T, dtype=np.int32)
                 _lib.call("sk_sell_order", Tc.ctypes.data_as(ctypes.c_void_p), int(self.nv), int(self.nt),
@@ -153,7 +171,7 @@ class BackwardEulerSim:
         _lib.call("sk_sim_coarse_info", self._h, ctypes.byref(n), ctypes.byref(nc), ctypes.byref(built),
                   ctypes.byref(ms))
         return {"aggregates": n.value, "coarse_unknowns": nc.value, "built": bool(built.value),
-                "build_ms": ms.value}
+                "build_ms": ms.value, "vertices_per_aggregate": self.coarse_vertices}
 
     def step(self, x_curr, x_prev, x_bc=None, out=None):
         """One BE step on device tensors (3nv,). Returns (x_next, info dict)."""

@@ -45,8 +45,10 @@ def _coarse_order(X, per_agg):
     import numpy as np
     order = np.empty(len(X), np.int32)
     Xc = np.ascontiguousarray(X, dtype=np.float64)
+    n_agg = ctypes.c_int()
     _lib.call("sk_coarse_order", Xc.ctypes.data_as(ctypes.c_void_p), len(X), per_agg,
-              order.ctypes.data_as(ctypes.c_void_p))
+              order.ctypes.data_as(ctypes.c_void_p), ctypes.byref(n_agg))
+    _coarse_order.n_agg = n_agg.value
     return order
 
 
@@ -79,7 +81,7 @@ def test_coarse_order_groups_spatial_bins():
         assert np.all(np.diff(k) >= 0)                           # grouped by bin, bins ascending
         same = np.diff(k) == 0
         assert np.all(np.diff(order)[same] > 0)                   # ascending inside a bin
-        assert len(np.unique(key)) == len(np.unique(k))
+        assert len(np.unique(key)) == len(np.unique(k)) == _coarse_order.n_agg
     # 13^3 vertices / 27 = 81.4 bins -> edge 0.231 -> round(4.33) = 4 per axis
     assert len(np.unique(bins(X12, 27))) == 64
     # 775 vertices / 50 = 15.5 bins in a 0.75 x 0.1 x 0.1 box -> edge 0.0785 -> 10 x 1 x 1

@@ -99,10 +99,18 @@ def test_two_level_beam_gravity_matches_block_jacobi():
 
 
 def test_two_level_deterministic_and_size_limit():
+    """Bitwise determinism; the C entry point rejects a coarse space above its cap, and the
+    simulator raises its aggregate size so that it never asks for one."""
+    import ctypes
     from simkit_amd import _lib
     _, _, sim = _cube_sim(10, preconditioner="two_level", coarse_vertices=27)
     a, ia = sim.rollout(3)
     b, ib = sim.rollout(3)
     assert all(np.array_equal(a[k], b[k]) for k in a) and ia == ib
+    _, _, big = _cube_sim(20, preconditioner="bjacobi")
+    X = np.ascontiguousarray(unit_cube(20)[0])
     with pytest.raises(_lib.SimkitAmdError, match="coarse unknowns"):
-        _cube_sim(20, preconditioner="two_level", coarse_vertices=1)
+        _lib.call("sk_sim_set_coarse", big._h, X.ctypes.data_as(ctypes.c_void_p), 1)
+    _, _, auto = _cube_sim(20, preconditioner="two_level", coarse_vertices=1)
+    info = auto.coarse_info()
+    assert 1 < info["coarse_unknowns"] <= 4096 and info["vertices_per_aggregate"] >= 6 * 9261 / 4096
