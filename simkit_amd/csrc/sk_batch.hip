@@ -3,10 +3,13 @@
 //
 // The problems are stored as one union mesh whose vertices and tets are
 // grouped per problem (contiguous ranges). Mesh-wide phases (element
-// gradient, element Hessian + PSD projection, block gather into BSR FP32
-// values) run once over the union. The phases with per-problem control flow
-// are persistent kernels with one workgroup per problem (looping over
-// problems when there are more problems than workgroups):
+// gradient, element Hessian + PSD projection, block gather into FP32 values)
+// run once over the union. The values are sliced per problem: chunks of 64
+// consecutive rows (one wave's rows in the per-problem sweeps), each padded to
+// its longest row and stored component-major, so that the wave's loads of one
+// block component fall in two cache lines. The phases with per-problem
+// control flow are persistent kernels with one workgroup per problem (looping
+// over problems when there are more problems than workgroups):
 //   * PCG: block-Jacobi CG on the problem's free DOFs in FP32 with FP64
 //     residual replacement (the mixed-precision scheme of the single-mesh
 //     simulator), all scalars reduced inside the workgroup; optionally with a
@@ -40,10 +43,16 @@ struct BatchArgs {
   const int32_t* vstart;   // [P+1]
   const int32_t* tstart;   // [P+1]
   const int32_t* pid;      // [nv] problem of each vertex
-  const int32_t* bptr;
-  const int32_t* bcol;
-  const float* vals;       // BSR, 9 per block
-  const float* binv;       // 9 per vertex
+  // Per-problem sliced matrix ("wave-ELL"): each problem's rows are cut into
+  // chunks of kWave consecutive rows starting at its first vertex; chunk c's
+  // slots woff[c] .. woff[c+1] - 1 hold block s of its row i at slot
+  // woff[c] + s kWave + i (rows padded with zero blocks to the chunk's longest
+  // row), component m at vals[(q - i) 9 + m kWave + i].
+  const int32_t* wchunk0;  // [P+1] first chunk of each problem
+  const int32_t* woff;     // [nchunk+1] first slot of each chunk (a multiple of kWave)
+  const int32_t* wcol;     // [nslot] column vertex of each slot (padding: the row's own vertex)
+  const float* vals;
+  const float* binv;       // chunk c's row i: component m at binv[9 kWave c + m kWave + i]
   const uint8_t* fixed;
   ElemView em;
   const double* mu;
@@ -143,20 +152,44 @@ __device__ __forceinline__ void all_max(double (&v)[NV], double* lds) {
 
 __device__ __forceinline__ bool is_fixed(const uint8_t* fixed, int64_t k) { return fixed && fixed[k]; }
 
-// y = A_row(v) . src over the row's blocks; LANES lanes share a row (lane j
-// takes slots j, j + LANES, ...) and combine with xor shuffles.
-template <int LANES, typename V>
-__device__ __forceinline__ void row_product(const BatchArgs& a, int64_t v, const V* __restrict__ src, int j,
-                                            double (&y)[3]) {
-  y[0] = y[1] = y[2] = 0.0;
-  if (v >= 0) {
-    const int32_t b1 = a.bptr[v + 1];
-    for (int32_t b = a.bptr[v] + j; b < b1; b += LANES) {
-      const int64_t w = 3 * (int64_t)a.bcol[b];
-      const float* A = a.vals + 9 * (int64_t)b;
-      const double s0 = (double)src[w], s1 = (double)src[w + 1], s2 = (double)src[w + 2];
+// y = A_row . src for row i of chunk c (valid: the thread has a row);
+// LANES lanes share a row (lane j takes slot groups j, j + LANES, ...) and
+// combine with xor shuffles. src is indexed by 3 * column - off; R: the
+// accumulation type.
+template <int LANES, typename R, typename V>
+__device__ __forceinline__ void row_product(const BatchArgs& a, bool valid, int64_t c, int i,
+                                            const V* __restrict__ src, int64_t off, int j, R (&y)[3]) {
+  constexpr int step = LANES * kWave, kU = 4;
+  y[0] = y[1] = y[2] = R(0);
+  if (valid) {
+    const int32_t g1 = a.woff[c + 1];
+    int32_t g = a.woff[c] + j * kWave;
+    // kU slot groups at a time: their loads are all issued before the first
+    // product (one memory latency per kU groups); products in slot order
+    for (; g + (kU - 1) * step < g1; g += kU * step) {
+      int64_t w[kU];
+      float A[kU][9];
 #pragma unroll
-      for (int p = 0; p < 3; ++p) y[p] += (double)A[3 * p] * s0 + (double)A[3 * p + 1] * s1 + (double)A[3 * p + 2] * s2;
+      for (int u = 0; u < kU; ++u) {
+        w[u] = 3 * (int64_t)a.wcol[g + u * step + i] - off;
+        const float* Au = a.vals + 9 * (int64_t)(g + u * step) + i;
+#pragma unroll
+        for (int m = 0; m < 9; ++m) A[u][m] = Au[m * kWave];
+      }
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const R s0 = (R)src[w[u]], s1 = (R)src[w[u] + 1], s2 = (R)src[w[u] + 2];
+#pragma unroll
+        for (int p = 0; p < 3; ++p) y[p] += (R)A[u][3 * p] * s0 + (R)A[u][3 * p + 1] * s1 + (R)A[u][3 * p + 2] * s2;
+      }
+    }
+    for (; g < g1; g += step) {
+      const int64_t w = 3 * (int64_t)a.wcol[g + i] - off;
+      const float* A = a.vals + 9 * (int64_t)g + i;
+      const R s0 = (R)src[w], s1 = (R)src[w + 1], s2 = (R)src[w + 2];
+#pragma unroll
+      for (int p = 0; p < 3; ++p)
+        y[p] += (R)A[3 * p * kWave] * s0 + (R)A[(3 * p + 1) * kWave] * s1 + (R)A[(3 * p + 2) * kWave] * s2;
     }
   }
 #pragma unroll
@@ -165,29 +198,13 @@ __device__ __forceinline__ void row_product(const BatchArgs& a, int64_t v, const
     for (int p = 0; p < 3; ++p) y[p] += __shfl_xor(y[p], off, kWave);
 }
 
-template <int LANES>
-__device__ __forceinline__ void row_product_f(const BatchArgs& a, int64_t v, const float* __restrict__ src,
-                                              int64_t off, int j, float (&y)[3]) {
-  y[0] = y[1] = y[2] = 0.f;
-  if (v >= 0) {
-    const int32_t b1 = a.bptr[v + 1];
-    for (int32_t b = a.bptr[v] + j; b < b1; b += LANES) {
-      const int64_t w = 3 * (int64_t)a.bcol[b] - off;
-      const float* A = a.vals + 9 * (int64_t)b;
-      const float s0 = src[w], s1 = src[w + 1], s2 = src[w + 2];
+// z = B^-1 r with the diagonal inverse of row i of chunk c.
+__device__ __forceinline__ void apply_binv3(const BatchArgs& a, int64_t c, int i, float r0, float r1, float r2,
+                                            float (&z)[3]) {
+  const float* B = a.binv + 9 * kWave * c + i;
 #pragma unroll
-      for (int p = 0; p < 3; ++p) y[p] += A[3 * p] * s0 + A[3 * p + 1] * s1 + A[3 * p + 2] * s2;
-    }
-  }
-#pragma unroll
-  for (int off = 1; off < LANES; off <<= 1)
-#pragma unroll
-    for (int p = 0; p < 3; ++p) y[p] += __shfl_xor(y[p], off, kWave);
-}
-
-__device__ __forceinline__ void apply_binv3(const float* B, float r0, float r1, float r2, float (&z)[3]) {
-#pragma unroll
-  for (int p = 0; p < 3; ++p) z[p] = B[3 * p] * r0 + B[3 * p + 1] * r1 + B[3 * p + 2] * r2;
+  for (int p = 0; p < 3; ++p)
+    z[p] = B[3 * p * kWave] * r0 + B[(3 * p + 1) * kWave] * r1 + B[(3 * p + 2) * kWave] * r2;
 }
 
 // The workgroup's shared flag read: one thread reads, all threads branch alike.
@@ -289,15 +306,36 @@ __device__ double coarse_apply(const BatchArgs& a, const CoarseLds& cl, int p, d
     }
   }
   __syncthreads();
+  // y = E^-1 c. E^-1 is stored symmetric, so y_i = sum_j E[j][i] c_j: the lanes
+  // of a wave take 64 consecutive i (coalesced reads along row j), wave w the
+  // j = w, w + nw, ... (kU rows' loads in flight at once); the nw partials of
+  // each i are then summed in wave order (lds holds nw x 64 doubles).
   const float* E = a.einv + a.einv_off[p];
   const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
-  for (int i = wid; i < nc; i += NT / kWave) {
+  constexpr int nw = NT / kWave, kU = 8;
+  for (int i0 = 0; i0 < nc; i0 += kWave) {
+    const int i = i0 + lane;
     double s = 0.0;
-    for (int j = lane; j < nc; j += kWave) s += (double)E[(int64_t)i * nc + j] * cl.c[j];
-    s = wave_sum(s);
-    if (lane == 0) cl.y[i] = s;
+    if (i < nc) {
+      int jr = wid;
+      for (; jr + (kU - 1) * nw < nc; jr += kU * nw) {
+        float e[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) e[u] = E[(int64_t)(jr + u * nw) * nc + i];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) s += (double)e[u] * cl.c[jr + u * nw];
+      }
+      for (; jr < nc; jr += nw) s += (double)E[(int64_t)jr * nc + i] * cl.c[jr];
+    }
+    lds[wid * kWave + lane] = s;
+    __syncthreads();
+    if (wid == 0 && i < nc) {
+      double t = 0.0;
+      for (int w = 0; w < nw; ++w) t += lds[w * kWave + lane];
+      cl.y[i] = t;
+    }
+    __syncthreads();
   }
-  __syncthreads();
   for (int t = threadIdx.x; t < nagg * 6; t += NT) {
     const int l = t / 6, q = t % 6, g = g0 + l;
     const int c0 = a.ccol[g] - cb, k = a.ccol[g + 1] - a.ccol[g];
@@ -331,7 +369,8 @@ __device__ __forceinline__ void coarse_term(const BatchArgs& a, const CoarseLds&
 // ---------------------------------------------------------------------------
 template <int NT, int LANES, bool KLDS, bool KC>
 __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
-  __shared__ double lds[4 * (NT / kWave)];
+  // reductions (4 doubles per wave); with KC also the coarse product's partials (NT doubles)
+  __shared__ double lds[KC ? NT : 4 * (NT / kWave)];
   // KLDS: p, r (and A p without KC) of the problem; KC: the coarse scratch after them (declared
   // as double so that the base is 8-byte aligned for the scratch's FP64 part)
   extern __shared__ double dyn_d[];
@@ -350,7 +389,7 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
       for (int t = threadIdx.x; t < nraw; t += NT) cl.raw[t] = 0.f;
     }
     if (!block_read(a.active + p)) continue;
-    const int64_t v0 = a.vstart[p], v1 = a.vstart[p + 1];
+    const int64_t v0 = a.vstart[p], v1 = a.vstart[p + 1], wc0 = a.wchunk0[p];
     const int64_t nrow = v1 - v0;
     const int64_t sweeps = (nrow + RS - 1) / RS;
     // p and r: in LDS (indexed from the problem's first DOF) or in the global
@@ -363,7 +402,8 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
     // init: r64 = b = -g (free), dx = 0, r32 = b, x32 = 0, z = M^-1 r, p = z
     double acc[2] = {0.0, 0.0};
     for (int64_t s = 0; s < sweeps; ++s) {
-      const int64_t v = v0 + s * RS + slot;
+      const int64_t rel = s * RS + slot, v = v0 + rel, c = wc0 + rel / kWave;
+      const int ci = (int)(rel % kWave);
       float tr[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
       if (v < v1 && j == 0) {
         float r[3], z[3];
@@ -378,7 +418,7 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
           a.x32[k] = 0.f;
           acc[1] += b * b;
         }
-        apply_binv3(a.binv + 9 * v, r[0], r[1], r[2], z);
+        apply_binv3(a, c, ci, r[0], r[1], r[2], z);
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
           if (!KC) P[3 * v + q - off] = z[q];
@@ -394,11 +434,12 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
     if (KC) {
       rz += coarse_apply<NT, LANES>(a, cl, p, lds);
       for (int64_t s = 0; s < sweeps; ++s) {
-        const int64_t v = v0 + s * RS + slot;
+        const int64_t rel = s * RS + slot, v = v0 + rel, c = wc0 + rel / kWave;
+        const int ci = (int)(rel % kWave);
         if (v < v1 && j == 0) {
           float z[3], zc[3];
           const float* rv = Rv + 3 * v - off;
-          apply_binv3(a.binv + 9 * v, rv[0], rv[1], rv[2], z);
+          apply_binv3(a, c, ci, rv[0], rv[1], rv[2], z);
           coarse_term(a, cl, v, v - v0, zc);
 #pragma unroll
           for (int q = 0; q < 3; ++q) P[3 * v + q - off] = z[q] + zc[q];
@@ -422,9 +463,10 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
       }
       double d1[1] = {0.0};
       for (int64_t s = 0; s < sweeps; ++s) {
-        const int64_t v = v0 + s * RS + slot;
+        const int64_t rel = s * RS + slot, v = v0 + rel, c = wc0 + rel / kWave;
+        const int ci = (int)(rel % kWave);
         float y[3];
-        row_product_f<LANES>(a, v < v1 ? v : -1, P, off, j, y);
+        row_product<LANES, float>(a, v < v1, c, ci, P, off, j, y);
         if (v < v1 && j == 0) {
 #pragma unroll
           for (int q = 0; q < 3; ++q) {
@@ -445,7 +487,8 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
       // x += alpha p, r -= alpha A p, r.r; with KC also r.(B^-1 r) and the restriction of r
       double d2[2] = {0.0, 0.0};
       for (int64_t s = 0; s < sweeps; ++s) {
-        const int64_t v = v0 + s * RS + slot;
+        const int64_t rel = s * RS + slot, v = v0 + rel, c = wc0 + rel / kWave;
+        const int ci = (int)(rel % kWave);
         float tr[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         if (v < v1 && j == 0) {
           float r[3];
@@ -459,7 +502,7 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
           }
           if (KC) {
             float z[3];
-            apply_binv3(a.binv + 9 * v, r[0], r[1], r[2], z);
+            apply_binv3(a, c, ci, r[0], r[1], r[2], z);
 #pragma unroll
             for (int q = 0; q < 3; ++q) d2[1] += (double)r[q] * (double)z[q];
             restrict_terms(a, v, r, tr);
@@ -495,10 +538,11 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
         __syncthreads();
         double d3[2] = {0.0, 0.0};
         for (int64_t s = 0; s < sweeps; ++s) {
-          const int64_t v = v0 + s * RS + slot;
+          const int64_t rel = s * RS + slot, v = v0 + rel, c = wc0 + rel / kWave;
+          const int ci = (int)(rel % kWave);
           double y[3];
           float tr[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-          row_product<LANES, double>(a, v < v1 ? v : -1, a.dx, j, y);
+          row_product<LANES, double>(a, v < v1, c, ci, a.dx, 0, j, y);
           if (v < v1 && j == 0) {
             float rf[3];
 #pragma unroll
@@ -512,7 +556,7 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
             }
             if (KC) {
               float z[3];
-              apply_binv3(a.binv + 9 * v, rf[0], rf[1], rf[2], z);
+              apply_binv3(a, c, ci, rf[0], rf[1], rf[2], z);
 #pragma unroll
               for (int q = 0; q < 3; ++q) d3[1] += (double)rf[q] * (double)z[q];
               restrict_terms(a, v, rf, tr);
@@ -538,11 +582,12 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
         const float beta = (float)(rz_new / rz);
         rz = rz_new;
         for (int64_t s = 0; s < sweeps; ++s) {
-          const int64_t v = v0 + s * RS + slot;
+          const int64_t rel = s * RS + slot, v = v0 + rel, c = wc0 + rel / kWave;
+          const int ci = (int)(rel % kWave);
           if (v < v1 && j == 0) {
             float z[3], zc[3];
             const float* rv = Rv + 3 * v - off;
-            apply_binv3(a.binv + 9 * v, rv[0], rv[1], rv[2], z);
+            apply_binv3(a, c, ci, rv[0], rv[1], rv[2], z);
             coarse_term(a, cl, v, v - v0, zc);
 #pragma unroll
             for (int q = 0; q < 3; ++q) P[3 * v + q - off] = z[q] + zc[q] + beta * P[3 * v + q - off];
@@ -554,11 +599,12 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
       // z = B^-1 r, r.z, then p = z + beta p
       double d4[1] = {0.0};
       for (int64_t s = 0; s < sweeps; ++s) {
-        const int64_t v = v0 + s * RS + slot;
+        const int64_t rel = s * RS + slot, v = v0 + rel, c = wc0 + rel / kWave;
+        const int ci = (int)(rel % kWave);
         if (v < v1 && j == 0) {
           float z[3];
           const float* rv = Rv + 3 * v - off;
-          apply_binv3(a.binv + 9 * v, rv[0], rv[1], rv[2], z);
+          apply_binv3(a, c, ci, rv[0], rv[1], rv[2], z);
 #pragma unroll
           for (int q = 0; q < 3; ++q) d4[0] += (double)rv[q] * (double)z[q];
         }
@@ -567,11 +613,12 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
       const float beta = (float)(d4[0] / rz);
       rz = d4[0];
       for (int64_t s = 0; s < sweeps; ++s) {
-        const int64_t v = v0 + s * RS + slot;
+        const int64_t rel = s * RS + slot, v = v0 + rel, c = wc0 + rel / kWave;
+        const int ci = (int)(rel % kWave);
         if (v < v1 && j == 0) {
           float z[3];
           const float* rv = Rv + 3 * v - off;
-          apply_binv3(a.binv + 9 * v, rv[0], rv[1], rv[2], z);
+          apply_binv3(a, c, ci, rv[0], rv[1], rv[2], z);
 #pragma unroll
           for (int q = 0; q < 3; ++q) P[3 * v + q - off] = z[q] + beta * P[3 * v + q - off];
         }
@@ -757,6 +804,7 @@ struct sk_batch {
   DevBuf<uint8_t> fixed;
   DevBuf<double> x, xt, g, dx, r64;
   DevBuf<float> vals32, binv32, r32, p32, Ap32, x32;
+  DevBuf<int32_t> wchunk0, woff, wcol, wpos, wrow;  // the sliced per-problem matrix (BatchArgs)
   DevBuf<int> active, status, newton_iters, ls_trials, refines, stalled, n_active;
   DevBuf<long long> pcg_iters;
   DevBuf<double> g0, gnorm, tstep;
@@ -786,8 +834,9 @@ BatchArgs sk_batch::args() const {
   a.vstart = vstart.p;
   a.tstart = tstart.p;
   a.pid = pid.p;
-  a.bptr = mesh->bptr.p;
-  a.bcol = mesh->bcol.p;
+  a.wchunk0 = wchunk0.p;
+  a.woff = woff.p;
+  a.wcol = wcol.p;
   a.vals = vals32.p;
   a.binv = binv32.p;
   a.fixed = fixed.p;
@@ -855,8 +904,9 @@ size_t pcg_shm_bytes(const sk_batch* B, int lds) {
   return b;
 }
 size_t pcg_lds_bytes(const sk_batch* B) { return pcg_shm_bytes(B, 1); }
-// 60 KiB keeps the total, with the kernel's static reduction scratch, under 64 KiB.
-bool pcg_lds_fits(const sk_batch* B) { return B->max_rows > 0 && pcg_lds_bytes(B) <= 60 * 1024; }
+// 54 KiB keeps the total, with the kernel's static scratch (up to 8 KiB at 1024
+// threads), under 64 KiB.
+bool pcg_lds_fits(const sk_batch* B) { return B->max_rows > 0 && pcg_lds_bytes(B) <= 54 * 1024; }
 
 int launch_pcg(const sk_batch* B, const BatchArgs& a, int block, int lanes, int lds) {
   const int g = B->grid;
@@ -909,7 +959,8 @@ struct GalerkinBatchArgs {
   const int32_t* vstart;
   const int32_t* bptr;
   const int32_t* bcol;
-  const float* vals;       // BSR FP32
+  const int32_t* wpos;     // slot of each block in the sliced FP32 values
+  const float* vals;
   const uint8_t* fixed;
   const float* cdv;
   const float* ctm;
@@ -958,10 +1009,12 @@ __global__ __launch_bounds__(kBlock) void batch_galerkin_kernel(GalerkinBatchArg
         const int64_t w = a.bcol[b];
         if (g0 + (int)((w - v0) / a.agg_rows) != J) continue;
         double A[9];
+        const int64_t q = a.wpos[b], i = q % kWave;
+        const float* src = a.vals + 9 * (q - i) + i;
 #pragma unroll
         for (int m = 0; m < 9; ++m) {
           const bool fr = !(a.fixed && (a.fixed[3 * v + m / 3] || a.fixed[3 * w + m % 3]));
-          A[m] = fr ? (double)a.vals[9 * (int64_t)b + m] : 0.0;
+          A[m] = fr ? (double)src[m * kWave] : 0.0;
         }
         double Rw[3][6], AR[3][6];
 #pragma unroll
@@ -1042,9 +1095,9 @@ int hessian_phase(sk_batch* B, bool only_active);
 // at the batch's current positions (synchronises the batch stream).
 int build_coarse(sk_batch* B) {
   SK_TRY(hessian_phase(B, false));
-  GalerkinBatchArgs g{B->vstart.p, B->mesh->bptr.p, B->mesh->bcol.p, B->vals32.p, B->fixed.p, B->cdv.p, B->ctm.p,
-                      B->cagg0.p, B->ccol.p, B->cagg_prob.p, B->cadj_ptr.p, B->cadj.p, B->cnq.p, B->einv_off.p,
-                      B->cE.p, B->agg_rows};
+  GalerkinBatchArgs g{B->vstart.p, B->mesh->bptr.p, B->mesh->bcol.p, B->wpos.p, B->vals32.p, B->fixed.p,
+                      B->cdv.p, B->ctm.p, B->cagg0.p, B->ccol.p, B->cagg_prob.p, B->cadj_ptr.p, B->cadj.p,
+                      B->cnq.p, B->einv_off.p, B->cE.p, B->agg_rows};
   SK_HIP_CHECK(hipMemsetAsync(B->cE.p, 0, B->cE.n * sizeof(double), B->st));
   SK_HIP_CHECK(hipMemsetAsync(B->cbad.p, 0, B->P * sizeof(int), B->st));
   batch_galerkin_kernel<<<B->n_agg, kBlock, 0, B->st>>>(g);
@@ -1069,7 +1122,7 @@ int gradient_phase(sk_batch* B) {
                             1.0 / (B->prm.h * B->prm.h), B->st);
 }
 
-// Element Hessian + BSR FP32 gather. only_active: skip the elements and
+// Element Hessian + FP32 gather into the sliced layout. only_active: skip the elements and
 // blocks of problems whose active flag (read on the device) is 0 (their
 // values are not read until they are assembled again); else every problem.
 int hessian_phase(sk_batch* B, bool only_active) {
@@ -1078,8 +1131,8 @@ int hessian_phase(sk_batch* B, bool only_active) {
   const int32_t* pid = only_active ? B->pid.p : nullptr;
   SK_TRY(launch_elem_hess(m, B->prm.material, B->x.p, nullptr, B->mu.p, B->lam.p, B->vol.p, B->prm.psd, true,
                           B->st, act, pid));
-  return launch_gather_hess(m, B->vals32.p, B->mass.p, 1.0 / (B->prm.h * B->prm.h), B->fixed.p, B->binv32.p,
-                            kLayoutBSR, true, B->st, act, pid);
+  return launch_gather_hess_slots(m, B->vals32.p, B->mass.p, 1.0 / (B->prm.h * B->prm.h), B->fixed.p,
+                                  B->binv32.p, B->wpos.p, B->wrow.p, kWave, true, B->st, act, pid);
 }
 
 // One Newton iteration of every active problem (no host synchronization).
@@ -1180,6 +1233,9 @@ int sk_batch_create(sk_mesh* mesh, int nprob, const int64_t* vstart, const int64
         SK_REQUIRE(v >= vstart[q] && v < vstart[q + 1], "sk_batch_create: a tet references another problem");
       }
   }
+  // The per-problem sliced matrix: chunks of kWave rows from each problem's
+  // first vertex, each padded with zero blocks to its longest row.
+  std::vector<int32_t> wchunk0(nprob + 1), woff(1, 0), wcol, wpos(mesh->nblk), wrow(mesh->nv);
   {
     // The per-problem SpMV reads only the problem's own rows: every block of
     // the mesh's pattern must couple two vertices of one problem.
@@ -1190,6 +1246,32 @@ int sk_batch_create(sk_mesh* mesh, int nprob, const int64_t* vstart, const int64
       for (int32_t b = bp[v]; b < bp[v + 1]; ++b)
         SK_REQUIRE(bc[b] >= 0 && bc[b] < mesh->nv && pid[bc[b]] == pid[v],
                    "sk_batch_create: the mesh couples vertices of different problems");
+    int64_t nslot = 0;
+    for (int q = 0; q < nprob; ++q) {
+      wchunk0[q] = (int32_t)(woff.size() - 1);
+      for (int64_t r0 = vstart[q]; r0 < vstart[q + 1]; r0 += kWave) {
+        const int64_t r1 = std::min<int64_t>(vstart[q + 1], r0 + kWave), c = (int64_t)woff.size() - 1;
+        int32_t width = 0;
+        for (int64_t v = r0; v < r1; ++v) width = std::max(width, bp[v + 1] - bp[v]);
+        const int64_t base = nslot;
+        nslot += (int64_t)width * kWave;
+        SK_REQUIRE(nslot <= INT32_MAX, "sk_batch_create: matrix too large for 32-bit slot indices");
+        woff.push_back((int32_t)nslot);
+        wcol.resize(nslot, (int32_t)r0);  // slots of rows past the problem's end are never read
+        for (int64_t v = r0; v < r1; ++v) {
+          const int i = (int)(v - r0);
+          wrow[v] = (int32_t)(c * kWave + i);
+          for (int32_t k = 0; k < width; ++k) {
+            const int64_t sl = base + (int64_t)k * kWave + i;
+            const int32_t b = bp[v] + k;
+            wcol[sl] = b < bp[v + 1] ? bc[b] : (int32_t)v;
+            if (b < bp[v + 1]) wpos[b] = (int32_t)sl;
+          }
+        }
+      }
+    }
+    wchunk0[nprob] = (int32_t)(woff.size() - 1);
+    if (wcol.empty()) wcol.push_back(0);
   }
   sk_batch* B = new sk_batch();
   B->mesh = mesh;
@@ -1218,8 +1300,15 @@ int sk_batch_create(sk_mesh* mesh, int nprob, const int64_t* vstart, const int64
     }
     for (DevBuf<double>* b : {&B->x, &B->xt, &B->g, &B->dx, &B->r64}) SK_TRY(b->alloc(n3));
     for (DevBuf<float>* b : {&B->r32, &B->p32, &B->Ap32, &B->x32}) SK_TRY(b->alloc(n3));
-    SK_TRY(B->vals32.alloc(9 * (size_t)mesh->nblk));
-    SK_TRY(B->binv32.alloc(9 * (size_t)mesh->nv));
+    const size_t nchunk = woff.size() - 1, nslot = (size_t)woff.back();
+    SK_TRY(B->vals32.alloc(9 * std::max<size_t>(nslot, 1)));
+    SK_HIP_CHECK(hipMemset(B->vals32.p, 0, B->vals32.n * sizeof(float)));  // padding blocks stay zero
+    SK_TRY(B->binv32.alloc(9 * (size_t)kWave * std::max<size_t>(nchunk, 1)));
+    SK_TRY(B->wchunk0.upload(wchunk0));
+    SK_TRY(B->woff.upload(woff));
+    SK_TRY(B->wcol.upload(wcol));
+    SK_TRY(B->wpos.upload(wpos));
+    SK_TRY(B->wrow.upload(wrow));
     for (DevBuf<int>* b : {&B->active, &B->status, &B->newton_iters, &B->ls_trials, &B->refines, &B->stalled}) {
       SK_TRY(b->alloc(nprob));
       SK_HIP_CHECK(hipMemset(b->p, 0, nprob * sizeof(int)));

@@ -148,6 +148,13 @@ enum ValueLayout : int { kLayoutCSR = 0, kLayoutSELL = 1, kLayoutBSR = 2 };
 int launch_gather_hess(sk_mesh* m, void* values, const double* mass, double inv_h2, const uint8_t* fixed,
                        void* binv, int layout, bool fp32, hipStream_t st, const int* active = nullptr,
                        const int32_t* pid = nullptr);
+// The same gather into a caller's sliced layout of chunk width C (a power of
+// two): block b goes to slot q = pos[b], component m of it at
+// values[(q - i) * 9 + m * C + i] with i = q % C; the diagonal inverse of
+// vertex v to the same addresses with q = row_pos[v].
+int launch_gather_hess_slots(sk_mesh* m, void* values, const double* mass, double inv_h2, const uint8_t* fixed,
+                             void* binv, const int32_t* pos, const int32_t* row_pos, int C, bool fp32,
+                             hipStream_t st, const int* active = nullptr, const int32_t* pid = nullptr);
 // Copies n doubles from device to the mesh's pinned buffer and waits.
 int read_scalars(sk_mesh* m, const double* dev, int n, hipStream_t st);
 
