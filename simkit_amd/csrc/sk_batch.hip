@@ -5,9 +5,10 @@
 // grouped per problem (contiguous ranges). Mesh-wide phases (element
 // gradient, element Hessian + PSD projection, block gather into FP32 values)
 // run once over the union. The values are sliced per problem: chunks of 64
-// consecutive rows (one wave's rows in the per-problem sweeps), each padded to
-// its longest row and stored component-major, so that the wave's loads of one
-// block component fall in two cache lines. The phases with per-problem
+// consecutive rows (one wave's rows in the per-problem sweeps) stored
+// component-major, the chunk's rows on its lanes in descending block count, so
+// that a wave's load of one block component is one contiguous run of the lanes
+// whose rows still have blocks. The phases with per-problem
 // control flow are persistent kernels with one workgroup per problem (looping
 // over problems when there are more problems than workgroups):
 //   * PCG: block-Jacobi CG on the problem's free DOFs in FP32 with FP64
@@ -31,8 +32,10 @@
 #include "sk_internal.h"
 #include "sk_materials.h"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <numeric>
 #include <vector>
 
 namespace sk {
@@ -43,16 +46,19 @@ struct BatchArgs {
   const int32_t* vstart;   // [P+1]
   const int32_t* tstart;   // [P+1]
   const int32_t* pid;      // [nv] problem of each vertex
-  // Per-problem sliced matrix ("wave-ELL"): each problem's rows are cut into
-  // chunks of kWave consecutive rows starting at its first vertex; chunk c's
-  // slots woff[c] .. woff[c+1] - 1 hold block s of its row i at slot
-  // woff[c] + s kWave + i (rows padded with zero blocks to the chunk's longest
-  // row), component m at vals[(q - i) 9 + m kWave + i].
+  // Per-problem sliced matrix: each problem's rows are cut into chunks of
+  // kWave consecutive rows starting at its first vertex. Lane i of chunk c
+  // holds the chunk's row wlane[c kWave + i] & 0xff (the rows in descending
+  // block count, so the lanes still in a slot group are the first ones) with
+  // wlane >> 8 blocks; its block s sits at slot q = woff[c] + s kWave + i,
+  // component m at vals[(q - i) 9 + m kWave + i]. Slots past a row's blocks
+  // are never read.
   const int32_t* wchunk0;  // [P+1] first chunk of each problem
   const int32_t* woff;     // [nchunk+1] first slot of each chunk (a multiple of kWave)
-  const int32_t* wcol;     // [nslot] column vertex of each slot (padding: the row's own vertex)
+  const int32_t* wcol;     // [nslot] column vertex of each slot
+  const int32_t* wlane;    // [nchunk kWave] (block count << 8) | row within the chunk
   const float* vals;
-  const float* binv;       // chunk c's row i: component m at binv[9 kWave c + m kWave + i]
+  const float* binv;       // chunk c's lane i: component m at binv[9 kWave c + m kWave + i]
   const uint8_t* fixed;
   ElemView em;
   const double* mu;
@@ -152,17 +158,36 @@ __device__ __forceinline__ void all_max(double (&v)[NV], double* lds) {
 
 __device__ __forceinline__ bool is_fixed(const uint8_t* fixed, int64_t k) { return fixed && fixed[k]; }
 
-// y = A_row . src for row i of chunk c (valid: the thread has a row);
-// LANES lanes share a row (lane j takes slot groups j, j + LANES, ...) and
-// combine with xor shuffles. src is indexed by 3 * column - off; R: the
-// accumulation type.
+// A thread's row in a sweep: problem-relative sweep position rel -> chunk c,
+// lane i, vertex v and block count.
+struct RowAt {
+  int64_t v, c;
+  int i, nb;
+  bool ok;  // rel is within the problem
+};
+__device__ __forceinline__ RowAt row_at(const BatchArgs& a, int64_t v0, int64_t nrow, int64_t wc0, int64_t rel) {
+  RowAt r;
+  r.ok = rel < nrow;
+  r.c = wc0 + rel / kWave;
+  r.i = (int)(rel % kWave);
+  const int32_t w = r.ok ? a.wlane[r.c * kWave + r.i] : 0;
+  r.v = v0 + (rel - r.i) + (w & 0xff);
+  r.nb = w >> 8;
+  return r;
+}
+
+// y = A_row . src for the row of lane r.i of chunk r.c (when r.ok); LANES lanes
+// share a row (lane j takes slot groups j, j + LANES, ...) and combine with
+// xor shuffles. src is indexed by 3 * column - off; R: the accumulation type.
 template <int LANES, typename R, typename V>
-__device__ __forceinline__ void row_product(const BatchArgs& a, bool valid, int64_t c, int i,
-                                            const V* __restrict__ src, int64_t off, int j, R (&y)[3]) {
+__device__ __forceinline__ void row_product(const BatchArgs& a, const RowAt& r, const V* __restrict__ src,
+                                            int64_t off, int j, R (&y)[3]) {
   constexpr int step = LANES * kWave, kU = 4;
+  const int64_t c = r.c;
+  const int i = r.i;
   y[0] = y[1] = y[2] = R(0);
-  if (valid) {
-    const int32_t g1 = a.woff[c + 1];
+  if (r.ok) {
+    const int32_t g1 = a.woff[c] + r.nb * kWave;
     int32_t g = a.woff[c] + j * kWave;
     // kU slot groups at a time: their loads are all issued before the first
     // product (one memory latency per kU groups); products in slot order
@@ -402,10 +427,10 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
     // init: r64 = b = -g (free), dx = 0, r32 = b, x32 = 0, z = M^-1 r, p = z
     double acc[2] = {0.0, 0.0};
     for (int64_t s = 0; s < sweeps; ++s) {
-      const int64_t rel = s * RS + slot, v = v0 + rel, c = wc0 + rel / kWave;
-      const int ci = (int)(rel % kWave);
+      const RowAt R = row_at(a, v0, nrow, wc0, s * RS + slot);
+      const int64_t v = R.v;
       float tr[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-      if (v < v1 && j == 0) {
+      if (R.ok && j == 0) {
         float r[3], z[3];
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
@@ -418,7 +443,7 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
           a.x32[k] = 0.f;
           acc[1] += b * b;
         }
-        apply_binv3(a, c, ci, r[0], r[1], r[2], z);
+        apply_binv3(a, R.c, R.i, r[0], r[1], r[2], z);
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
           if (!KC) P[3 * v + q - off] = z[q];
@@ -434,12 +459,12 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
     if (KC) {
       rz += coarse_apply<NT, LANES>(a, cl, p, lds);
       for (int64_t s = 0; s < sweeps; ++s) {
-        const int64_t rel = s * RS + slot, v = v0 + rel, c = wc0 + rel / kWave;
-        const int ci = (int)(rel % kWave);
-        if (v < v1 && j == 0) {
+        const RowAt R = row_at(a, v0, nrow, wc0, s * RS + slot);
+        const int64_t v = R.v;
+        if (R.ok && j == 0) {
           float z[3], zc[3];
           const float* rv = Rv + 3 * v - off;
-          apply_binv3(a, c, ci, rv[0], rv[1], rv[2], z);
+          apply_binv3(a, R.c, R.i, rv[0], rv[1], rv[2], z);
           coarse_term(a, cl, v, v - v0, zc);
 #pragma unroll
           for (int q = 0; q < 3; ++q) P[3 * v + q - off] = z[q] + zc[q];
@@ -463,11 +488,11 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
       }
       double d1[1] = {0.0};
       for (int64_t s = 0; s < sweeps; ++s) {
-        const int64_t rel = s * RS + slot, v = v0 + rel, c = wc0 + rel / kWave;
-        const int ci = (int)(rel % kWave);
+        const RowAt R = row_at(a, v0, nrow, wc0, s * RS + slot);
+        const int64_t v = R.v;
         float y[3];
-        row_product<LANES, float>(a, v < v1, c, ci, P, off, j, y);
-        if (v < v1 && j == 0) {
+        row_product<LANES, float>(a, R, P, off, j, y);
+        if (R.ok && j == 0) {
 #pragma unroll
           for (int q = 0; q < 3; ++q) {
             const int64_t k = 3 * v + q;
@@ -487,10 +512,10 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
       // x += alpha p, r -= alpha A p, r.r; with KC also r.(B^-1 r) and the restriction of r
       double d2[2] = {0.0, 0.0};
       for (int64_t s = 0; s < sweeps; ++s) {
-        const int64_t rel = s * RS + slot, v = v0 + rel, c = wc0 + rel / kWave;
-        const int ci = (int)(rel % kWave);
+        const RowAt R = row_at(a, v0, nrow, wc0, s * RS + slot);
+        const int64_t v = R.v;
         float tr[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        if (v < v1 && j == 0) {
+        if (R.ok && j == 0) {
           float r[3];
 #pragma unroll
           for (int q = 0; q < 3; ++q) {
@@ -502,7 +527,7 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
           }
           if (KC) {
             float z[3];
-            apply_binv3(a, c, ci, r[0], r[1], r[2], z);
+            apply_binv3(a, R.c, R.i, r[0], r[1], r[2], z);
 #pragma unroll
             for (int q = 0; q < 3; ++q) d2[1] += (double)r[q] * (double)z[q];
             restrict_terms(a, v, r, tr);
@@ -538,12 +563,12 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
         __syncthreads();
         double d3[2] = {0.0, 0.0};
         for (int64_t s = 0; s < sweeps; ++s) {
-          const int64_t rel = s * RS + slot, v = v0 + rel, c = wc0 + rel / kWave;
-          const int ci = (int)(rel % kWave);
+          const RowAt R = row_at(a, v0, nrow, wc0, s * RS + slot);
+          const int64_t v = R.v;
           double y[3];
           float tr[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-          row_product<LANES, double>(a, v < v1, c, ci, a.dx, 0, j, y);
-          if (v < v1 && j == 0) {
+          row_product<LANES, double>(a, R, a.dx, 0, j, y);
+          if (R.ok && j == 0) {
             float rf[3];
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
@@ -556,7 +581,7 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
             }
             if (KC) {
               float z[3];
-              apply_binv3(a, c, ci, rf[0], rf[1], rf[2], z);
+              apply_binv3(a, R.c, R.i, rf[0], rf[1], rf[2], z);
 #pragma unroll
               for (int q = 0; q < 3; ++q) d3[1] += (double)rf[q] * (double)z[q];
               restrict_terms(a, v, rf, tr);
@@ -582,12 +607,12 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
         const float beta = (float)(rz_new / rz);
         rz = rz_new;
         for (int64_t s = 0; s < sweeps; ++s) {
-          const int64_t rel = s * RS + slot, v = v0 + rel, c = wc0 + rel / kWave;
-          const int ci = (int)(rel % kWave);
-          if (v < v1 && j == 0) {
+          const RowAt R = row_at(a, v0, nrow, wc0, s * RS + slot);
+          const int64_t v = R.v;
+          if (R.ok && j == 0) {
             float z[3], zc[3];
             const float* rv = Rv + 3 * v - off;
-            apply_binv3(a, c, ci, rv[0], rv[1], rv[2], z);
+            apply_binv3(a, R.c, R.i, rv[0], rv[1], rv[2], z);
             coarse_term(a, cl, v, v - v0, zc);
 #pragma unroll
             for (int q = 0; q < 3; ++q) P[3 * v + q - off] = z[q] + zc[q] + beta * P[3 * v + q - off];
@@ -599,12 +624,12 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
       // z = B^-1 r, r.z, then p = z + beta p
       double d4[1] = {0.0};
       for (int64_t s = 0; s < sweeps; ++s) {
-        const int64_t rel = s * RS + slot, v = v0 + rel, c = wc0 + rel / kWave;
-        const int ci = (int)(rel % kWave);
-        if (v < v1 && j == 0) {
+        const RowAt R = row_at(a, v0, nrow, wc0, s * RS + slot);
+        const int64_t v = R.v;
+        if (R.ok && j == 0) {
           float z[3];
           const float* rv = Rv + 3 * v - off;
-          apply_binv3(a, c, ci, rv[0], rv[1], rv[2], z);
+          apply_binv3(a, R.c, R.i, rv[0], rv[1], rv[2], z);
 #pragma unroll
           for (int q = 0; q < 3; ++q) d4[0] += (double)rv[q] * (double)z[q];
         }
@@ -613,12 +638,12 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
       const float beta = (float)(d4[0] / rz);
       rz = d4[0];
       for (int64_t s = 0; s < sweeps; ++s) {
-        const int64_t rel = s * RS + slot, v = v0 + rel, c = wc0 + rel / kWave;
-        const int ci = (int)(rel % kWave);
-        if (v < v1 && j == 0) {
+        const RowAt R = row_at(a, v0, nrow, wc0, s * RS + slot);
+        const int64_t v = R.v;
+        if (R.ok && j == 0) {
           float z[3];
           const float* rv = Rv + 3 * v - off;
-          apply_binv3(a, c, ci, rv[0], rv[1], rv[2], z);
+          apply_binv3(a, R.c, R.i, rv[0], rv[1], rv[2], z);
 #pragma unroll
           for (int q = 0; q < 3; ++q) P[3 * v + q - off] = z[q] + beta * P[3 * v + q - off];
         }
@@ -804,7 +829,7 @@ struct sk_batch {
   DevBuf<uint8_t> fixed;
   DevBuf<double> x, xt, g, dx, r64;
   DevBuf<float> vals32, binv32, r32, p32, Ap32, x32;
-  DevBuf<int32_t> wchunk0, woff, wcol, wpos, wrow;  // the sliced per-problem matrix (BatchArgs)
+  DevBuf<int32_t> wchunk0, woff, wcol, wlane, wpos, wrow;  // the sliced per-problem matrix (BatchArgs)
   DevBuf<int> active, status, newton_iters, ls_trials, refines, stalled, n_active;
   DevBuf<long long> pcg_iters;
   DevBuf<double> g0, gnorm, tstep;
@@ -837,6 +862,7 @@ BatchArgs sk_batch::args() const {
   a.wchunk0 = wchunk0.p;
   a.woff = woff.p;
   a.wcol = wcol.p;
+  a.wlane = wlane.p;
   a.vals = vals32.p;
   a.binv = binv32.p;
   a.fixed = fixed.p;
@@ -1234,8 +1260,9 @@ int sk_batch_create(sk_mesh* mesh, int nprob, const int64_t* vstart, const int64
       }
   }
   // The per-problem sliced matrix: chunks of kWave rows from each problem's
-  // first vertex, each padded with zero blocks to its longest row.
-  std::vector<int32_t> wchunk0(nprob + 1), woff(1, 0), wcol, wpos(mesh->nblk), wrow(mesh->nv);
+  // first vertex, the rows of a chunk on its lanes in descending block count
+  // (ties in row order), slot groups as wide as the chunk's longest row.
+  std::vector<int32_t> wchunk0(nprob + 1), woff(1, 0), wcol, wlane, wpos(mesh->nblk), wrow(mesh->nv);
   {
     // The per-problem SpMV reads only the problem's own rows: every block of
     // the mesh's pattern must couple two vertices of one problem.
@@ -1250,22 +1277,29 @@ int sk_batch_create(sk_mesh* mesh, int nprob, const int64_t* vstart, const int64
     for (int q = 0; q < nprob; ++q) {
       wchunk0[q] = (int32_t)(woff.size() - 1);
       for (int64_t r0 = vstart[q]; r0 < vstart[q + 1]; r0 += kWave) {
-        const int64_t r1 = std::min<int64_t>(vstart[q + 1], r0 + kWave), c = (int64_t)woff.size() - 1;
-        int32_t width = 0;
-        for (int64_t v = r0; v < r1; ++v) width = std::max(width, bp[v + 1] - bp[v]);
+        const int m = (int)(std::min<int64_t>(vstart[q + 1], r0 + kWave) - r0);
+        const int64_t c = (int64_t)woff.size() - 1;
+        auto nb = [&](int o) { return bp[r0 + o + 1] - bp[r0 + o]; };
+        std::vector<int> lane(m);
+        std::iota(lane.begin(), lane.end(), 0);
+        std::stable_sort(lane.begin(), lane.end(), [&](int x, int y) { return nb(x) > nb(y); });
+        const int32_t width = m ? nb(lane[0]) : 0;
+        SK_REQUIRE(width < (1 << 23), "sk_batch_create: row with too many blocks");
         const int64_t base = nslot;
         nslot += (int64_t)width * kWave;
         SK_REQUIRE(nslot <= INT32_MAX, "sk_batch_create: matrix too large for 32-bit slot indices");
         woff.push_back((int32_t)nslot);
-        wcol.resize(nslot, (int32_t)r0);  // slots of rows past the problem's end are never read
-        for (int64_t v = r0; v < r1; ++v) {
-          const int i = (int)(v - r0);
+        wcol.resize(nslot, (int32_t)r0);  // slots past a row's blocks are never read
+        wlane.resize((c + 1) * kWave, 0);
+        for (int i = 0; i < m; ++i) {
+          const int o = lane[i];
+          const int64_t v = r0 + o;
+          wlane[c * kWave + i] = (nb(o) << 8) | o;
           wrow[v] = (int32_t)(c * kWave + i);
-          for (int32_t k = 0; k < width; ++k) {
+          for (int32_t k = 0; k < nb(o); ++k) {
             const int64_t sl = base + (int64_t)k * kWave + i;
-            const int32_t b = bp[v] + k;
-            wcol[sl] = b < bp[v + 1] ? bc[b] : (int32_t)v;
-            if (b < bp[v + 1]) wpos[b] = (int32_t)sl;
+            wcol[sl] = bc[bp[v] + k];
+            wpos[bp[v] + k] = (int32_t)sl;
           }
         }
       }
@@ -1307,6 +1341,7 @@ int sk_batch_create(sk_mesh* mesh, int nprob, const int64_t* vstart, const int64
     SK_TRY(B->wchunk0.upload(wchunk0));
     SK_TRY(B->woff.upload(woff));
     SK_TRY(B->wcol.upload(wcol));
+    SK_TRY(B->wlane.upload(wlane));
     SK_TRY(B->wpos.upload(wpos));
     SK_TRY(B->wrow.upload(wrow));
     for (DevBuf<int>* b : {&B->active, &B->status, &B->newton_iters, &B->ls_trials, &B->refines, &B->stalled}) {
