@@ -8,7 +8,8 @@
 // consecutive rows (one wave's rows in the per-problem sweeps) stored
 // component-major, the chunk's rows on its lanes in descending block count, so
 // that a wave's load of one block component is one contiguous run of the lanes
-// whose rows still have blocks. The phases with per-problem
+// whose rows still have blocks; components 0-3 and 4-7 of a block are float4s
+// (three loads per block). The phases with per-problem
 // control flow are persistent kernels with one workgroup per problem (looping
 // over problems when there are more problems than workgroups):
 //   * PCG: block-Jacobi CG on the problem's free DOFs in FP32 with FP64
@@ -51,14 +52,14 @@ struct BatchArgs {
   // holds the chunk's row wlane[c kWave + i] & 0xff (the rows in descending
   // block count, so the lanes still in a slot group are the first ones) with
   // wlane >> 8 blocks; its block s sits at slot q = woff[c] + s kWave + i,
-  // component m at vals[(q - i) 9 + m kWave + i]. Slots past a row's blocks
-  // are never read.
+  // in the quad-interleaved layout (quad_index: components 0-3 and 4-7 as
+  // float4s, 8 alone). Slots past a row's blocks are never read.
   const int32_t* wchunk0;  // [P+1] first chunk of each problem
   const int32_t* woff;     // [nchunk+1] first slot of each chunk (a multiple of kWave)
   const int32_t* wcol;     // [nslot] column vertex of each slot
   const int32_t* wlane;    // [nchunk kWave] (block count << 8) | row within the chunk
   const float* vals;
-  const float* binv;       // chunk c's lane i: component m at binv[9 kWave c + m kWave + i]
+  const float* binv;       // chunk c's lane i: slot kWave c + i of the quad layout
   const uint8_t* fixed;
   ElemView em;
   const double* mu;
@@ -158,6 +159,25 @@ __device__ __forceinline__ void all_max(double (&v)[NV], double* lds) {
 
 __device__ __forceinline__ bool is_fixed(const uint8_t* fixed, int64_t k) { return fixed && fixed[k]; }
 
+static_assert(kQuadC == kWave, "a slot group of the quad layout is one wave's rows");
+
+// Lane i's 9-float block of the slot group starting at slot g (a multiple of
+// kQuadC) in the quad-interleaved layout: two float4 loads and one float.
+__device__ __forceinline__ void load_block(const float* __restrict__ base, int64_t g, int i, float (&A)[9]) {
+  const float* G = base + 9 * g;
+  const float4 q0 = *reinterpret_cast<const float4*>(G + 4 * i);
+  const float4 q1 = *reinterpret_cast<const float4*>(G + 4 * kQuadC + 4 * i);
+  A[0] = q0.x;
+  A[1] = q0.y;
+  A[2] = q0.z;
+  A[3] = q0.w;
+  A[4] = q1.x;
+  A[5] = q1.y;
+  A[6] = q1.z;
+  A[7] = q1.w;
+  A[8] = G[8 * kQuadC + i];
+}
+
 // A thread's row in a sweep: problem-relative sweep position rel -> chunk c,
 // lane i, vertex v and block count.
 struct RowAt {
@@ -197,9 +217,7 @@ __device__ __forceinline__ void row_product(const BatchArgs& a, const RowAt& r, 
 #pragma unroll
       for (int u = 0; u < kU; ++u) {
         w[u] = 3 * (int64_t)a.wcol[g + u * step + i] - off;
-        const float* Au = a.vals + 9 * (int64_t)(g + u * step) + i;
-#pragma unroll
-        for (int m = 0; m < 9; ++m) A[u][m] = Au[m * kWave];
+        load_block(a.vals, g + u * step, i, A[u]);
       }
 #pragma unroll
       for (int u = 0; u < kU; ++u) {
@@ -210,11 +228,11 @@ __device__ __forceinline__ void row_product(const BatchArgs& a, const RowAt& r, 
     }
     for (; g < g1; g += step) {
       const int64_t w = 3 * (int64_t)a.wcol[g + i] - off;
-      const float* A = a.vals + 9 * (int64_t)g + i;
+      float A[9];
+      load_block(a.vals, g, i, A);
       const R s0 = (R)src[w], s1 = (R)src[w + 1], s2 = (R)src[w + 2];
 #pragma unroll
-      for (int p = 0; p < 3; ++p)
-        y[p] += (R)A[3 * p * kWave] * s0 + (R)A[(3 * p + 1) * kWave] * s1 + (R)A[(3 * p + 2) * kWave] * s2;
+      for (int p = 0; p < 3; ++p) y[p] += (R)A[3 * p] * s0 + (R)A[3 * p + 1] * s1 + (R)A[3 * p + 2] * s2;
     }
   }
 #pragma unroll
@@ -226,10 +244,10 @@ __device__ __forceinline__ void row_product(const BatchArgs& a, const RowAt& r, 
 // z = B^-1 r with the diagonal inverse of row i of chunk c.
 __device__ __forceinline__ void apply_binv3(const BatchArgs& a, int64_t c, int i, float r0, float r1, float r2,
                                             float (&z)[3]) {
-  const float* B = a.binv + 9 * kWave * c + i;
+  float B[9];
+  load_block(a.binv, kWave * c, i, B);
 #pragma unroll
-  for (int p = 0; p < 3; ++p)
-    z[p] = B[3 * p * kWave] * r0 + B[(3 * p + 1) * kWave] * r1 + B[(3 * p + 2) * kWave] * r2;
+  for (int p = 0; p < 3; ++p) z[p] = B[3 * p] * r0 + B[3 * p + 1] * r1 + B[3 * p + 2] * r2;
 }
 
 // The workgroup's shared flag read: one thread reads, all threads branch alike.
@@ -1035,12 +1053,11 @@ __global__ __launch_bounds__(kBlock) void batch_galerkin_kernel(GalerkinBatchArg
         const int64_t w = a.bcol[b];
         if (g0 + (int)((w - v0) / a.agg_rows) != J) continue;
         double A[9];
-        const int64_t q = a.wpos[b], i = q % kWave;
-        const float* src = a.vals + 9 * (q - i) + i;
+        const int64_t q = a.wpos[b];
 #pragma unroll
         for (int m = 0; m < 9; ++m) {
           const bool fr = !(a.fixed && (a.fixed[3 * v + m / 3] || a.fixed[3 * w + m % 3]));
-          A[m] = fr ? (double)src[m * kWave] : 0.0;
+          A[m] = fr ? (double)a.vals[quad_index(q, m)] : 0.0;
         }
         double Rw[3][6], AR[3][6];
 #pragma unroll
@@ -1157,8 +1174,8 @@ int hessian_phase(sk_batch* B, bool only_active) {
   const int32_t* pid = only_active ? B->pid.p : nullptr;
   SK_TRY(launch_elem_hess(m, B->prm.material, B->x.p, nullptr, B->mu.p, B->lam.p, B->vol.p, B->prm.psd, true,
                           B->st, act, pid));
-  return launch_gather_hess_slots(m, B->vals32.p, B->mass.p, 1.0 / (B->prm.h * B->prm.h), B->fixed.p,
-                                  B->binv32.p, B->wpos.p, B->wrow.p, kWave, true, B->st, act, pid);
+  return launch_gather_hess_quad(m, B->vals32.p, B->mass.p, 1.0 / (B->prm.h * B->prm.h), B->fixed.p,
+                                 B->binv32.p, B->wpos.p, B->wrow.p, true, B->st, act, pid);
 }
 
 // One Newton iteration of every active problem (no host synchronization).

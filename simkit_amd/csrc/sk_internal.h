@@ -148,13 +148,21 @@ enum ValueLayout : int { kLayoutCSR = 0, kLayoutSELL = 1, kLayoutBSR = 2 };
 int launch_gather_hess(sk_mesh* m, void* values, const double* mass, double inv_h2, const uint8_t* fixed,
                        void* binv, int layout, bool fp32, hipStream_t st, const int* active = nullptr,
                        const int32_t* pid = nullptr);
-// The same gather into a caller's sliced layout of chunk width C (a power of
-// two): block b goes to slot q = pos[b], component m of it at
-// values[(q - i) * 9 + m * C + i] with i = q % C; the diagonal inverse of
-// vertex v to the same addresses with q = row_pos[v].
-int launch_gather_hess_slots(sk_mesh* m, void* values, const double* mass, double inv_h2, const uint8_t* fixed,
-                             void* binv, const int32_t* pos, const int32_t* row_pos, int C, bool fp32,
-                             hipStream_t st, const int* active = nullptr, const int32_t* pid = nullptr);
+// Quad-interleaved sliced layout of 9-float blocks in slot groups of kQuadC
+// lanes: slot q = G + i (G a multiple of kQuadC) keeps components 0-3 and 4-7
+// of lane i as the float4s at 9 G + 4 i and 9 G + 4 kQuadC + 4 i, and
+// component 8 at 9 G + 8 kQuadC + i, so that a wave reads a group's blocks
+// with three coalesced loads.
+constexpr int kQuadC = 64;
+__host__ __device__ inline int64_t quad_index(int64_t q, int m) {
+  const int64_t i = q & (kQuadC - 1), b = 9 * (q - i);
+  return m < 4 ? b + 4 * i + m : (m < 8 ? b + 4 * kQuadC + 4 * i + (m - 4) : b + 8 * kQuadC + i);
+}
+// The same gather into a caller's quad-interleaved layout: block b goes to
+// slot pos[b], the diagonal inverse of vertex v to slot row_pos[v] of binv.
+int launch_gather_hess_quad(sk_mesh* m, void* values, const double* mass, double inv_h2, const uint8_t* fixed,
+                            void* binv, const int32_t* pos, const int32_t* row_pos, bool fp32, hipStream_t st,
+                            const int* active = nullptr, const int32_t* pid = nullptr);
 // Copies n doubles from device to the mesh's pinned buffer and waits.
 int read_scalars(sk_mesh* m, const double* dev, int n, hipStream_t st);
 

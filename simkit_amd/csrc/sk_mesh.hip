@@ -245,9 +245,10 @@ struct GatherHArgs {
   double inv_h2;
   const uint8_t* fixed;
   void* binv;
-  const int32_t* sell_pos;  // non-null: sliced layout of chunk width pos_c (slot of each block)
-  int pos_c;
-  const int32_t* binv_pos;  // non-null: the diagonal inverses in the sliced layout (slot of each vertex)
+  const int32_t* sell_pos;  // non-null: sliced layout (slot of each block): SELL of chunk width
+  int pos_c;                //   pos_c, or quad-interleaved (quad_index) with quad = 1
+  int quad;
+  const int32_t* binv_pos;  // non-null: the diagonal inverses in the quad layout (slot of each vertex)
   int bsr;                  // 1: BSR layout (block b at values[9 b + 3 p + q])
   const int* active;        // optional (batched problems): skip blocks whose problem is inactive
   const int32_t* pid;
@@ -291,7 +292,11 @@ __global__ __launch_bounds__(kBlock) void gather_hess_kernel(GatherHArgs a) {
     for (int p = 0; p < 3; ++p) acc[4 * p] += (R)(a.mass[3 * (int64_t)v + p] * a.inv_h2);
   }
   R* values = reinterpret_cast<R*>(a.values);
-  if (a.sell_pos) {
+  if (a.sell_pos && a.quad) {
+    const int64_t q = a.sell_pos[b];
+#pragma unroll
+    for (int m = 0; m < 9; ++m) values[quad_index(q, m)] = acc[m];
+  } else if (a.sell_pos) {
     const int64_t q = a.sell_pos[b];
     const int64_t i = q & (a.pos_c - 1);
     R* out = values + (q - i) * 9 + i;
@@ -322,16 +327,13 @@ __global__ __launch_bounds__(kBlock) void gather_hess_kernel(GatherHArgs a) {
     const R det = A[0] * c[0] + A[1] * c[1] + A[2] * c[2];
     const R inv = R(1) / det;
     R* binv = reinterpret_cast<R*>(a.binv);
-    int64_t base = 9 * (int64_t)v, stride = 1;
-    if (a.binv_pos) {
-      const int64_t q = a.binv_pos[v], i = q & (a.pos_c - 1);
-      base = (q - i) * 9 + i;
-      stride = a.pos_c;
-    }
 #pragma unroll
     for (int p = 0; p < 3; ++p)
 #pragma unroll
-      for (int q = 0; q < 3; ++q) binv[base + (3 * p + q) * stride] = f[p] * f[q] * c[3 * q + p] * inv;
+      for (int q = 0; q < 3; ++q) {
+        const int64_t at = a.binv_pos ? quad_index(a.binv_pos[v], 3 * p + q) : 9 * (int64_t)v + 3 * p + q;
+        binv[at] = f[p] * f[q] * c[3 * q + p] * inv;
+      }
   }
 }
 
@@ -468,7 +470,7 @@ int launch_gather_hess(sk_mesh* m, void* values, const double* mass, double inv_
   SK_REQUIRE(layout == kLayoutCSR || layout == kLayoutSELL || layout == kLayoutBSR, "invalid value layout");
   SK_REQUIRE(!active == !pid, "launch_gather_hess: active and pid go together");
   GatherHArgs a{m->nblk, m->bptr.p, m->bcol.p, m->brow.p, m->cptr.p, m->clist.p, m->Kst.p,
-                values, mass, inv_h2, fixed, binv, layout == kLayoutSELL ? m->sell_pos.p : nullptr, kSellC,
+                values, mass, inv_h2, fixed, binv, layout == kLayoutSELL ? m->sell_pos.p : nullptr, kSellC, 0,
                 nullptr, layout == kLayoutBSR ? 1 : 0, active, pid};
   if (fp32) gather_hess_kernel<float><<<grid_for(m->nblk), kBlock, 0, st>>>(a);
   else gather_hess_kernel<double><<<grid_for(m->nblk), kBlock, 0, st>>>(a);
@@ -476,14 +478,13 @@ int launch_gather_hess(sk_mesh* m, void* values, const double* mass, double inv_
   return SK_OK;
 }
 
-int launch_gather_hess_slots(sk_mesh* m, void* values, const double* mass, double inv_h2, const uint8_t* fixed,
-                             void* binv, const int32_t* pos, const int32_t* row_pos, int C, bool fp32,
-                             hipStream_t st, const int* active, const int32_t* pid) {
-  SK_REQUIRE(pos && (!binv || row_pos), "launch_gather_hess_slots: missing slot map");
-  SK_REQUIRE(C > 0 && (C & (C - 1)) == 0, "launch_gather_hess_slots: chunk width must be a power of two");
-  SK_REQUIRE(!active == !pid, "launch_gather_hess_slots: active and pid go together");
+int launch_gather_hess_quad(sk_mesh* m, void* values, const double* mass, double inv_h2, const uint8_t* fixed,
+                            void* binv, const int32_t* pos, const int32_t* row_pos, bool fp32, hipStream_t st,
+                            const int* active, const int32_t* pid) {
+  SK_REQUIRE(pos && (!binv || row_pos), "launch_gather_hess_quad: missing slot map");
+  SK_REQUIRE(!active == !pid, "launch_gather_hess_quad: active and pid go together");
   GatherHArgs a{m->nblk, m->bptr.p, m->bcol.p, m->brow.p, m->cptr.p, m->clist.p, m->Kst.p,
-                values, mass, inv_h2, fixed, binv, pos, C, row_pos, 0, active, pid};
+                values, mass, inv_h2, fixed, binv, pos, kQuadC, 1, row_pos, 0, active, pid};
   if (fp32) gather_hess_kernel<float><<<grid_for(m->nblk), kBlock, 0, st>>>(a);
   else gather_hess_kernel<double><<<grid_for(m->nblk), kBlock, 0, st>>>(a);
   SK_HIP_CHECK(hipGetLastError());
