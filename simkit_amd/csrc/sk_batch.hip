@@ -52,14 +52,14 @@ struct BatchArgs {
   // holds the chunk's row wlane[c kWave + i] & 0xff (the rows in descending
   // block count, so the lanes still in a slot group are the first ones) with
   // wlane >> 8 blocks; its block s sits at slot q = woff[c] + s kWave + i,
-  // in the quad-interleaved layout (quad_index: components 0-3 and 4-7 as
+  // in the slot_index<kQuadC, kQuadVW> layout (components 0-3 and 4-7 as
   // float4s, 8 alone). Slots past a row's blocks are never read.
   const int32_t* wchunk0;  // [P+1] first chunk of each problem
   const int32_t* woff;     // [nchunk+1] first slot of each chunk (a multiple of kWave)
   const int32_t* wcol;     // [nslot] column vertex of each slot
   const int32_t* wlane;    // [nchunk kWave] (block count << 8) | row within the chunk
   const float* vals;
-  const float* binv;       // chunk c's lane i: slot kWave c + i of the quad layout
+  const float* binv;       // chunk c's lane i: slot kWave c + i of the same layout
   const uint8_t* fixed;
   ElemView em;
   const double* mu;
@@ -161,23 +161,6 @@ __device__ __forceinline__ bool is_fixed(const uint8_t* fixed, int64_t k) { retu
 
 static_assert(kQuadC == kWave, "a slot group of the quad layout is one wave's rows");
 
-// Lane i's 9-float block of the slot group starting at slot g (a multiple of
-// kQuadC) in the quad-interleaved layout: two float4 loads and one float.
-__device__ __forceinline__ void load_block(const float* __restrict__ base, int64_t g, int i, float (&A)[9]) {
-  const float* G = base + 9 * g;
-  const float4 q0 = *reinterpret_cast<const float4*>(G + 4 * i);
-  const float4 q1 = *reinterpret_cast<const float4*>(G + 4 * kQuadC + 4 * i);
-  A[0] = q0.x;
-  A[1] = q0.y;
-  A[2] = q0.z;
-  A[3] = q0.w;
-  A[4] = q1.x;
-  A[5] = q1.y;
-  A[6] = q1.z;
-  A[7] = q1.w;
-  A[8] = G[8 * kQuadC + i];
-}
-
 // A thread's row in a sweep: problem-relative sweep position rel -> chunk c,
 // lane i, vertex v and block count.
 struct RowAt {
@@ -217,7 +200,7 @@ __device__ __forceinline__ void row_product(const BatchArgs& a, const RowAt& r, 
 #pragma unroll
       for (int u = 0; u < kU; ++u) {
         w[u] = 3 * (int64_t)a.wcol[g + u * step + i] - off;
-        load_block(a.vals, g + u * step, i, A[u]);
+        load_slot<float, kQuadC, kQuadVW>(a.vals, g + u * step, i, A[u]);
       }
 #pragma unroll
       for (int u = 0; u < kU; ++u) {
@@ -229,7 +212,7 @@ __device__ __forceinline__ void row_product(const BatchArgs& a, const RowAt& r, 
     for (; g < g1; g += step) {
       const int64_t w = 3 * (int64_t)a.wcol[g + i] - off;
       float A[9];
-      load_block(a.vals, g, i, A);
+      load_slot<float, kQuadC, kQuadVW>(a.vals, g, i, A);
       const R s0 = (R)src[w], s1 = (R)src[w + 1], s2 = (R)src[w + 2];
 #pragma unroll
       for (int p = 0; p < 3; ++p) y[p] += (R)A[3 * p] * s0 + (R)A[3 * p + 1] * s1 + (R)A[3 * p + 2] * s2;
@@ -245,7 +228,7 @@ __device__ __forceinline__ void row_product(const BatchArgs& a, const RowAt& r, 
 __device__ __forceinline__ void apply_binv3(const BatchArgs& a, int64_t c, int i, float r0, float r1, float r2,
                                             float (&z)[3]) {
   float B[9];
-  load_block(a.binv, kWave * c, i, B);
+  load_slot<float, kQuadC, kQuadVW>(a.binv, kWave * c, i, B);
 #pragma unroll
   for (int p = 0; p < 3; ++p) z[p] = B[3 * p] * r0 + B[3 * p + 1] * r1 + B[3 * p + 2] * r2;
 }
@@ -1057,7 +1040,7 @@ __global__ __launch_bounds__(kBlock) void batch_galerkin_kernel(GalerkinBatchArg
 #pragma unroll
         for (int m = 0; m < 9; ++m) {
           const bool fr = !(a.fixed && (a.fixed[3 * v + m / 3] || a.fixed[3 * w + m % 3]));
-          A[m] = fr ? (double)a.vals[quad_index(q, m)] : 0.0;
+          A[m] = fr ? (double)a.vals[slot_index<kQuadC, kQuadVW>(q, m)] : 0.0;
         }
         double Rw[3][6], AR[3][6];
 #pragma unroll

@@ -100,13 +100,11 @@ __global__ __launch_bounds__(kBlock) void coarse_galerkin_kernel(GalerkinArgs<R>
         const int w = a.bcol[b];
         if (a.vagg[w] != J) continue;
         const int64_t q = a.sell_pos[b];
-        const int64_t i = q % kSellC;
-        const R* src = a.vals + (q - i) * 9 + i;
         double A[9];
 #pragma unroll
         for (int m = 0; m < 9; ++m) {
           const bool fr = !(a.fixed && (a.fixed[3 * (int64_t)v + m / 3] || a.fixed[3 * (int64_t)w + m % 3]));
-          A[m] = fr ? (double)src[m * kSellC] : 0.0;
+          A[m] = fr ? (double)a.vals[slot_index<kSellC, 1>(q, m)] : 0.0;
         }
         const double dw[3] = {a.dv[3 * w], a.dv[3 * w + 1], a.dv[3 * w + 2]};
         double Rw[3][6], AR[3][6];

@@ -94,9 +94,9 @@ struct sk_mesh {
   // chunk_deg[c] slots per row and starts at slot chunk_off[c]. Slot
   // q = chunk_off[c] + s*kSellC + i holds block s of the chunk's row i; its
   // column vertex is scol[q] and its value m sits at
-  // values[(q - i) * 9 + m * kSellC + i] (component-major within a slot group),
-  // so each component load of a chunk is one contiguous run. Padding slots
-  // have value 0.
+  // values[(q - i) * 9 + m * kSellC + i] (slot_index<kSellC, 1>:
+  // component-major within a slot group), so each component load of a chunk
+  // is one contiguous run. Padding slots have value 0.
   sk::DevBuf<int32_t> sell_row;   // [nchunk * kSellC] original row, -1 for padding
   sk::DevBuf<int32_t> chunk_deg;  // [nchunk]
   sk::DevBuf<int32_t> chunk_off;  // [nchunk + 1]
@@ -148,18 +148,47 @@ enum ValueLayout : int { kLayoutCSR = 0, kLayoutSELL = 1, kLayoutBSR = 2 };
 int launch_gather_hess(sk_mesh* m, void* values, const double* mass, double inv_h2, const uint8_t* fixed,
                        void* binv, int layout, bool fp32, hipStream_t st, const int* active = nullptr,
                        const int32_t* pid = nullptr);
-// Quad-interleaved sliced layout of 9-float blocks in slot groups of kQuadC
-// lanes: slot q = G + i (G a multiple of kQuadC) keeps components 0-3 and 4-7
-// of lane i as the float4s at 9 G + 4 i and 9 G + 4 kQuadC + 4 i, and
-// component 8 at 9 G + 8 kQuadC + i, so that a wave reads a group's blocks
-// with three coalesced loads.
-constexpr int kQuadC = 64;
-__host__ __device__ inline int64_t quad_index(int64_t q, int m) {
-  const int64_t i = q & (kQuadC - 1), b = 9 * (q - i);
-  return m < 4 ? b + 4 * i + m : (m < 8 ? b + 4 * kQuadC + 4 * i + (m - 4) : b + 8 * kQuadC + i);
+// Sliced layout of 9-component blocks in slot groups of C lanes (a power of
+// two): slot q = G + i (G a multiple of C) keeps lane i's components 0-7 in
+// vectors of VW components (VW = 1, or 16 bytes: 4 floats / 2 doubles) and
+// component 8 alone,
+//   component m < 8 at 9 G + (m / VW) C VW + VW i + m % VW,
+//   component 8     at 9 G + 8 C + i,
+// so that each load of a wave covers one contiguous run per slot group
+// (VW = 1: component-major; VW = 4 for float: three loads per block).
+template <int C, int VW>
+__host__ __device__ inline int64_t slot_index(int64_t q, int m) {
+  const int64_t i = q & (C - 1), b = 9 * (q - i);
+  return m < 8 ? b + (m / VW) * C * VW + VW * i + m % VW : b + 8 * C + i;
 }
-// The same gather into a caller's quad-interleaved layout: block b goes to
-// slot pos[b], the diagonal inverse of vertex v to slot row_pos[v] of binv.
+// Lane i's block of the slot group starting at slot g (a multiple of C).
+template <typename T, int C, int VW>
+__device__ __forceinline__ void load_slot(const T* __restrict__ base, int64_t g, int i, T (&A)[9]) {
+  const T* G = base + 9 * g;
+  if constexpr (VW == 4 && sizeof(T) == 4) {
+    const float4 q0 = *reinterpret_cast<const float4*>(G + 4 * i);
+    const float4 q1 = *reinterpret_cast<const float4*>(G + 4 * C + 4 * i);
+    A[0] = q0.x;
+    A[1] = q0.y;
+    A[2] = q0.z;
+    A[3] = q0.w;
+    A[4] = q1.x;
+    A[5] = q1.y;
+    A[6] = q1.z;
+    A[7] = q1.w;
+  } else {
+    static_assert(VW == 1, "load_slot: VW is 1, or 4 for float");
+#pragma unroll
+    for (int m = 0; m < 8; ++m) A[m] = G[m * C + i];
+  }
+  A[8] = G[8 * C + i];
+}
+// Slot groups of the batch's per-problem matrix: one wave's rows, float4
+// vectors of components.
+constexpr int kQuadC = 64, kQuadVW = 4;
+// The same gather into a caller's sliced layout of slot groups of kQuadC
+// lanes: block b goes to slot pos[b], the diagonal inverse of vertex v to slot
+// row_pos[v] of binv.
 int launch_gather_hess_quad(sk_mesh* m, void* values, const double* mass, double inv_h2, const uint8_t* fixed,
                             void* binv, const int32_t* pos, const int32_t* row_pos, bool fp32, hipStream_t st,
                             const int* active = nullptr, const int32_t* pid = nullptr);

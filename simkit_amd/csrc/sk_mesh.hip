@@ -245,9 +245,8 @@ struct GatherHArgs {
   double inv_h2;
   const uint8_t* fixed;
   void* binv;
-  const int32_t* sell_pos;  // non-null: sliced layout (slot of each block): SELL of chunk width
-  int pos_c;                //   pos_c, or quad-interleaved (quad_index) with quad = 1
-  int quad;
+  const int32_t* sell_pos;  // non-null: sliced layout (slot_index; slot of each block): the
+  int quad;                 //   SELL <kSellC, 1>, or with quad = 1 <kQuadC, kQuadVW> (FP32)
   const int32_t* binv_pos;  // non-null: the diagonal inverses in the quad layout (slot of each vertex)
   int bsr;                  // 1: BSR layout (block b at values[9 b + 3 p + q])
   const int* active;        // optional (batched problems): skip blocks whose problem is inactive
@@ -292,16 +291,11 @@ __global__ __launch_bounds__(kBlock) void gather_hess_kernel(GatherHArgs a) {
     for (int p = 0; p < 3; ++p) acc[4 * p] += (R)(a.mass[3 * (int64_t)v + p] * a.inv_h2);
   }
   R* values = reinterpret_cast<R*>(a.values);
-  if (a.sell_pos && a.quad) {
+  if (a.sell_pos) {
     const int64_t q = a.sell_pos[b];
 #pragma unroll
-    for (int m = 0; m < 9; ++m) values[quad_index(q, m)] = acc[m];
-  } else if (a.sell_pos) {
-    const int64_t q = a.sell_pos[b];
-    const int64_t i = q & (a.pos_c - 1);
-    R* out = values + (q - i) * 9 + i;
-#pragma unroll
-    for (int m = 0; m < 9; ++m) out[m * a.pos_c] = acc[m];
+    for (int m = 0; m < 9; ++m)
+      values[a.quad ? slot_index<kQuadC, kQuadVW>(q, m) : slot_index<kSellC, 1>(q, m)] = acc[m];
   } else if (a.bsr) {
     R* out = values + 9 * b;
 #pragma unroll
@@ -331,7 +325,8 @@ __global__ __launch_bounds__(kBlock) void gather_hess_kernel(GatherHArgs a) {
     for (int p = 0; p < 3; ++p)
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
-        const int64_t at = a.binv_pos ? quad_index(a.binv_pos[v], 3 * p + q) : 9 * (int64_t)v + 3 * p + q;
+        const int64_t at =
+            a.binv_pos ? slot_index<kQuadC, kQuadVW>(a.binv_pos[v], 3 * p + q) : 9 * (int64_t)v + 3 * p + q;
         binv[at] = f[p] * f[q] * c[3 * q + p] * inv;
       }
   }
@@ -470,7 +465,7 @@ int launch_gather_hess(sk_mesh* m, void* values, const double* mass, double inv_
   SK_REQUIRE(layout == kLayoutCSR || layout == kLayoutSELL || layout == kLayoutBSR, "invalid value layout");
   SK_REQUIRE(!active == !pid, "launch_gather_hess: active and pid go together");
   GatherHArgs a{m->nblk, m->bptr.p, m->bcol.p, m->brow.p, m->cptr.p, m->clist.p, m->Kst.p,
-                values, mass, inv_h2, fixed, binv, layout == kLayoutSELL ? m->sell_pos.p : nullptr, kSellC, 0,
+                values, mass, inv_h2, fixed, binv, layout == kLayoutSELL ? m->sell_pos.p : nullptr, 0,
                 nullptr, layout == kLayoutBSR ? 1 : 0, active, pid};
   if (fp32) gather_hess_kernel<float><<<grid_for(m->nblk), kBlock, 0, st>>>(a);
   else gather_hess_kernel<double><<<grid_for(m->nblk), kBlock, 0, st>>>(a);
@@ -482,9 +477,10 @@ int launch_gather_hess_quad(sk_mesh* m, void* values, const double* mass, double
                             void* binv, const int32_t* pos, const int32_t* row_pos, bool fp32, hipStream_t st,
                             const int* active, const int32_t* pid) {
   SK_REQUIRE(pos && (!binv || row_pos), "launch_gather_hess_quad: missing slot map");
+  SK_REQUIRE(fp32, "launch_gather_hess_quad: the quad layout holds FP32 values");
   SK_REQUIRE(!active == !pid, "launch_gather_hess_quad: active and pid go together");
   GatherHArgs a{m->nblk, m->bptr.p, m->bcol.p, m->brow.p, m->cptr.p, m->clist.p, m->Kst.p,
-                values, mass, inv_h2, fixed, binv, pos, kQuadC, 1, row_pos, 0, active, pid};
+                values, mass, inv_h2, fixed, binv, pos, 1, row_pos, 0, active, pid};
   if (fp32) gather_hess_kernel<float><<<grid_for(m->nblk), kBlock, 0, st>>>(a);
   else gather_hess_kernel<double><<<grid_for(m->nblk), kBlock, 0, st>>>(a);
   SK_HIP_CHECK(hipGetLastError());

@@ -323,11 +323,10 @@ __device__ __forceinline__ void sell_row_product(const SellView& sv, const T* __
     const int64_t w = 3 * (int64_t)sv.scol[g + i];
     V pv[3];
     xv(w, pv);
-    const T* __restrict__ A = vals + g * 9 + i;
+    T A[9];
+    load_slot<T, kSellC, 1>(vals, g, i, A);
 #pragma unroll
-    for (int p = 0; p < 3; ++p)
-      y[p] += (V)A[(3 * p) * kSellC] * pv[0] + (V)A[(3 * p + 1) * kSellC] * pv[1] +
-              (V)A[(3 * p + 2) * kSellC] * pv[2];
+    for (int p = 0; p < 3; ++p) y[p] += (V)A[3 * p] * pv[0] + (V)A[3 * p + 1] * pv[1] + (V)A[3 * p + 2] * pv[2];
   }
 #pragma unroll
   for (int off2 = kSellC; off2 < kWave; off2 <<= 1) {
