@@ -50,14 +50,15 @@ struct BatchArgs {
   // Per-problem sliced matrix: each problem's rows are cut into chunks of
   // kWave consecutive rows starting at its first vertex. Lane i of chunk c
   // holds the chunk's row wlane[c kWave + i] & 0xff (the rows in descending
-  // block count, so the lanes still in a slot group are the first ones) with
-  // wlane >> 8 blocks; its block s sits at slot q = woff[c] + s kWave + i,
+  // block count, so the lanes still in a slot group are the first ones), the
+  // row's fixed-DOF mask in bits 8-10 and its block count from bit 11; its
+  // block s sits at slot q = woff[c] + s kWave + i,
   // in the slot_index<kQuadC, kQuadVW> layout (components 0-3 and 4-7 as
   // float4s, 8 alone). Slots past a row's blocks are never read.
   const int32_t* wchunk0;  // [P+1] first chunk of each problem
   const int32_t* woff;     // [nchunk+1] first slot of each chunk (a multiple of kWave)
   const int32_t* wcol;     // [nslot] column vertex of each slot
-  const int32_t* wlane;    // [nchunk kWave] (block count << 8) | row within the chunk
+  const int32_t* wlane;    // [nchunk kWave] (block count << 11) | (fixed mask << 8) | row in the chunk
   const float* vals;
   const float* binv;       // chunk c's lane i: slot kWave c + i of the same layout
   const uint8_t* fixed;
@@ -166,6 +167,7 @@ static_assert(kQuadC == kWave, "a slot group of the quad layout is one wave's ro
 struct RowAt {
   int64_t v, c;
   int i, nb;
+  int fx;   // fixed-DOF mask of the row (bit q: DOF 3 v + q)
   bool ok;  // rel is within the problem
 };
 __device__ __forceinline__ RowAt row_at(const BatchArgs& a, int64_t v0, int64_t nrow, int64_t wc0, int64_t rel) {
@@ -175,7 +177,8 @@ __device__ __forceinline__ RowAt row_at(const BatchArgs& a, int64_t v0, int64_t 
   r.i = (int)(rel % kWave);
   const int32_t w = r.ok ? a.wlane[r.c * kWave + r.i] : 0;
   r.v = v0 + (rel - r.i) + (w & 0xff);
-  r.nb = w >> 8;
+  r.fx = (w >> 8) & 7;
+  r.nb = w >> 11;
   return r;
 }
 
@@ -375,15 +378,16 @@ __device__ double coarse_apply(const BatchArgs& a, const CoarseLds& cl, int p, d
   return cy[0];
 }
 
-// Coarse part of z at row v (problem-relative row `row`), zero on fixed DOFs.
-__device__ __forceinline__ void coarse_term(const BatchArgs& a, const CoarseLds& cl, int64_t v, int64_t row,
+// Coarse part of z at row v (problem-relative row `row`), zero on the DOFs of
+// the fixed mask fx.
+__device__ __forceinline__ void coarse_term(const BatchArgs& a, const CoarseLds& cl, int64_t v, int64_t row, int fx,
                                             float (&zc)[3]) {
   const double* uw = cl.uw + 6 * (row / a.agg_rows);
   const double d0 = a.cdv[3 * v], d1 = a.cdv[3 * v + 1], d2 = a.cdv[3 * v + 2];
   const double t[3] = {uw[0] + uw[4] * d2 - uw[5] * d1, uw[1] + uw[5] * d0 - uw[3] * d2,
                        uw[2] + uw[3] * d1 - uw[4] * d0};
 #pragma unroll
-  for (int q = 0; q < 3; ++q) zc[q] = is_fixed(a.fixed, 3 * v + q) ? 0.f : (float)t[q];
+  for (int q = 0; q < 3; ++q) zc[q] = ((fx >> q) & 1) ? 0.f : (float)t[q];
 }
 
 // ---------------------------------------------------------------------------
@@ -397,8 +401,8 @@ template <int NT, int LANES, bool KLDS, bool KC>
 __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
   // reductions (4 doubles per wave); with KC also the coarse product's partials (NT doubles)
   __shared__ double lds[KC ? NT : 4 * (NT / kWave)];
-  // KLDS: p, r (and A p without KC) of the problem; KC: the coarse scratch after them (declared
-  // as double so that the base is 8-byte aligned for the scratch's FP64 part)
+  // KLDS: p, r, A p and the FP32 solution of the problem; KC: the coarse scratch after them
+  // (declared as double so that the base is 8-byte aligned for the scratch's FP64 part)
   extern __shared__ double dyn_d[];
   float* dyn = reinterpret_cast<float*>(dyn_d);
   constexpr int RS = NT / LANES;  // rows per sweep
@@ -418,13 +422,13 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
     const int64_t v0 = a.vstart[p], v1 = a.vstart[p + 1], wc0 = a.wchunk0[p];
     const int64_t nrow = v1 - v0;
     const int64_t sweeps = (nrow + RS - 1) / RS;
-    // p and r: in LDS (indexed from the problem's first DOF) or in the global
-    // vectors; A p in LDS only without the coarse scratch
+    // p, r, A p and x32: in LDS (indexed from the problem's first DOF) or in
+    // the global vectors
     const int64_t off = KLDS ? 3 * v0 : 0;
-    const int64_t offA = (KLDS && !KC) ? 3 * v0 : 0;
     float* P = KLDS ? dyn : a.p32;
     float* Rv = KLDS ? dyn + 3 * a.max_rows : a.r32;
-    float* AP = (KLDS && !KC) ? dyn + 6 * a.max_rows : a.Ap32;
+    float* AP = KLDS ? dyn + 6 * a.max_rows : a.Ap32;
+    float* X = KLDS ? dyn + 9 * a.max_rows : a.x32;
     // init: r64 = b = -g (free), dx = 0, r32 = b, x32 = 0, z = M^-1 r, p = z
     double acc[2] = {0.0, 0.0};
     for (int64_t s = 0; s < sweeps; ++s) {
@@ -436,12 +440,12 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
           const int64_t k = 3 * v + q;
-          const double b = is_fixed(a.fixed, k) ? 0.0 : -a.g[k];
+          const double b = ((R.fx >> q) & 1) ? 0.0 : -a.g[k];
           a.r64[k] = b;
           a.dx[k] = 0.0;
           r[q] = (float)b;
           Rv[k - off] = r[q];
-          a.x32[k] = 0.f;
+          X[k - off] = 0.f;
           acc[1] += b * b;
         }
         apply_binv3(a, R.c, R.i, r[0], r[1], r[2], z);
@@ -466,7 +470,7 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
           float z[3], zc[3];
           const float* rv = Rv + 3 * v - off;
           apply_binv3(a, R.c, R.i, rv[0], rv[1], rv[2], z);
-          coarse_term(a, cl, v, v - v0, zc);
+          coarse_term(a, cl, v, v - v0, R.fx, zc);
 #pragma unroll
           for (int q = 0; q < 3; ++q) P[3 * v + q - off] = z[q] + zc[q];
         }
@@ -497,8 +501,8 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
 #pragma unroll
           for (int q = 0; q < 3; ++q) {
             const int64_t k = 3 * v + q;
-            const float ap = is_fixed(a.fixed, k) ? 0.f : y[q];
-            AP[k - offA] = ap;
+            const float ap = ((R.fx >> q) & 1) ? 0.f : y[q];
+            AP[k - off] = ap;
             d1[0] += (double)P[k - off] * (double)ap;
           }
         }
@@ -521,8 +525,8 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
 #pragma unroll
           for (int q = 0; q < 3; ++q) {
             const int64_t k = 3 * v + q;
-            a.x32[k] += alpha * P[k - off];
-            r[q] = Rv[k - off] - alpha * AP[k - offA];
+            X[k - off] += alpha * P[k - off];
+            r[q] = Rv[k - off] - alpha * AP[k - off];
             Rv[k - off] = r[q];
             d2[0] += (double)r[q] * (double)r[q];
           }
@@ -556,8 +560,8 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
               const int64_t k = 3 * v + q;
-              a.dx[k] += (double)a.x32[k];
-              a.x32[k] = 0.f;
+              a.dx[k] += (double)X[k - off];
+              X[k - off] = 0.f;
             }
           }
         }
@@ -574,7 +578,7 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
               const int64_t k = 3 * v + q;
-              const double r = is_fixed(a.fixed, k) ? 0.0 : -a.g[k] - y[q];
+              const double r = ((R.fx >> q) & 1) ? 0.0 : -a.g[k] - y[q];
               a.r64[k] = r;
               rf[q] = (float)r;
               Rv[k - off] = rf[q];
@@ -614,7 +618,7 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
             float z[3], zc[3];
             const float* rv = Rv + 3 * v - off;
             apply_binv3(a, R.c, R.i, rv[0], rv[1], rv[2], z);
-            coarse_term(a, cl, v, v - v0, zc);
+            coarse_term(a, cl, v, v - v0, R.fx, zc);
 #pragma unroll
             for (int q = 0; q < 3; ++q) P[3 * v + q - off] = z[q] + zc[q] + beta * P[3 * v + q - off];
           }
@@ -658,8 +662,8 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
         if (v < v1 && j == 0)
 #pragma unroll
           for (int q = 0; q < 3; ++q) {
-            a.dx[3 * v + q] += (double)a.x32[3 * v + q];
-            a.x32[3 * v + q] = 0.f;
+            a.dx[3 * v + q] += (double)X[3 * v + q - off];
+            X[3 * v + q - off] = 0.f;
           }
       }
     }
@@ -919,10 +923,9 @@ BatchArgs sk_batch::args() const {
 
 namespace {
 
-// Whether the problems' p, r and Ap (3 max_rows floats each) fit the LDS of one workgroup.
-// Dynamic LDS of the PCG kernel: the LDS-resident vectors (p, r and, without
-// the coarse scratch, A p) followed by the coarse scratch.
-int pcg_vec_floats(const sk_batch* B, int lds) { return lds ? (B->coarse() ? 6 : 9) * B->max_rows : 0; }
+// Dynamic LDS of the PCG kernel: the LDS-resident vectors (p, r, A p and the
+// FP32 solution, 3 max_rows floats each) followed by the coarse scratch.
+int pcg_vec_floats(const sk_batch* B, int lds) { return lds ? 12 * B->max_rows : 0; }
 size_t pcg_shm_bytes(const sk_batch* B, int lds) {
   size_t b = (size_t)pcg_vec_floats(B, lds) * sizeof(float);
   if (B->coarse())
@@ -931,9 +934,9 @@ size_t pcg_shm_bytes(const sk_batch* B, int lds) {
   return b;
 }
 size_t pcg_lds_bytes(const sk_batch* B) { return pcg_shm_bytes(B, 1); }
-// 54 KiB keeps the total, with the kernel's static scratch (up to 8 KiB at 1024
-// threads), under 64 KiB.
-bool pcg_lds_fits(const sk_batch* B) { return B->max_rows > 0 && pcg_lds_bytes(B) <= 54 * 1024; }
+// 150 KiB keeps the total, with the kernel's static scratch (up to 8.5 KiB at
+// 1024 threads), within the 160 KiB a workgroup may use.
+bool pcg_lds_fits(const sk_batch* B) { return B->max_rows > 0 && pcg_lds_bytes(B) <= 150 * 1024; }
 
 int launch_pcg(const sk_batch* B, const BatchArgs& a, int block, int lanes, int lds) {
   const int g = B->grid;
@@ -1273,6 +1276,8 @@ int sk_batch_create(sk_mesh* mesh, int nprob, const int64_t* vstart, const int64
       for (int32_t b = bp[v]; b < bp[v + 1]; ++b)
         SK_REQUIRE(bc[b] >= 0 && bc[b] < mesh->nv && pid[bc[b]] == pid[v],
                    "sk_batch_create: the mesh couples vertices of different problems");
+    std::vector<uint8_t> fx(3 * mesh->nv, 0);  // the fixed-DOF flags, packed into wlane
+    if (fixed) SK_HIP_CHECK(hipMemcpy(fx.data(), fixed, fx.size(), hipMemcpyDeviceToHost));
     int64_t nslot = 0;
     for (int q = 0; q < nprob; ++q) {
       wchunk0[q] = (int32_t)(woff.size() - 1);
@@ -1284,7 +1289,7 @@ int sk_batch_create(sk_mesh* mesh, int nprob, const int64_t* vstart, const int64
         std::iota(lane.begin(), lane.end(), 0);
         std::stable_sort(lane.begin(), lane.end(), [&](int x, int y) { return nb(x) > nb(y); });
         const int32_t width = m ? nb(lane[0]) : 0;
-        SK_REQUIRE(width < (1 << 23), "sk_batch_create: row with too many blocks");
+        SK_REQUIRE(width < (1 << 20), "sk_batch_create: row with too many blocks");
         const int64_t base = nslot;
         nslot += (int64_t)width * kWave;
         SK_REQUIRE(nslot <= INT32_MAX, "sk_batch_create: matrix too large for 32-bit slot indices");
@@ -1294,7 +1299,8 @@ int sk_batch_create(sk_mesh* mesh, int nprob, const int64_t* vstart, const int64
         for (int i = 0; i < m; ++i) {
           const int o = lane[i];
           const int64_t v = r0 + o;
-          wlane[c * kWave + i] = (nb(o) << 8) | o;
+          const int mask = (fx[3 * v] ? 1 : 0) | (fx[3 * v + 1] ? 2 : 0) | (fx[3 * v + 2] ? 4 : 0);
+          wlane[c * kWave + i] = (nb(o) << 11) | (mask << 8) | o;
           wrow[v] = (int32_t)(c * kWave + i);
           for (int32_t k = 0; k < nb(o); ++k) {
             const int64_t sl = base + (int64_t)k * kWave + i;
