@@ -170,12 +170,13 @@ struct RowAt {
   int fx;   // fixed-DOF mask of the row (bit q: DOF 3 v + q)
   bool ok;  // rel is within the problem
 };
-__device__ __forceinline__ RowAt row_at(const BatchArgs& a, int64_t v0, int64_t nrow, int64_t wc0, int64_t rel) {
+// wl: the problem's lane words (wlane from its first chunk on).
+__device__ __forceinline__ RowAt row_at(const int32_t* wl, int64_t v0, int64_t nrow, int64_t wc0, int64_t rel) {
   RowAt r;
   r.ok = rel < nrow;
   r.c = wc0 + rel / kWave;
   r.i = (int)(rel % kWave);
-  const int32_t w = r.ok ? a.wlane[r.c * kWave + r.i] : 0;
+  const int32_t w = r.ok ? wl[rel] : 0;
   r.v = v0 + (rel - r.i) + (w & 0xff);
   r.fx = (w >> 8) & 7;
   r.nb = w >> 11;
@@ -302,8 +303,9 @@ __device__ __forceinline__ void restrict_wave(const BatchArgs& a, const CoarseLd
   }
 }
 
-__device__ __forceinline__ void restrict_terms(const BatchArgs& a, int64_t v, const float (&r)[3], float (&t)[6]) {
-  const float d0 = a.cdv[3 * v], d1 = a.cdv[3 * v + 1], d2 = a.cdv[3 * v + 2];
+// d: the vertex's offset from its aggregate's centroid (3 floats).
+__device__ __forceinline__ void restrict_terms(const float* d, const float (&r)[3], float (&t)[6]) {
+  const float d0 = d[0], d1 = d[1], d2 = d[2];
   t[0] = r[0];
   t[1] = r[1];
   t[2] = r[2];
@@ -378,16 +380,25 @@ __device__ double coarse_apply(const BatchArgs& a, const CoarseLds& cl, int p, d
   return cy[0];
 }
 
-// Coarse part of z at row v (problem-relative row `row`), zero on the DOFs of
-// the fixed mask fx.
-__device__ __forceinline__ void coarse_term(const BatchArgs& a, const CoarseLds& cl, int64_t v, int64_t row, int fx,
-                                            float (&zc)[3]) {
+// Coarse part of z at problem-relative row `row` with centroid offset d (3
+// floats), zero on the DOFs of the fixed mask fx.
+__device__ __forceinline__ void coarse_term(const BatchArgs& a, const CoarseLds& cl, const float* d, int64_t row,
+                                            int fx, float (&zc)[3]) {
   const double* uw = cl.uw + 6 * (row / a.agg_rows);
-  const double d0 = a.cdv[3 * v], d1 = a.cdv[3 * v + 1], d2 = a.cdv[3 * v + 2];
+  const double d0 = d[0], d1 = d[1], d2 = d[2];
   const double t[3] = {uw[0] + uw[4] * d2 - uw[5] * d1, uw[1] + uw[5] * d0 - uw[3] * d2,
                        uw[2] + uw[3] * d1 - uw[4] * d0};
 #pragma unroll
   for (int q = 0; q < 3; ++q) zc[q] = ((fx >> q) & 1) ? 0.f : (float)t[q];
+}
+
+// Dynamic-LDS floats of the PCG kernel's LDS-resident per-problem data, before
+// the coarse scratch: p, r, A p, x32 and z (3 max_rows floats each), the lane
+// words (kWave per chunk) and, with the coarse space, the centroid offsets
+// (3 max_rows floats).
+__host__ __device__ inline int pcg_lane_words(int max_rows) { return kWave * ((max_rows + kWave - 1) / kWave); }
+__host__ __device__ inline int pcg_lds_vec_floats(int max_rows, bool kc) {
+  return 15 * max_rows + pcg_lane_words(max_rows) + (kc ? 3 * max_rows : 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -401,8 +412,9 @@ template <int NT, int LANES, bool KLDS, bool KC>
 __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
   // reductions (4 doubles per wave); with KC also the coarse product's partials (NT doubles)
   __shared__ double lds[KC ? NT : 4 * (NT / kWave)];
-  // KLDS: p, r, A p and the FP32 solution of the problem; KC: the coarse scratch after them
-  // (declared as double so that the base is 8-byte aligned for the scratch's FP64 part)
+  // KLDS: the problem's vectors, lane words and centroid offsets (pcg_lds_vec_floats); KC: the
+  // coarse scratch after them (declared as double so that the base is 8-byte aligned for the
+  // scratch's FP64 part)
   extern __shared__ double dyn_d[];
   float* dyn = reinterpret_cast<float*>(dyn_d);
   constexpr int RS = NT / LANES;  // rows per sweep
@@ -429,10 +441,40 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
     float* Rv = KLDS ? dyn + 3 * a.max_rows : a.r32;
     float* AP = KLDS ? dyn + 6 * a.max_rows : a.Ap32;
     float* X = KLDS ? dyn + 9 * a.max_rows : a.x32;
+    // KLDS: z = B^-1 r of the current residual kept from the sweep that forms
+    // it, and the problem's lane words and centroid offsets staged once
+    float* Z = KLDS ? dyn + 12 * a.max_rows : nullptr;
+    const int32_t* wl = a.wlane + wc0 * kWave;
+    float* CD = (KLDS && KC) ? dyn + 15 * a.max_rows + pcg_lane_words(a.max_rows) : nullptr;
+    if (KLDS) {
+      int32_t* wls = reinterpret_cast<int32_t*>(dyn + 15 * a.max_rows);
+      for (int64_t t = threadIdx.x; t < kWave * ((nrow + kWave - 1) / kWave); t += NT) wls[t] = wl[t];
+      if (KC)
+        for (int64_t t = threadIdx.x; t < 3 * nrow; t += NT) CD[t] = a.cdv[3 * v0 + t];
+      wl = wls;
+      __syncthreads();
+    }
+    auto cdv_at = [&](int64_t v) -> const float* { return (KLDS && KC) ? CD + 3 * (v - v0) : a.cdv + 3 * v; };
+    // z of row v: kept (KLDS) or B^-1 r
+    auto z_of = [&](const RowAt& R, int64_t v, float (&z)[3]) {
+      if (KLDS) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) z[q] = Z[3 * v + q - off];
+      } else {
+        const float* rv = Rv + 3 * v - off;
+        apply_binv3(a, R.c, R.i, rv[0], rv[1], rv[2], z);
+      }
+    };
+    auto keep_z = [&](int64_t v, const float (&z)[3]) {
+      if (KLDS) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) Z[3 * v + q - off] = z[q];
+      }
+    };
     // init: r64 = b = -g (free), dx = 0, r32 = b, x32 = 0, z = M^-1 r, p = z
     double acc[2] = {0.0, 0.0};
     for (int64_t s = 0; s < sweeps; ++s) {
-      const RowAt R = row_at(a, v0, nrow, wc0, s * RS + slot);
+      const RowAt R = row_at(wl, v0, nrow, wc0, s * RS + slot);
       const int64_t v = R.v;
       float tr[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
       if (R.ok && j == 0) {
@@ -449,12 +491,13 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
           acc[1] += b * b;
         }
         apply_binv3(a, R.c, R.i, r[0], r[1], r[2], z);
+        keep_z(v, z);
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
           if (!KC) P[3 * v + q - off] = z[q];
           acc[0] += (double)r[q] * (double)z[q];
         }
-        if (KC) restrict_terms(a, v, r, tr);
+        if (KC) restrict_terms(cdv_at(v), r, tr);
       }
       if (KC) restrict_wave<LANES>(a, cl, s * RS + wrow, nagg, tr);
     }
@@ -464,13 +507,12 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
     if (KC) {
       rz += coarse_apply<NT, LANES>(a, cl, p, lds);
       for (int64_t s = 0; s < sweeps; ++s) {
-        const RowAt R = row_at(a, v0, nrow, wc0, s * RS + slot);
+        const RowAt R = row_at(wl, v0, nrow, wc0, s * RS + slot);
         const int64_t v = R.v;
         if (R.ok && j == 0) {
           float z[3], zc[3];
-          const float* rv = Rv + 3 * v - off;
-          apply_binv3(a, R.c, R.i, rv[0], rv[1], rv[2], z);
-          coarse_term(a, cl, v, v - v0, R.fx, zc);
+          z_of(R, v, z);
+          coarse_term(a, cl, cdv_at(v), v - v0, R.fx, zc);
 #pragma unroll
           for (int q = 0; q < 3; ++q) P[3 * v + q - off] = z[q] + zc[q];
         }
@@ -493,7 +535,7 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
       }
       double d1[1] = {0.0};
       for (int64_t s = 0; s < sweeps; ++s) {
-        const RowAt R = row_at(a, v0, nrow, wc0, s * RS + slot);
+        const RowAt R = row_at(wl, v0, nrow, wc0, s * RS + slot);
         const int64_t v = R.v;
         float y[3];
         row_product<LANES, float>(a, R, P, off, j, y);
@@ -517,7 +559,7 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
       // x += alpha p, r -= alpha A p, r.r; with KC also r.(B^-1 r) and the restriction of r
       double d2[2] = {0.0, 0.0};
       for (int64_t s = 0; s < sweeps; ++s) {
-        const RowAt R = row_at(a, v0, nrow, wc0, s * RS + slot);
+        const RowAt R = row_at(wl, v0, nrow, wc0, s * RS + slot);
         const int64_t v = R.v;
         float tr[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         if (R.ok && j == 0) {
@@ -533,9 +575,10 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
           if (KC) {
             float z[3];
             apply_binv3(a, R.c, R.i, r[0], r[1], r[2], z);
+            keep_z(v, z);
 #pragma unroll
             for (int q = 0; q < 3; ++q) d2[1] += (double)r[q] * (double)z[q];
-            restrict_terms(a, v, r, tr);
+            restrict_terms(cdv_at(v), r, tr);
           }
         }
         if (KC) restrict_wave<LANES>(a, cl, s * RS + wrow, nagg, tr);
@@ -568,7 +611,7 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
         __syncthreads();
         double d3[2] = {0.0, 0.0};
         for (int64_t s = 0; s < sweeps; ++s) {
-          const RowAt R = row_at(a, v0, nrow, wc0, s * RS + slot);
+          const RowAt R = row_at(wl, v0, nrow, wc0, s * RS + slot);
           const int64_t v = R.v;
           double y[3];
           float tr[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
@@ -587,9 +630,10 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
             if (KC) {
               float z[3];
               apply_binv3(a, R.c, R.i, rf[0], rf[1], rf[2], z);
+              keep_z(v, z);
 #pragma unroll
               for (int q = 0; q < 3; ++q) d3[1] += (double)rf[q] * (double)z[q];
-              restrict_terms(a, v, rf, tr);
+              restrict_terms(cdv_at(v), rf, tr);
             }
           }
           if (KC) restrict_wave<LANES>(a, cl, s * RS + wrow, nagg, tr);
@@ -612,13 +656,12 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
         const float beta = (float)(rz_new / rz);
         rz = rz_new;
         for (int64_t s = 0; s < sweeps; ++s) {
-          const RowAt R = row_at(a, v0, nrow, wc0, s * RS + slot);
+          const RowAt R = row_at(wl, v0, nrow, wc0, s * RS + slot);
           const int64_t v = R.v;
           if (R.ok && j == 0) {
             float z[3], zc[3];
-            const float* rv = Rv + 3 * v - off;
-            apply_binv3(a, R.c, R.i, rv[0], rv[1], rv[2], z);
-            coarse_term(a, cl, v, v - v0, R.fx, zc);
+            z_of(R, v, z);
+            coarse_term(a, cl, cdv_at(v), v - v0, R.fx, zc);
 #pragma unroll
             for (int q = 0; q < 3; ++q) P[3 * v + q - off] = z[q] + zc[q] + beta * P[3 * v + q - off];
           }
@@ -629,12 +672,13 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
       // z = B^-1 r, r.z, then p = z + beta p
       double d4[1] = {0.0};
       for (int64_t s = 0; s < sweeps; ++s) {
-        const RowAt R = row_at(a, v0, nrow, wc0, s * RS + slot);
+        const RowAt R = row_at(wl, v0, nrow, wc0, s * RS + slot);
         const int64_t v = R.v;
         if (R.ok && j == 0) {
           float z[3];
           const float* rv = Rv + 3 * v - off;
           apply_binv3(a, R.c, R.i, rv[0], rv[1], rv[2], z);
+          keep_z(v, z);
 #pragma unroll
           for (int q = 0; q < 3; ++q) d4[0] += (double)rv[q] * (double)z[q];
         }
@@ -643,12 +687,11 @@ __global__ __launch_bounds__(NT) void batch_pcg_kernel(BatchArgs a) {
       const float beta = (float)(d4[0] / rz);
       rz = d4[0];
       for (int64_t s = 0; s < sweeps; ++s) {
-        const RowAt R = row_at(a, v0, nrow, wc0, s * RS + slot);
+        const RowAt R = row_at(wl, v0, nrow, wc0, s * RS + slot);
         const int64_t v = R.v;
         if (R.ok && j == 0) {
           float z[3];
-          const float* rv = Rv + 3 * v - off;
-          apply_binv3(a, R.c, R.i, rv[0], rv[1], rv[2], z);
+          z_of(R, v, z);
 #pragma unroll
           for (int q = 0; q < 3; ++q) P[3 * v + q - off] = z[q] + beta * P[3 * v + q - off];
         }
@@ -923,9 +966,9 @@ BatchArgs sk_batch::args() const {
 
 namespace {
 
-// Dynamic LDS of the PCG kernel: the LDS-resident vectors (p, r, A p and the
-// FP32 solution, 3 max_rows floats each) followed by the coarse scratch.
-int pcg_vec_floats(const sk_batch* B, int lds) { return lds ? 12 * B->max_rows : 0; }
+// Dynamic LDS of the PCG kernel: the LDS-resident per-problem data
+// (pcg_lds_vec_floats) followed by the coarse scratch.
+int pcg_vec_floats(const sk_batch* B, int lds) { return lds ? pcg_lds_vec_floats(B->max_rows, B->coarse()) : 0; }
 size_t pcg_shm_bytes(const sk_batch* B, int lds) {
   size_t b = (size_t)pcg_vec_floats(B, lds) * sizeof(float);
   if (B->coarse())
