@@ -337,12 +337,13 @@ int sk_batch_set_shape(sk_batch* batch, int block, int lanes, int lds);
  * the free DOFs, orthonormalised) of each run of agg_rows consecutive
  * vertices of the problem (a multiple of 64, at most 512). E = W^T A W is
  * formed from the system assembled at the start of the next sk_batch_step
- * (every_step = 0: only then; 1: at the start of every step, which keeps it
- * matched to large rotations) and inverted per problem on the device. X: host
+ * (period = 0: only then; k >= 1: again at the start of every k-th step after
+ * it, 1 keeping it matched to large rotations) and inverted per problem on the
+ * device. X: host
  * rest positions of the union mesh (nv x 3). agg_rows <= 0 returns to
  * block-Jacobi. Consecutive vertices should be spatially close (as the
  * cantilever generator numbers them). */
-int sk_batch_set_coarse(sk_batch* batch, const double* X, int agg_rows, int every_step);
+int sk_batch_set_coarse(sk_batch* batch, const double* X, int agg_rows, int period);
 /* Times the per-problem PCG kernel on the first Newton system of a step from
  * (x_curr, x_prev, x_bc) for every (lds, block, lanes) in {0, 1} x {128, 256,
  * 512, 1024} x {1, 4} (reps launches each, after one warm-up; lds = 1 only

@@ -39,12 +39,12 @@ class BatchSim:
     def __init__(self, meshes, mu, lam, rho=1000.0, h=1.0 / 60.0, material="snh", fixed_vertices=None, gravity=0.0,
                  psd=True, newton_rtol=1e-6, newton_max_iter=100, pcg_rtol=1e-8, pcg_max_iter=100000,
                  inner_rtol=1e-2, ls_alpha=0.01, ls_beta=0.5, ls_max_iter=100, ls_threshold=1e-12, block=0, lanes=0,
-                 lds=0, use_graph=True, coarse_rows=0, coarse_every_step=True):
+                 lds=0, use_graph=True, coarse_rows=0, coarse_period=3):
         """meshes: list of (X (n,3), T (t,4)); mu, lam: per problem (scalar each) or per union element;
         fixed_vertices: list (per problem, local vertex ids) or None.
         coarse_rows > 0: two-level per-problem PCG with rigid-motion aggregates of that many
-        consecutive vertices (a multiple of 64); 0: block-Jacobi. coarse_every_step: re-form the
-        coarse operators at the start of every step (else once, at the first step)."""
+        consecutive vertices (a multiple of 64); 0: block-Jacobi. coarse_period: re-form the
+        coarse operators at the start of every coarse_period-th step (0: once, at the first step)."""
         require_gpu()
         P = len(meshes)
         Xs = [np.asarray(X, np.float64) for X, _ in meshes]
@@ -116,7 +116,7 @@ class BatchSim:
         if self.coarse_rows > 0:
             Xc = np.ascontiguousarray(X, dtype=np.float64)
             _lib.call("sk_batch_set_coarse", self._h, Xc.ctypes.data_as(ctypes.c_void_p), self.coarse_rows,
-                      1 if coarse_every_step else 0)
+                      int(coarse_period))
 
     def problem_slice(self, p):
         return slice(3 * int(self.vstart[p]), 3 * int(self.vstart[p + 1]))

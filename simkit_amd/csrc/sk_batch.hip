@@ -891,7 +891,8 @@ struct sk_batch {
   int max_rows = 0;
   // two-level preconditioner (agg_rows > 0 when enabled)
   int agg_rows = 0, n_agg = 0, max_agg = 0, max_nc = 0;
-  bool coarse_built = false, coarse_every_step = false;
+  bool coarse_built = false;
+  int coarse_period = 0, steps_since_coarse = 0;  // re-form every coarse_period steps (0: once)
   DevBuf<float> cdv, ctm, einv;
   DevBuf<int32_t> cagg0, ccol, cagg_prob, cadj_ptr, cadj, cnq;
   DevBuf<int64_t> einv_off;
@@ -1437,7 +1438,11 @@ int sk_batch_step(sk_batch* B, const double* x_curr, const double* x_prev, const
   SK_HIP_CHECK(hipStreamWaitEvent(B->st, B->ev_in, 0));
   const int block = batch_block(B), lanes = batch_lanes(B), lds = batch_lds(B);
   SK_TRY(start_step(B, x_curr, x_prev, x_bc));
-  if (B->coarse() && (!B->coarse_built || B->coarse_every_step)) SK_TRY(build_coarse(B));
+  if (B->coarse() && (!B->coarse_built || (B->coarse_period > 0 && B->steps_since_coarse >= B->coarse_period))) {
+    SK_TRY(build_coarse(B));
+    B->steps_since_coarse = 0;
+  }
+  ++B->steps_since_coarse;
   int n_active = 0, r = 0;
   SK_TRY(read_active(B, &n_active));
   if (B->prm.use_graph) SK_TRY(ensure_graph(B, block, lanes, lds));
@@ -1482,8 +1487,9 @@ int sk_batch_step(sk_batch* B, const double* x_curr, const double* x_prev, const
   return SK_OK;
 }
 
-int sk_batch_set_coarse(sk_batch* B, const double* X, int agg_rows, int every_step) {
+int sk_batch_set_coarse(sk_batch* B, const double* X, int agg_rows, int period) {
   SK_REQUIRE(B, "sk_batch_set_coarse: null batch");
+  SK_REQUIRE(period >= 0, "sk_batch_set_coarse: period must be >= 0");
   if (B->exec) (void)hipGraphExecDestroy(B->exec);
   if (B->graph) (void)hipGraphDestroy(B->graph);
   B->exec = nullptr;
@@ -1557,7 +1563,7 @@ int sk_batch_set_coarse(sk_batch* B, const double* X, int agg_rows, int every_st
     adj_ptr[g + 1] = (int32_t)adj.size();
   }
   B->agg_rows = agg_rows;
-  B->coarse_every_step = every_step != 0;
+  B->coarse_period = period;
   B->n_agg = n_agg;
   B->max_agg = max_agg;
   B->max_nc = max_nc;
