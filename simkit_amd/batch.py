@@ -44,7 +44,10 @@ class BatchSim:
         fixed_vertices: list (per problem, local vertex ids) or None.
         coarse_rows > 0: two-level per-problem PCG with rigid-motion aggregates of that many
         consecutive vertices (a multiple of 64); 0: block-Jacobi. coarse_period: re-form the
-        coarse operators at the start of every coarse_period-th step (0: once, at the first step)."""
+        coarse operators at the start of every coarse_period-th step (0: once, at the first step).
+        The operators carry over between calls: with a period above 1 a second rollout starts from
+        the last ones formed, so its positions agree with the first to the PCG tolerance rather
+        than bitwise (a fresh BatchSim given the same calls reproduces them bitwise)."""
         require_gpu()
         P = len(meshes)
         Xs = [np.asarray(X, np.float64) for X, _ in meshes]

@@ -1564,6 +1564,7 @@ int sk_batch_set_coarse(sk_batch* B, const double* X, int agg_rows, int period) 
   }
   B->agg_rows = agg_rows;
   B->coarse_period = period;
+  B->steps_since_coarse = 0;
   B->n_agg = n_agg;
   B->max_agg = max_agg;
   B->max_nc = max_nc;

@@ -140,13 +140,34 @@ def test_batch_two_level_fewer_iterations_deterministic():
     meshes, fixed, mu, lam = _beams(4, [40, 64, 25])
     plain = _batch(meshes, fixed, mu, lam)
     _, pinfo, _ = plain.rollout(4)
-    two = _batch(meshes, fixed, mu, lam, coarse_rows=64)
-    k1, i1, r1 = two.rollout(4)
-    k2, i2, r2 = two.rollout(4)
+    k1, i1, r1 = _batch(meshes, fixed, mu, lam, coarse_rows=64).rollout(4)
+    k2, i2, r2 = _batch(meshes, fixed, mu, lam, coarse_rows=64).rollout(4)
     assert all(np.array_equal(k1[n], k2[n]) for n in k1) and i1 == i2 and r1 == r2
     it_plain = sum(i["pcg_iters"] for st in pinfo for i in st)
     it_two = sum(i["pcg_iters"] for st in i1 for i in st)
     assert it_two < 0.5 * it_plain, (it_two, it_plain)
+
+
+def test_batch_two_level_coarse_period():
+    """Coarse operators re-formed once, every step or every third step: the same positions to the
+    PCG tolerance; a second rollout of a period-3 batch starts from the operators left by the first
+    and still agrees; a negative period is rejected."""
+    import ctypes
+    from simkit_amd import _lib
+    meshes, fixed, mu, lam = _beams(6, [30, 12, 21])
+    steps = 7
+    runs = {k: _batch(meshes, fixed, mu, lam, coarse_rows=64, coarse_period=k) for k in (0, 1, 3)}
+    out = {k: b.rollout(steps) for k, b in runs.items()}
+    again = runs[3].rollout(steps)
+    for k, (kept, infos, _) in list(out.items()) + [("3 again", again)]:
+        assert all(i["status"] == 0 for st in infos for i in st), (k, infos)
+        for n in range(1, steps + 1):
+            ref = out[1][0][n]
+            assert np.abs(kept[n] - ref).max() <= TOL * np.abs(ref).max(), (k, n)
+    b = runs[3]
+    X = np.ascontiguousarray(b.X, dtype=np.float64)
+    with pytest.raises(_lib.SimkitAmdError, match="period"):
+        _lib.call("sk_batch_set_coarse", b._h, X.ctypes.data_as(ctypes.c_void_p), 64, -1)
 
 
 def test_batch_two_level_bad_rows():
