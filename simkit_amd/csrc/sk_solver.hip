@@ -73,6 +73,78 @@ __device__ __forceinline__ PcgCtl spmv_prologue(PcgState* st, const double* upar
   return c;
 }
 
+// The same prologue in two halves, so that its loads (the flag, this thread's
+// update partials, r.z of k - 1) are in flight while the caller issues its own
+// first loads: spmv_prologue_issue only loads (every address valid, values
+// used in the second half); spmv_prologue_finish publishes the flag, reduces
+// and decides exactly as spmv_prologue. Partials past kPreParts per thread are
+// read in the second half.
+constexpr int kPreParts = 4;
+struct PrologueLoads {
+  int done;
+  double part[2][kPreParts];  // upart[v gu + threadIdx.x + q kBlock], valid where that index < gu
+  double rzprev;              // st->rz[(k - 1) & 1] (unused for k = 0)
+};
+
+__device__ __forceinline__ PrologueLoads spmv_prologue_issue(const PcgState* st, const double* upart, int gu,
+                                                             int k) {
+  PrologueLoads L;
+  L.done = st->done;
+#pragma unroll
+  for (int v = 0; v < 2; ++v)
+#pragma unroll
+    for (int q = 0; q < kPreParts; ++q) {
+      const int i = (int)threadIdx.x + q * kBlock;
+      L.part[v][q] = upart[v * gu + (i < gu ? i : gu - 1)];
+    }
+  L.rzprev = st->rz[(k + 1) & 1];
+  return L;
+}
+
+__device__ __forceinline__ PcgCtl spmv_prologue_finish(const PrologueLoads& L, PcgState* st, const double* upart,
+                                                       int gu, int k, double rtol2, int restart) {
+  __shared__ int s_done;
+  __shared__ double lds[2 * kBlock / kWave];
+  __shared__ double res[2];
+  PcgCtl c{0.0, false};
+  if (threadIdx.x == 0) s_done = L.done;  // thread 0's read decides for the block
+  __syncthreads();
+  if (s_done != 0) return c;
+  double acc[2];
+#pragma unroll
+  for (int v = 0; v < 2; ++v) {
+    double sum = 0.0;
+#pragma unroll
+    for (int q = 0; q < kPreParts; ++q)
+      if ((int)threadIdx.x + q * kBlock < gu) sum += L.part[v][q];
+    for (int i = (int)threadIdx.x + kPreParts * kBlock; i < gu; i += kBlock) sum += upart[v * gu + i];
+    acc[v] = sum;
+  }
+  block_sum<2>(acc, lds);
+  if (threadIdx.x == 0) {
+    res[0] = acc[0];
+    res[1] = acc[1];
+  }
+  __syncthreads();
+  const double rz = res[0], rr = res[1];
+  const bool fresh = (k == 0) || restart;
+  const double tol2 = fresh ? rtol2 * rr : st->tol2;
+  const bool finite = isfinite(rr) && isfinite(rz);
+  const bool conv = finite && !(rr > tol2);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->rz[k & 1] = rz;
+    st->rr = rr;
+    if (fresh) st->tol2 = tol2;
+    if (k == 0) st->bb = rr;
+    if (!finite) st->done = 3;
+    else if (conv) st->done = 1;
+  }
+  if (!finite || conv) return c;
+  c.beta = (k == 0) ? 0.0 : rz / L.rzprev;
+  c.run = true;
+  return c;
+}
+
 // Shared prologue of the update launches: reduce p.Ap, detect breakdown.
 __device__ __forceinline__ bool update_prologue(PcgState* st, const double* spart, int gs, int k, double& alpha) {
   if (block_reads_done(st)) return false;
@@ -363,26 +435,112 @@ __device__ __forceinline__ void spmv_chunk(const BlockPcgArgs<T>& a, T beta, int
   }
 }
 
+// A lane's first slot (s = j) of a chunk: its block and the z and previous-p
+// entries at its column, loaded before beta is known. Every load is
+// unconditional (slot clamped to the chunk's last, chunk to 0 for a wave
+// without one); has says whether the lane has the slot.
+template <typename T>
+struct FirstSlot {
+  T A[9], z[3], pin[3];
+  int32_t deg;
+  bool has;
+};
+
+template <typename T>
+__device__ __forceinline__ FirstSlot<T> first_slot(const BlockPcgArgs<T>& a, int64_t ch, int i, int j) {
+  FirstSlot<T> f;
+  const int64_t c = ch >= 0 ? ch : 0;
+  f.deg = a.sv.chunk_deg[c];
+  f.has = ch >= 0 && j < f.deg;
+  const int64_t g = a.sv.chunk_off[c] + (int64_t)(j < f.deg ? j : f.deg - 1) * kSellC;
+  const int64_t w = 3 * (int64_t)a.sv.scol[g + i];
+  const T* __restrict__ A = a.vals + g * 9 + i;
+#pragma unroll
+  for (int m = 0; m < 9; ++m) f.A[m] = A[m * kSellC];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    f.z[q] = a.z[w + q];
+    f.pin[q] = a.pin[w + q];
+  }
+  return f;
+}
+
+// spmv_chunk with the lane's first slot already loaded: the same products in
+// the same order (slot j first, then j + kSellLanes, ...).
+template <typename T>
+__device__ __forceinline__ void spmv_chunk_first(const BlockPcgArgs<T>& a, T beta, int64_t ch, int lane,
+                                                 const FirstSlot<T>& f, double& dot) {
+  const int i = lane % kSellC, j = lane / kSellC;
+  T y[3] = {T(0), T(0), T(0)};
+  const T* __restrict__ z = a.z;
+  const T* __restrict__ pin = a.pin;
+  if (f.has) {
+    T pv[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) pv[q] = f.z[q] + beta * f.pin[q];
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+      y[p] += (T)f.A[3 * p] * pv[0] + (T)f.A[3 * p + 1] * pv[1] + (T)f.A[3 * p + 2] * pv[2];
+  }
+  const int64_t off = a.sv.chunk_off[ch];
+  for (int32_t s = j + kSellLanes; s < f.deg; s += kSellLanes) {
+    const int64_t g = off + (int64_t)s * kSellC;
+    const int64_t w = 3 * (int64_t)a.sv.scol[g + i];
+    T pv[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) pv[q] = z[w + q] + beta * pin[w + q];
+    const T* __restrict__ A = a.vals + g * 9 + i;
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+      y[p] += (T)A[(3 * p) * kSellC] * pv[0] + (T)A[(3 * p + 1) * kSellC] * pv[1] +
+              (T)A[(3 * p + 2) * kSellC] * pv[2];
+  }
+#pragma unroll
+  for (int off2 = kSellC; off2 < kWave; off2 <<= 1) {
+#pragma unroll
+    for (int p = 0; p < 3; ++p) y[p] += __shfl_xor(y[p], off2, kWave);
+  }
+  const int32_t row = a.sv.sell_row[ch * kSellC + i];
+  if (row >= 0 && j == 0) {
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+      const int64_t k = 3 * (int64_t)row + p;
+      const T pn = z[k] + beta * pin[k];
+      const T ap = (a.fixed && a.fixed[k]) ? T(0) : y[p];
+      a.pout[k] = pn;
+      a.Ap[k] = ap;
+      dot += (double)pn * (double)ap;
+    }
+  }
+}
+
 // SpMV of PCG iteration k on the SELL matrix: forms p = z + beta p_prev on
 // the fly for every column touched, writes the new p and A p of its rows and
 // a partial of p.Ap. kRange = false: one chunk per wave (the grid covers every
-// chunk); true: each workgroup walks a contiguous range of chunk groups.
+// chunk), the wave's first slots loaded while the prologue's loads are in
+// flight; true: each workgroup walks a contiguous range of chunk groups.
 template <typename T, bool kRange>
 __global__ __launch_bounds__(kBlock) void bpcg_spmv_kernel(BlockPcgArgs<T> a) {
+  const int lane = threadIdx.x % kWave;
+  double dot = 0.0;
+  if (!kRange) {
+    const PrologueLoads L = spmv_prologue_issue(a.st, a.upart, a.gu, a.k);
+    const int64_t ch = sell_chunk(a.sv, threadIdx.x / kWave);
+    const FirstSlot<T> f = first_slot(a, ch, lane % kSellC, lane / kSellC);
+    const PcgCtl c = spmv_prologue_finish(L, a.st, a.upart, a.gu, a.k, a.rtol2, a.restart);
+    if (!c.run) return;
+    if (ch >= 0) spmv_chunk_first(a, (T)c.beta, ch, lane, f, dot);
+    double vals[1] = {dot};
+    store_block_partials<1>(vals, a.spart);
+    return;
+  }
   const PcgCtl c = spmv_prologue(a.st, a.upart, a.gu, a.k, a.rtol2, a.restart);
   if (!c.run) return;
   const T beta = (T)c.beta;
-  const int lane = threadIdx.x % kWave;
-  double dot = 0.0;
-  if (kRange) {
-    const int64_t per = chunks_per_block(a.sv);
-    for (int64_t q = 0; q < per; ++q) {
-      const int64_t ch = block_chunk(a.sv, threadIdx.x / kWave, q, per);
-      if (ch == -2) break;
-      if (ch >= 0) spmv_chunk(a, beta, ch, lane, dot);
-    }
-  } else {
-    const int64_t ch = sell_chunk(a.sv, threadIdx.x / kWave);
+  const int64_t per = chunks_per_block(a.sv);
+  for (int64_t q = 0; q < per; ++q) {
+    const int64_t ch = block_chunk(a.sv, threadIdx.x / kWave, q, per);
+    if (ch == -2) break;
     if (ch >= 0) spmv_chunk(a, beta, ch, lane, dot);
   }
   double vals[1] = {dot};
