@@ -517,31 +517,27 @@ __device__ __forceinline__ void spmv_chunk_first(const BlockPcgArgs<T>& a, T bet
 // SpMV of PCG iteration k on the SELL matrix: forms p = z + beta p_prev on
 // the fly for every column touched, writes the new p and A p of its rows and
 // a partial of p.Ap. kRange = false: one chunk per wave (the grid covers every
-// chunk), the wave's first slots loaded while the prologue's loads are in
-// flight; true: each workgroup walks a contiguous range of chunk groups.
+// chunk); true: each workgroup walks a contiguous range of chunk groups. The
+// first chunk's first slots are loaded while the prologue's loads are in
+// flight.
 template <typename T, bool kRange>
 __global__ __launch_bounds__(kBlock) void bpcg_spmv_kernel(BlockPcgArgs<T> a) {
-  const int lane = threadIdx.x % kWave;
+  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
   double dot = 0.0;
-  if (!kRange) {
-    const PrologueLoads L = spmv_prologue_issue(a.st, a.upart, a.gu, a.k);
-    const int64_t ch = sell_chunk(a.sv, threadIdx.x / kWave);
-    const FirstSlot<T> f = first_slot(a, ch, lane % kSellC, lane / kSellC);
-    const PcgCtl c = spmv_prologue_finish(L, a.st, a.upart, a.gu, a.k, a.rtol2, a.restart);
-    if (!c.run) return;
-    if (ch >= 0) spmv_chunk_first(a, (T)c.beta, ch, lane, f, dot);
-    double vals[1] = {dot};
-    store_block_partials<1>(vals, a.spart);
-    return;
-  }
-  const PcgCtl c = spmv_prologue(a.st, a.upart, a.gu, a.k, a.rtol2, a.restart);
+  const PrologueLoads L = spmv_prologue_issue(a.st, a.upart, a.gu, a.k);
+  const int64_t per = kRange ? chunks_per_block(a.sv) : 1;
+  const int64_t ch0 = kRange ? block_chunk(a.sv, wid, 0, per) : sell_chunk(a.sv, wid);
+  const FirstSlot<T> f = first_slot(a, ch0, lane % kSellC, lane / kSellC);
+  const PcgCtl c = spmv_prologue_finish(L, a.st, a.upart, a.gu, a.k, a.rtol2, a.restart);
   if (!c.run) return;
   const T beta = (T)c.beta;
-  const int64_t per = chunks_per_block(a.sv);
-  for (int64_t q = 0; q < per; ++q) {
-    const int64_t ch = block_chunk(a.sv, threadIdx.x / kWave, q, per);
-    if (ch == -2) break;
-    if (ch >= 0) spmv_chunk(a, beta, ch, lane, dot);
+  if (ch0 >= 0) spmv_chunk_first(a, beta, ch0, lane, f, dot);
+  if (kRange && ch0 != -2) {
+    for (int64_t q = 1; q < per; ++q) {
+      const int64_t ch = block_chunk(a.sv, wid, q, per);
+      if (ch == -2) break;
+      if (ch >= 0) spmv_chunk(a, beta, ch, lane, dot);
+    }
   }
   double vals[1] = {dot};
   store_block_partials<1>(vals, a.spart);
