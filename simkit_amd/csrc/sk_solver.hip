@@ -145,6 +145,66 @@ __device__ __forceinline__ PcgCtl spmv_prologue_finish(const PrologueLoads& L, P
   return c;
 }
 
+// The update prologue in two halves, as for the SpMV: update_prologue_issue
+// loads the flag, this thread's SpMV partials of p.Ap (kPreSParts, clamped
+// indices) and r.z of k; update_prologue_finish publishes the flag, reduces in
+// the order of sum_partials and decides exactly as update_prologue.
+constexpr int kPreSParts = 8;
+struct UpdatePrologueLoads {
+  int done;
+  double part[kPreSParts];  // spart[threadIdx.x + q kBlock], valid where that index < gs
+  double rz;                // st->rz[k & 1]
+};
+
+__device__ __forceinline__ UpdatePrologueLoads update_prologue_issue(const PcgState* st, const double* spart, int gs,
+                                                                     int k) {
+  UpdatePrologueLoads L;
+  L.done = st->done;
+#pragma unroll
+  for (int q = 0; q < kPreSParts; ++q) {
+    const int i = (int)threadIdx.x + q * kBlock;
+    L.part[q] = spart[i < gs ? i : gs - 1];
+  }
+  L.rz = st->rz[k & 1];
+  return L;
+}
+
+__device__ __forceinline__ bool update_prologue_finish(const UpdatePrologueLoads& L, PcgState* st,
+                                                       const double* spart, int gs, int k, double& alpha) {
+  __shared__ int s_done;
+  __shared__ double lds[kBlock / kWave];
+  __shared__ double res;
+  if (threadIdx.x == 0) s_done = L.done;  // thread 0's read decides for the block
+  __syncthreads();
+  if (s_done != 0) return false;
+  double acc[1];
+  {
+    double sum = 0.0;
+#pragma unroll
+    for (int q = 0; q < kPreSParts; ++q)
+      if ((int)threadIdx.x + q * kBlock < gs) sum += L.part[q];
+    for (int i = (int)threadIdx.x + kPreSParts * kBlock; i < gs; i += kBlock) sum += spart[i];
+    acc[0] = sum;
+  }
+  block_sum<1>(acc, lds);
+  if (threadIdx.x == 0) res = acc[0];
+  __syncthreads();
+  const double pAp = res;
+  if (!(pAp > 0.0)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->pAp = pAp;
+      st->done = isfinite(pAp) ? 2 : 3;
+    }
+    return false;
+  }
+  alpha = L.rz / pAp;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->pAp = pAp;
+    st->iter = k + 1;
+  }
+  return true;
+}
+
 // Shared prologue of the update launches: reduce p.Ap, detect breakdown.
 __device__ __forceinline__ bool update_prologue(PcgState* st, const double* spart, int gs, int k, double& alpha) {
   if (block_reads_done(st)) return false;
@@ -588,6 +648,7 @@ __device__ __forceinline__ void apply_update(const BlockPcgArgs<T>& a, int64_t v
 
 template <typename T, bool kAgg>
 __global__ __launch_bounds__(kBlock) void bpcg_update_kernel(BlockPcgArgs<T> a) {
+  const UpdatePrologueLoads L = update_prologue_issue(a.st, a.spart, a.gs, a.k);
   const AggPre pre = agg_prefetch<kAgg>(a);
   // this thread's first vertex and the stride to its next ones (for_vertices' order)
   int64_t v = -1, t = 0, t_end = 0, stride = 0;
@@ -605,7 +666,7 @@ __global__ __launch_bounds__(kBlock) void bpcg_update_kernel(BlockPcgArgs<T> a) 
   UpdateIn<T> u;
   if (v >= 0) load_update(a, v, u);
   double alpha_d;
-  if (!update_prologue(a.st, a.spart, a.gs, a.k, alpha_d)) return;
+  if (!update_prologue_finish(L, a.st, a.spart, a.gs, a.k, alpha_d)) return;
   const T alpha = (T)alpha_d;
   double rz = 0.0, rr = 0.0, raw[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (v >= 0) apply_update<T, kAgg>(a, v, u, alpha, rz, rr, raw);
