@@ -109,6 +109,10 @@ struct sk_mesh {
   // XCD gets the same share of heavy and light rows from one spatial region;
   // -1 pads the lists to the common length lpad (a multiple of 4).
   sk::DevBuf<int32_t> corder;
+  // The same order with each entry's chunk metadata, one 16-byte load per
+  // wave: (chunk, chunk_deg, chunk_off, 0); padding entries (-1, and chunk 0's
+  // slot count and offset, so that addresses formed from them stay valid).
+  sk::DevBuf<int4> cmeta;
   int64_t lpad = 0;
   sk::DevBuf<double> Kst;     // [nt][90] staged element stiffness blocks
   sk::DevBuf<double> fst;     // [nt][12] staged element forces

@@ -586,6 +586,11 @@ static int build_mesh(const int32_t* T, int64_t nv, int64_t nt, sk_mesh* m) {
   const int64_t lpad = (int64_t)((lmax + 3) / 4 * 4);
   std::vector<int32_t> corder(8 * lpad, -1);
   for (int j = 0; j < 8; ++j) std::copy(xl[j].begin(), xl[j].end(), corder.begin() + j * lpad);
+  std::vector<int4> cmeta(8 * lpad);
+  for (size_t e = 0; e < cmeta.size(); ++e) {
+    const int32_t c = corder[e];
+    cmeta[e] = make_int4(c, chunk_deg[c >= 0 ? c : 0], chunk_off[c >= 0 ? c : 0], 0);
+  }
   std::vector<int32_t> scol(nslot, 0), sell_pos(nblk);
   for (int64_t c = 0; c < nchunk; ++c)
     for (int i = 0; i < kSellC; ++i) {
@@ -622,6 +627,7 @@ static int build_mesh(const int32_t* T, int64_t nv, int64_t nt, sk_mesh* m) {
   SK_TRY(m->scol.upload(scol));
   SK_TRY(m->sell_pos.upload(sell_pos));
   SK_TRY(m->corder.upload(corder));
+  SK_TRY(m->cmeta.upload(cmeta));
   m->lpad = lpad;
   m->nchunk = nchunk;
   m->nslot = nslot;

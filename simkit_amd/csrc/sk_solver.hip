@@ -241,6 +241,7 @@ struct SellView {
   const int32_t* scol;
   const int32_t* corder;  // XCD-balanced chunk order, 8 lists of lpad (-1: none)
   int64_t lpad;
+  const int4* cmeta;      // corder's entries with their chunks' slot count and offset
 };
 
 template <typename T>
@@ -434,6 +435,19 @@ __device__ __forceinline__ int64_t block_chunk(const SellView& sv, int wid, int6
   return sv.corder[(blockIdx.x % 8) * sv.lpad + pos];
 }
 
+// The entry (chunk, slot count, offset) of position q of this wave in its XCD's
+// list, as block_chunk / sell_chunk (range = false: q = 0 of the one-chunk-
+// per-wave form) but with one 16-byte load; chunk -2 past the end, -1 padding.
+template <bool kRange>
+__device__ __forceinline__ int4 chunk_entry(const SellView& sv, int wid, int64_t q, int64_t per) {
+  constexpr int kWaves = kBlock / kWave;
+  const int64_t pos = kRange ? ((int64_t)(blockIdx.x / 8) * per + q) * kWaves + wid
+                             : (int64_t)(blockIdx.x / 8) * kWaves + wid;
+  int4 e = sv.cmeta[(blockIdx.x % 8) * sv.lpad + (pos < sv.lpad ? pos : sv.lpad - 1)];
+  if (pos >= sv.lpad) e.x = kRange ? -2 : -1;
+  return e;
+}
+
 // One-chunk-per-wave form (the grid covers 8 * lpad waves); -1: no chunk.
 __device__ __forceinline__ int64_t sell_chunk(const SellView& sv, int wid) {
   constexpr int kWaves = kBlock / kWave;
@@ -497,22 +511,23 @@ __device__ __forceinline__ void spmv_chunk(const BlockPcgArgs<T>& a, T beta, int
 
 // A lane's first slot (s = j) of a chunk: its block and the z and previous-p
 // entries at its column, loaded before beta is known. Every load is
-// unconditional (slot clamped to the chunk's last, chunk to 0 for a wave
-// without one); has says whether the lane has the slot.
+// unconditional (slot clamped to the chunk's last; a padding entry carries
+// chunk 0's slot count and offset); has says whether the lane has the slot.
 template <typename T>
 struct FirstSlot {
   T A[9], z[3], pin[3];
-  int32_t deg;
+  int32_t deg, off;
   bool has;
 };
 
+// e: the chunk's entry (chunk_entry).
 template <typename T>
-__device__ __forceinline__ FirstSlot<T> first_slot(const BlockPcgArgs<T>& a, int64_t ch, int i, int j) {
+__device__ __forceinline__ FirstSlot<T> first_slot(const BlockPcgArgs<T>& a, int4 e, int i, int j) {
   FirstSlot<T> f;
-  const int64_t c = ch >= 0 ? ch : 0;
-  f.deg = a.sv.chunk_deg[c];
-  f.has = ch >= 0 && j < f.deg;
-  const int64_t g = a.sv.chunk_off[c] + (int64_t)(j < f.deg ? j : f.deg - 1) * kSellC;
+  f.deg = e.y;
+  f.off = e.z;
+  f.has = e.x >= 0 && j < f.deg;
+  const int64_t g = f.off + (int64_t)(j < f.deg ? j : f.deg - 1) * kSellC;
   const int64_t w = 3 * (int64_t)a.sv.scol[g + i];
   const T* __restrict__ A = a.vals + g * 9 + i;
 #pragma unroll
@@ -542,7 +557,7 @@ __device__ __forceinline__ void spmv_chunk_first(const BlockPcgArgs<T>& a, T bet
     for (int p = 0; p < 3; ++p)
       y[p] += (T)f.A[3 * p] * pv[0] + (T)f.A[3 * p + 1] * pv[1] + (T)f.A[3 * p + 2] * pv[2];
   }
-  const int64_t off = a.sv.chunk_off[ch];
+  const int64_t off = f.off;
   for (int32_t s = j + kSellLanes; s < f.deg; s += kSellLanes) {
     const int64_t g = off + (int64_t)s * kSellC;
     const int64_t w = 3 * (int64_t)a.sv.scol[g + i];
@@ -586,8 +601,9 @@ __global__ __launch_bounds__(kBlock) void bpcg_spmv_kernel(BlockPcgArgs<T> a) {
   double dot = 0.0;
   const PrologueLoads L = spmv_prologue_issue(a.st, a.upart, a.gu, a.k);
   const int64_t per = kRange ? chunks_per_block(a.sv) : 1;
-  const int64_t ch0 = kRange ? block_chunk(a.sv, wid, 0, per) : sell_chunk(a.sv, wid);
-  const FirstSlot<T> f = first_slot(a, ch0, lane % kSellC, lane / kSellC);
+  const int4 e0 = chunk_entry<kRange>(a.sv, wid, 0, per);
+  const int64_t ch0 = e0.x;
+  const FirstSlot<T> f = first_slot(a, e0, lane % kSellC, lane / kSellC);
   const PcgCtl c = spmv_prologue_finish(L, a.st, a.upart, a.gu, a.k, a.rtol2, a.restart);
   if (!c.run) return;
   const T beta = (T)c.beta;
@@ -1103,7 +1119,8 @@ int spmv_grid(int64_t rows) {
 int sell_grid(const sk_mesh* m) { return (int)(8 * (m->lpad / (kBlock / kWave))); }
 
 SellView sell_view(sk_mesh* m) {
-  return SellView{m->nchunk, m->sell_row.p, m->chunk_deg.p, m->chunk_off.p, m->scol.p, m->corder.p, m->lpad};
+  return SellView{m->nchunk, m->sell_row.p, m->chunk_deg.p, m->chunk_off.p, m->scol.p, m->corder.p, m->lpad,
+                  m->cmeta.p};
 }
 
 // Buffers of one precision's PCG.
