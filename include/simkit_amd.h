@@ -444,11 +444,12 @@ int sk_sim_set_coarse(sk_sim* sim, const double* X, int per_agg);
 int sk_sell_order(const int32_t* T, int64_t nv, int64_t nt, int32_t* order);
 /* Host-only, no counterpart in the reference: the vertices of X (host,
  * nv x 3) grouped by the aggregates sk_sim_set_coarse would form with
- * per_agg, ascending within each aggregate (order[nv]), and the number of
- * aggregates (n_agg, may be NULL). A mesh renumbered in this order gives
- * every aggregate a contiguous vertex range, which the two-level kernels then
- * read with unit stride. */
-int sk_coarse_order(const double* X, int64_t nv, int per_agg, int32_t* order, int* n_agg);
+ * per_agg, ascending within each aggregate (order[nv]), the number of
+ * aggregates (n_agg, may be NULL) and where each aggregate's run starts
+ * (agg_ptr[n_agg + 1], at most nv + 1 entries; may be NULL). A mesh
+ * renumbered in this order gives every aggregate a contiguous vertex range,
+ * which the two-level kernels then read with unit stride. */
+int sk_coarse_order(const double* X, int64_t nv, int per_agg, int32_t* order, int* n_agg, int32_t* agg_ptr);
 /* Aggregates, coarse unknowns, whether E^-1 has been formed, and the time it took (ms). */
 int sk_sim_coarse_info(const sk_sim* sim, int* n_agg, int* nc, int* built, double* build_ms);
 

@@ -207,7 +207,7 @@ SIGNATURES = {
     "sk_sim_spmv_reps": [_P, _INT, _P],
     "sk_sim_hess_reps": [_P, _INT, _P],
     "sk_sim_set_coarse": [_P, _P, _INT],
-    "sk_coarse_order": [_P, _I64, _INT, _P, _P],
+    "sk_coarse_order": [_P, _I64, _INT, _P, _P, _P],
     "sk_sell_order": [_P, _I64, _I64, _P],
     "sk_sim_coarse_info": [_P, _P, _P, _P, _P],
 }

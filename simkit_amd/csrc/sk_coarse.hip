@@ -364,11 +364,13 @@ int coarse_build(CoarseSpace& C, sk_mesh* m, const void* vals, bool fp32, const 
 
 }  // namespace sk
 
-extern "C" int sk_coarse_order(const double* X, int64_t nv, int per_agg, int32_t* order, int* n_agg) {
+extern "C" int sk_coarse_order(const double* X, int64_t nv, int per_agg, int32_t* order, int* n_agg,
+                               int32_t* agg_ptr_out) {
   SK_REQUIRE(order, "sk_coarse_order: null output");
   std::vector<int32_t> vagg, agg_ptr, agg_vert;
   SK_TRY(sk::coarse_bins(X, nv, per_agg, vagg, agg_ptr, agg_vert));
   std::copy(agg_vert.begin(), agg_vert.end(), order);
   if (n_agg) *n_agg = (int)agg_ptr.size() - 1;
+  if (agg_ptr_out) std::copy(agg_ptr.begin(), agg_ptr.end(), agg_ptr_out);
   return SK_OK;
 }

@@ -45,8 +45,9 @@ class BackwardEulerSim:
         raised as needed to keep the coarse space within the library's 4,096 unknowns; the coarse
         operator is formed from the first system solved). Both solve to pcg_rtol.
         renumber: the simulator numbers the vertices internally in the order its kernels read
-        them (two-level: aggregate by aggregate, so that each aggregate is a contiguous range;
-        block-Jacobi: the sliced-ELL row order, so that the SpMV's row stores are contiguous);
+        them (two-level: aggregate by aggregate, so that each aggregate is a contiguous range, and
+        in the sliced-ELL order within an aggregate; block-Jacobi: the sliced-ELL row order, so
+        that the SpMV's row stores are contiguous);
         states passed in and returned keep the caller's vertex order."""
         if preconditioner not in ("bjacobi", "two_level"):
             raise ValueError("preconditioner must be 'bjacobi' or 'two_level'")
@@ -62,25 +63,32 @@ class BackwardEulerSim:
         if preconditioner == "two_level":
             Xc = np.ascontiguousarray(X, dtype=np.float64)
             agg_order = np.empty(self.nv, np.int32)
+            agg_ptr = np.empty(self.nv + 1, np.int32)
             n_agg = ctypes.c_int()
             self.coarse_vertices = max(self.coarse_vertices, int(np.ceil(6 * self.nv / _COARSE_MAX)))
             while True:
                 _lib.call("sk_coarse_order", Xc.ctypes.data_as(ctypes.c_void_p), int(self.nv),
-                          self.coarse_vertices, agg_order.ctypes.data_as(ctypes.c_void_p), ctypes.byref(n_agg))
+                          self.coarse_vertices, agg_order.ctypes.data_as(ctypes.c_void_p), ctypes.byref(n_agg),
+                          agg_ptr.ctypes.data_as(ctypes.c_void_p))
                 if 6 * n_agg.value <= _COARSE_MAX:
                     break
                 self.coarse_vertices = int(np.ceil(self.coarse_vertices * 1.25))
+            agg_ptr = agg_ptr[:n_agg.value + 1]
         coarse_vertices = self.coarse_vertices
         # internal vertex order: order[t] = caller's vertex at internal position t
         self._order = self._pos = None
         if renumber:
             order = np.empty(self.nv, np.int32)
+            Tc = np.ascontiguousarray(T, dtype=np.int32)
+            _lib.call("sk_sell_order", Tc.ctypes.data_as(ctypes.c_void_p), int(self.nv), int(self.nt),
+                      order.ctypes.data_as(ctypes.c_void_p))
             if preconditioner == "two_level":
-                order[:] = agg_order
-            else:
-                Tc = np.ascontiguousarray(T, dtype=np.int32)
-                _lib.call("sk_sell_order", Tc.ctypes.data_as(ctypes.c_void_p), int(self.nv), int(self.nt),
-                          order.ctypes.data_as(ctypes.c_void_p))
+                # aggregate by aggregate, each aggregate's vertices in the sliced-ELL order (descending
+                # block count, ties by index), so that a SELL chunk's rows are mostly consecutive
+                rank = np.empty(self.nv, np.int64)
+                rank[order] = np.arange(self.nv)
+                key = np.repeat(np.arange(len(agg_ptr) - 1, dtype=np.int64), np.diff(agg_ptr)) * self.nv
+                order[:] = agg_order[np.argsort(key + rank[agg_order], kind="stable")]
             if not np.array_equal(order, np.arange(self.nv)):
                 pos = np.empty_like(order)
                 pos[order] = np.arange(self.nv, dtype=np.int32)

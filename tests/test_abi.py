@@ -46,9 +46,11 @@ def _coarse_order(X, per_agg):
     order = np.empty(len(X), np.int32)
     Xc = np.ascontiguousarray(X, dtype=np.float64)
     n_agg = ctypes.c_int()
+    ptr = np.empty(len(X) + 1, np.int32)
     _lib.call("sk_coarse_order", Xc.ctypes.data_as(ctypes.c_void_p), len(X), per_agg,
-              order.ctypes.data_as(ctypes.c_void_p), ctypes.byref(n_agg))
+              order.ctypes.data_as(ctypes.c_void_p), ctypes.byref(n_agg), ptr.ctypes.data_as(ctypes.c_void_p))
     _coarse_order.n_agg = n_agg.value
+    _coarse_order.ptr = ptr[:n_agg.value + 1].copy()
     return order
 
 
@@ -82,6 +84,10 @@ def test_coarse_order_groups_spatial_bins():
         same = np.diff(k) == 0
         assert np.all(np.diff(order)[same] > 0)                   # ascending inside a bin
         assert len(np.unique(key)) == len(np.unique(k)) == _coarse_order.n_agg
+        ptr = _coarse_order.ptr                                   # the runs' starts: one bin each
+        assert ptr[0] == 0 and ptr[-1] == len(X) and np.all(np.diff(ptr) > 0)
+        assert all(len(np.unique(k[ptr[a]:ptr[a + 1]])) == 1 for a in range(len(ptr) - 1))
+        assert np.all(np.diff(k[ptr[:-1]]) > 0)
     # 13^3 vertices / 27 = 81.4 bins -> edge 0.231 -> round(4.33) = 4 per axis
     assert len(np.unique(bins(X12, 27))) == 64
     # 775 vertices / 50 = 15.5 bins in a 0.75 x 0.1 x 0.1 box -> edge 0.0785 -> 10 x 1 x 1
