@@ -299,7 +299,7 @@ typedef struct sk_batch_params {
   double ls_threshold;
   int block;             /* threads per workgroup of the per-problem kernels (128, 256, 512, 1024); 0 = 256 */
   int lanes;             /* lanes per matrix row in the per-problem SpMV (1 or 4); 0 = 1 */
-  int lds;               /* 1: the per-problem PCG keeps p, r and A p in LDS (when the largest problem fits) */
+  int lds;               /* 1: the per-problem PCG keeps the problem's vectors and row data in LDS (when the largest problem fits) */
   int use_graph;         /* 1: one Newton iteration of the batch is captured in a hipGraph and replayed */
 } sk_batch_params;
 
