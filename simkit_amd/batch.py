@@ -211,6 +211,8 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--shape", default=None,
+                    help="block,lanes,lds of the per-problem kernels (e.g. 512,1,1) instead of autotuning")
     ap.add_argument("--compare-serial", type=int, default=0,
                     help="also advance the first K beams one at a time with the single-mesh simulator")
     ap.add_argument("--coarse-rows", type=int, default=0,
@@ -225,7 +227,11 @@ def main(argv=None):
     print(f"batch: {desc['problems']} beams, {desc['vertices']} vertices, {desc['tets']} tets ({t_setup:.1f}s setup)",
           flush=True)
     tuned = None
-    if not a.no_autotune:
+    if a.shape:
+        block, lanes, lds = (int(v) for v in a.shape.split(","))
+        sim._apply_choice(block, lanes, lds)
+        tuned = {"block": block, "lanes": lanes, "lds": lds}
+    elif not a.no_autotune:
         t1 = time.perf_counter()
         tuned = sim.autotune()
         print(f"autotune ({time.perf_counter() - t1:.1f}s): {tuned}", flush=True)
