@@ -30,7 +30,7 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 HESS_KERNEL = {"fp64": "elem_hess_kernel<SNH, 1, double>", "mixed": "elem_hess_kernel<SNH, 1, float>"}
 # dominant kernel of each path (the FP32 SpMV walks chunk ranges with 1024 workgroups)
-SPMV_KERNEL = {"fp64": "bpcg_spmv_kernel<double, false>", "mixed": "bpcg_spmv_kernel<float, true>"}
+SPMV_KERNEL = {"fp64": "bpcg_spmv_kernel<double, false>", "mixed": "bpcg_spmv_kernel<float, false>"}
 METRIC = "BE steps/s, 48^3-cell neo-Hookean (353k DOF), learned+FP32 vs plain FP64"
 
 

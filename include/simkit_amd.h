@@ -211,8 +211,7 @@ typedef struct sk_sim_params {
                             correction is accumulated in FP64 and the FP64 residual b - A x replaces the
                             FP32 one (CG continues with its search direction) */
   int max_refine;        /* SK_PREC_MIXED: residual replacements per Newton solve at most */
-  int spmv_blocks;       /* SpMV launch width cap (workgroups walk chunk ranges); 0 = default (FP64: one
-                            chunk per wave, FP32: 1024) */
+  int spmv_blocks;       /* SpMV launch width cap (workgroups walk chunk ranges); 0 = one chunk per wave */
   int update_blocks;     /* launch width cap of the vertex-wise PCG kernels; 0 = one lane per vertex */
   int pcg_chunk;         /* PCG iterations enqueued between convergence checks (the check of one chunk
                             waits until the next is enqueued); 0 = default (8 block-Jacobi, 4 two-level) */
