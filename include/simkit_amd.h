@@ -214,7 +214,8 @@ typedef struct sk_sim_params {
   int spmv_blocks;       /* SpMV launch width cap (workgroups walk chunk ranges); 0 = one chunk per wave */
   int update_blocks;     /* launch width cap of the vertex-wise PCG kernels; 0 = one lane per vertex */
   int pcg_chunk;         /* PCG iterations enqueued between convergence checks (the check of one chunk
-                            waits until the next is enqueued); 0 = default (8 block-Jacobi, 4 two-level) */
+                            waits until the next is enqueued); 0 = default (8 block-Jacobi; two-level: 4
+                            FP64, 2 mixed) */
 } sk_sim_params;
 
 typedef struct sk_step_info {

@@ -1102,9 +1102,13 @@ int copy_dev(DevBuf<double>& dst, const double* src, size_t n) {
 // PCG iterations enqueued per convergence check: past convergence up to about
 // 1.5 chunks of launches return in their prologue, so short solves (the
 // two-level preconditioner's) use shorter chunks.
+// Defaults measured at 48^3 (PCG time per iteration against 2..16): the
+// two-level launches past convergence cost a full coarse product, so its
+// chunks are shorter, and shortest with the mixed path's short refinement runs.
 int pcg_chunk(const sk_sim* S) {
   if (S->prm.pcg_chunk > 0) return S->prm.pcg_chunk;
-  return S->two_level() ? 4 : 8;
+  if (S->two_level()) return S->mixed() ? 2 : 4;
+  return 8;
 }
 constexpr int kPcgChunk = 16;  // sk_pcg_csr
 
