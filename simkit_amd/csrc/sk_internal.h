@@ -106,8 +106,9 @@ struct sk_mesh {
   // Chunk order of the SpMV kernels, balanced over the 8 XCDs: XCD j (the
   // hardware deals workgroup b to XCD b % 8) processes corder[j*lpad ..
   // (j+1)*lpad), the j-th eighth of every run of equal-degree chunks, so each
-  // XCD gets the same share of heavy and light rows from one spatial region;
-  // -1 pads the lists to the common length lpad (a multiple of 4).
+  // XCD gets the same share of heavy and light rows from one spatial region,
+  // the runs merged by relative position within the list; -1 pads the lists
+  // to the common length lpad (a multiple of 4).
   sk::DevBuf<int32_t> corder;
   // The same order with each entry's chunk metadata, one 16-byte load per
   // wave: (chunk, chunk_deg, chunk_off, 0); padding entries (-1, and chunk 0's

@@ -572,14 +572,28 @@ static int build_mesh(const int32_t* T, int64_t nv, int64_t nt, sk_mesh* m) {
     chunk_off[c + 1] = chunk_off[c] + kSellC * d;
   }
   const int64_t nslot = chunk_off[nchunk];
-  // XCD-balanced chunk order (see sk_mesh::corder)
+  // XCD-balanced chunk order (see sk_mesh::corder). Within an XCD's list the
+  // runs are merged by relative position (the k-th of n chunks of a run's
+  // share sorts at k / n): on a spatially ordered mesh, chunks of different
+  // degree from the same region share columns, and now run at about the same
+  // time on the same XCD, whose L2 then serves their common vector entries.
   std::vector<std::vector<int32_t>> xl(8);
+  std::vector<std::vector<std::pair<double, int32_t>>> xk(8);
   for (int64_t c0 = 0; c0 < nchunk;) {
     int64_t c1 = c0;
     while (c1 < nchunk && chunk_deg[c1] == chunk_deg[c0]) ++c1;
-    for (int j = 0; j < 8; ++j)
-      for (int64_t c = c0 + (c1 - c0) * j / 8; c < c0 + (c1 - c0) * (j + 1) / 8; ++c) xl[j].push_back((int32_t)c);
+    for (int j = 0; j < 8; ++j) {
+      const int64_t a0 = c0 + (c1 - c0) * j / 8, a1 = c0 + (c1 - c0) * (j + 1) / 8;
+      for (int64_t c = a0; c < a1; ++c) xk[j].emplace_back((double)(c - a0) / (double)(a1 - a0), (int32_t)c);
+    }
     c0 = c1;
+  }
+  for (int j = 0; j < 8; ++j) {
+    std::stable_sort(xk[j].begin(), xk[j].end(),
+                     [](const std::pair<double, int32_t>& a, const std::pair<double, int32_t>& b) {
+                       return a.first < b.first;
+                     });
+    for (const auto& e : xk[j]) xl[j].push_back(e.second);
   }
   size_t lmax = 1;
   for (auto& l : xl) lmax = std::max(lmax, l.size());

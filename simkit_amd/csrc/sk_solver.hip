@@ -510,13 +510,14 @@ __device__ __forceinline__ void spmv_chunk(const BlockPcgArgs<T>& a, T beta, int
 }
 
 // A lane's first slot (s = j) of a chunk: its block and the z and previous-p
-// entries at its column, loaded before beta is known. Every load is
-// unconditional (slot clamped to the chunk's last; a padding entry carries
-// chunk 0's slot count and offset); has says whether the lane has the slot.
+// entries at its column, loaded before beta is known, and the column of its
+// next slot (s = j + kSellLanes). Every load is unconditional (slots clamped
+// to the chunk's last; a padding entry carries chunk 0's slot count and
+// offset); has says whether the lane has the slot.
 template <typename T>
 struct FirstSlot {
   T A[9], z[3], pin[3];
-  int32_t deg, off;
+  int32_t deg, off, ncol;
   bool has;
 };
 
@@ -529,6 +530,8 @@ __device__ __forceinline__ FirstSlot<T> first_slot(const BlockPcgArgs<T>& a, int
   f.has = e.x >= 0 && j < f.deg;
   const int64_t g = f.off + (int64_t)(j < f.deg ? j : f.deg - 1) * kSellC;
   const int64_t w = 3 * (int64_t)a.sv.scol[g + i];
+  const int32_t jn = j + kSellLanes;
+  f.ncol = a.sv.scol[f.off + (int64_t)(jn < f.deg ? jn : f.deg - 1) * kSellC + i];
   const T* __restrict__ A = a.vals + g * 9 + i;
 #pragma unroll
   for (int m = 0; m < 9; ++m) f.A[m] = A[m * kSellC];
@@ -541,7 +544,10 @@ __device__ __forceinline__ FirstSlot<T> first_slot(const BlockPcgArgs<T>& a, int
 }
 
 // spmv_chunk with the lane's first slot already loaded: the same products in
-// the same order (slot j first, then j + kSellLanes, ...).
+// the same order (slot j first, then j + kSellLanes, ...). The loop is
+// software-pipelined: each pass issues its slot's vector and block loads
+// together with the next slot's column, so a slot costs one memory round trip
+// instead of two (column, then the entries at that column).
 template <typename T>
 __device__ __forceinline__ void spmv_chunk_first(const BlockPcgArgs<T>& a, T beta, int64_t ch, int lane,
                                                  const FirstSlot<T>& f, double& dot) {
@@ -558,9 +564,12 @@ __device__ __forceinline__ void spmv_chunk_first(const BlockPcgArgs<T>& a, T bet
       y[p] += (T)f.A[3 * p] * pv[0] + (T)f.A[3 * p + 1] * pv[1] + (T)f.A[3 * p + 2] * pv[2];
   }
   const int64_t off = f.off;
+  int32_t col = f.ncol;
   for (int32_t s = j + kSellLanes; s < f.deg; s += kSellLanes) {
     const int64_t g = off + (int64_t)s * kSellC;
-    const int64_t w = 3 * (int64_t)a.sv.scol[g + i];
+    const int64_t w = 3 * (int64_t)col;
+    const int32_t sn = s + kSellLanes;
+    col = a.sv.scol[off + (int64_t)(sn < f.deg ? sn : s) * kSellC + i];
     T pv[3];
 #pragma unroll
     for (int q = 0; q < 3; ++q) pv[q] = z[w + q] + beta * pin[w + q];
@@ -569,6 +578,8 @@ __device__ __forceinline__ void spmv_chunk_first(const BlockPcgArgs<T>& a, T bet
     for (int p = 0; p < 3; ++p)
       y[p] += (T)A[(3 * p) * kSellC] * pv[0] + (T)A[(3 * p + 1) * kSellC] * pv[1] +
               (T)A[(3 * p + 2) * kSellC] * pv[2];
+    // keeps the next column's load in this pass (issued with the others, waited for last)
+    asm volatile("" : "+v"(col));
   }
 #pragma unroll
   for (int off2 = kSellC; off2 < kWave; off2 <<= 1) {
