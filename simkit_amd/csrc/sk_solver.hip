@@ -101,8 +101,10 @@ __device__ __forceinline__ PrologueLoads spmv_prologue_issue(const PcgState* st,
   return L;
 }
 
+// hst (optional): the host's pinned mirror of iter and done, written with st.
 __device__ __forceinline__ PcgCtl spmv_prologue_finish(const PrologueLoads& L, PcgState* st, const double* upart,
-                                                       int gu, int k, double rtol2, int restart) {
+                                                       int gu, int k, double rtol2, int restart,
+                                                       PcgState* hst = nullptr) {
   __shared__ int s_done;
   __shared__ double lds[2 * kBlock / kWave];
   __shared__ double res[2];
@@ -136,8 +138,11 @@ __device__ __forceinline__ PcgCtl spmv_prologue_finish(const PrologueLoads& L, P
     st->rr = rr;
     if (fresh) st->tol2 = tol2;
     if (k == 0) st->bb = rr;
-    if (!finite) st->done = 3;
-    else if (conv) st->done = 1;
+    const int d = !finite ? 3 : conv ? 1 : 0;
+    if (d) {
+      st->done = d;
+      if (hst) hst->done = d;
+    }
   }
   if (!finite || conv) return c;
   c.beta = (k == 0) ? 0.0 : rz / L.rzprev;
@@ -170,7 +175,8 @@ __device__ __forceinline__ UpdatePrologueLoads update_prologue_issue(const PcgSt
 }
 
 __device__ __forceinline__ bool update_prologue_finish(const UpdatePrologueLoads& L, PcgState* st,
-                                                       const double* spart, int gs, int k, double& alpha) {
+                                                       const double* spart, int gs, int k, double& alpha,
+                                                       PcgState* hst = nullptr) {
   __shared__ int s_done;
   __shared__ double lds[kBlock / kWave];
   __shared__ double res;
@@ -194,6 +200,7 @@ __device__ __forceinline__ bool update_prologue_finish(const UpdatePrologueLoads
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       st->pAp = pAp;
       st->done = isfinite(pAp) ? 2 : 3;
+      if (hst) hst->done = st->done;
     }
     return false;
   }
@@ -201,6 +208,7 @@ __device__ __forceinline__ bool update_prologue_finish(const UpdatePrologueLoads
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->pAp = pAp;
     st->iter = k + 1;
+    if (hst) hst->iter = k + 1;
   }
   return true;
 }
@@ -267,6 +275,7 @@ struct BlockPcgArgs {
   int k;          // iteration index of this launch
   int restart;    // 1 on the first launch after a residual replacement
   CoarseView<T> cv;  // two-level preconditioner (cv.n_agg = 0: block-Jacobi only)
+  PcgState* hst;     // pinned host mirror of st's iter and done (block 0 writes both)
 };
 
 template <typename T>
@@ -386,6 +395,10 @@ __global__ __launch_bounds__(kBlock) void bpcg_init_kernel(BlockPcgArgs<T> a) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     a.st->done = 0;
     a.st->iter = 0;
+    if (a.hst) {
+      a.hst->done = 0;
+      a.hst->iter = 0;
+    }
   }
 }
 
@@ -411,7 +424,10 @@ __global__ __launch_bounds__(kBlock) void bpcg_restart_kernel(BlockPcgArgs<T> a)
     if (kAgg) restrict_add(a.cv, v, r, raw);
   });
   finish_vertex_pass<kAgg>(a, rz, rr, raw, pre);
-  if (blockIdx.x == 0 && threadIdx.x == 0) a.st->done = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.st->done = 0;
+    if (a.hst) a.hst->done = 0;
+  }
 }
 
 // The SpMV kernels take their chunks from the XCD-balanced order: workgroup
@@ -615,7 +631,7 @@ __global__ __launch_bounds__(kBlock) void bpcg_spmv_kernel(BlockPcgArgs<T> a) {
   const int4 e0 = chunk_entry<kRange>(a.sv, wid, 0, per);
   const int64_t ch0 = e0.x;
   const FirstSlot<T> f = first_slot(a, e0, lane % kSellC, lane / kSellC);
-  const PcgCtl c = spmv_prologue_finish(L, a.st, a.upart, a.gu, a.k, a.rtol2, a.restart);
+  const PcgCtl c = spmv_prologue_finish(L, a.st, a.upart, a.gu, a.k, a.rtol2, a.restart, a.hst);
   if (!c.run) return;
   const T beta = (T)c.beta;
   if (ch0 >= 0) spmv_chunk_first(a, beta, ch0, lane, f, dot);
@@ -693,7 +709,7 @@ __global__ __launch_bounds__(kBlock) void bpcg_update_kernel(BlockPcgArgs<T> a) 
   UpdateIn<T> u;
   if (v >= 0) load_update(a, v, u);
   double alpha_d;
-  if (!update_prologue_finish(L, a.st, a.spart, a.gs, a.k, alpha_d)) return;
+  if (!update_prologue_finish(L, a.st, a.spart, a.gs, a.k, alpha_d, a.hst)) return;
   const T alpha = (T)alpha_d;
   double rz = 0.0, rr = 0.0, raw[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (v >= 0) apply_update<T, kAgg>(a, v, u, alpha, rz, rr, raw);
@@ -1091,7 +1107,7 @@ struct sk_sim {
   // mixed precision: FP32 matrix, preconditioner and inner-PCG vectors
   DevBuf<float> vals32, binv32, x32, r32, z32, p0_32, p1_32, Ap32;
   DevBuf<PcgState> st;
-  PcgState* st_host = nullptr;  // 2 pinned slots (pipelined chunk checks)
+  PcgState* st_host = nullptr;  // pinned, coherent: the PCG kernels write iter and done through to it
   hipEvent_t chunk_ev[2] = {};
   hipEvent_t ev[6] = {};
   bool have_events = false;
@@ -1170,7 +1186,7 @@ BlockPcgArgs<T> block_pcg_args(sk_sim* S, const PcgBufs<T>& B, const T* b, doubl
   CoarseView<T> cv{};
   if (S->two_level()) cv = S->coarse.view<T>();
   return BlockPcgArgs<T>{m->nv, sell_view(m), B.vals, B.binv, S->fixed.p, b, B.x, B.r, B.z, B.p1, B.p0, B.Ap,
-                         S->st.p, rtol2, part, gu, part + 2 * gu, gs, 0, 0, cv};
+                         S->st.p, rtol2, part, gu, part + 2 * gu, gs, 0, 0, cv, S->st_host};
 }
 
 template <typename T>
@@ -1205,9 +1221,13 @@ int run_block_pcg(sk_sim* S, const PcgBufs<T>& B, const T* b, double rtol, int64
   if (tl) bpcg_coarse_kernel<T><<<a.gu, kBlock, 0, st>>>(a);
   SK_HIP_CHECK(hipGetLastError());
   const int chunk = pcg_chunk(S);
-  // Chunks of `chunk` iterations; the state copied after chunk c is checked
-  // only once chunk c + 1 has been enqueued, so the GPU is not idle while the
-  // host reads it. Launches past convergence exit in their prologue.
+  // Chunks of `chunk` iterations; the state after chunk c is checked only once
+  // chunk c + 1 has been enqueued, so the GPU is not idle while the host reads
+  // it. The kernels write iter and done through to the pinned mirror, so the
+  // check needs no copy in the stream; it may see a later state than chunk
+  // c's end, which only stops sooner. Launches past convergence exit in their
+  // prologue.
+  const volatile PcgState* h = S->st_host;
   int64_t launched = k0;
   int c = 0;
   for (bool stop = false; !stop; ++c) {
@@ -1225,19 +1245,18 @@ int run_block_pcg(sk_sim* S, const PcgBufs<T>& B, const T* b, double rtol, int64
       }
     }
     SK_HIP_CHECK(hipGetLastError());
-    SK_HIP_CHECK(hipMemcpyAsync(S->st_host + (c & 1), S->st.p, sizeof(PcgState), hipMemcpyDeviceToHost, st));
     SK_HIP_CHECK(hipEventRecord(S->chunk_ev[c & 1], st));
     if (c > 0) {
       SK_HIP_CHECK(hipEventSynchronize(S->chunk_ev[(c - 1) & 1]));
-      const PcgState& h = S->st_host[(c - 1) & 1];
-      if (h.done || h.iter >= max_iter) stop = true;
+      const int hd = h->done, hi = h->iter;
+      if (hd || hi >= max_iter) stop = true;
       if (launched > max_iter + 2 * chunk) stop = true;  // device state not advancing
-      if (k0 > 0 && h.iter < k0) stop = true;                 // continuation did not start
+      if (k0 > 0 && hi < k0) stop = true;                     // continuation did not start
     }
   }
   SK_HIP_CHECK(hipEventSynchronize(S->chunk_ev[(c - 1) & 1]));
-  *iters = S->st_host[(c - 1) & 1].iter;
-  *done = S->st_host[(c - 1) & 1].done;
+  *iters = h->iter;
+  *done = h->done;
   return SK_OK;
 }
 
@@ -1413,7 +1432,8 @@ int sk_sim_create(sk_mesh* mesh, const sk_sim_params* p, const double* mu, const
       SK_TRY(S->binv.alloc(9 * (size_t)mesh->nv));
     }
     SK_TRY(S->st.alloc(1));
-    SK_HIP_CHECK(hipHostMalloc((void**)&S->st_host, 2 * sizeof(PcgState), hipHostMallocDefault));
+    SK_HIP_CHECK(hipHostMalloc((void**)&S->st_host, sizeof(PcgState), hipHostMallocCoherent));
+    *S->st_host = PcgState{};
     for (auto& e : S->ev) SK_HIP_CHECK(hipEventCreate(&e));
     for (auto& e : S->chunk_ev) SK_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     S->have_events = true;
