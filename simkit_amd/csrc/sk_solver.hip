@@ -525,14 +525,21 @@ __device__ __forceinline__ void spmv_chunk(const BlockPcgArgs<T>& a, T beta, int
   }
 }
 
-// A lane's first slot (s = j) of a chunk: its block and the z and previous-p
-// entries at its column, loaded before beta is known, and the column of its
-// next slot (s = j + kSellLanes). Every load is unconditional (slots clamped
-// to the chunk's last; a padding entry carries chunk 0's slot count and
-// offset); has says whether the lane has the slot.
+// A lane's first slot (s = j) of a chunk: its block and column, in FP32 also
+// the z and previous-p entries at that column, loaded before beta is known,
+// and the column of its next slot (s = j + kSellLanes). Every load is
+// unconditional (slots clamped to the chunk's last; a padding entry carries
+// chunk 0's slot count and offset); has says whether the lane has the slot.
+// In FP64 the six vector entries would take the kernel past 64 VGPRs, below
+// full residency of its grid (measured: 22.35 us without them against 22.63
+// us with; FP32: 13.85 us with them against 14.3 us without).
+template <typename T>
+constexpr bool kPrefetchVec = sizeof(T) == 4;
+
 template <typename T>
 struct FirstSlot {
   T A[9], z[3], pin[3];
+  int64_t w;
   int32_t deg, off, ncol;
   bool has;
 };
@@ -551,10 +558,13 @@ __device__ __forceinline__ FirstSlot<T> first_slot(const BlockPcgArgs<T>& a, int
   const T* __restrict__ A = a.vals + g * 9 + i;
 #pragma unroll
   for (int m = 0; m < 9; ++m) f.A[m] = A[m * kSellC];
+  f.w = w;
+  if constexpr (kPrefetchVec<T>) {
 #pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    f.z[q] = a.z[w + q];
-    f.pin[q] = a.pin[w + q];
+    for (int q = 0; q < 3; ++q) {
+      f.z[q] = a.z[w + q];
+      f.pin[q] = a.pin[w + q];
+    }
   }
   return f;
 }
@@ -574,7 +584,8 @@ __device__ __forceinline__ void spmv_chunk_first(const BlockPcgArgs<T>& a, T bet
   if (f.has) {
     T pv[3];
 #pragma unroll
-    for (int q = 0; q < 3; ++q) pv[q] = f.z[q] + beta * f.pin[q];
+    for (int q = 0; q < 3; ++q)
+      pv[q] = kPrefetchVec<T> ? f.z[q] + beta * f.pin[q] : z[f.w + q] + beta * pin[f.w + q];
 #pragma unroll
     for (int p = 0; p < 3; ++p)
       y[p] += (T)f.A[3 * p] * pv[0] + (T)f.A[3 * p + 1] * pv[1] + (T)f.A[3 * p + 2] * pv[2];
