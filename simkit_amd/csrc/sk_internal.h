@@ -57,7 +57,16 @@ struct DevBuf {
 // Upper-triangle 3x3-block index of local pair (i <= j) of a tet: 0..9.
 __host__ __device__ constexpr int ublock(int i, int j) { return i * (7 - i) / 2 + j; }
 
-constexpr int kKst = 90;  // staged doubles per element: 10 upper 3x3 blocks of K_e
+// Staged element stiffness: the 10 upper 3x3 blocks of K_e. Block u (ublock)
+// of element e sits at R-unit offset kKstR<R> e + kBlkStride<R> u. FP64
+// blocks are padded to 10 doubles (the padding is never written) so that the
+// gather reads them with five 16-byte loads; FP32 blocks stay at 9 floats
+// (padded to 12 the extra bytes cost more than the wider loads saved).
+template <typename R>
+constexpr int kBlkStride = sizeof(R) == 8 ? 10 : 9;
+template <typename R>
+constexpr int kKstR = 10 * kBlkStride<R>;
+constexpr int kKst = 100;  // staging doubles per element (the larger of the two layouts)
 constexpr int kSellC = 16;       // rows per SELL chunk
 constexpr int kSellLanes = 4;    // lanes per row in the SELL SpMV (kSellC * kSellLanes = wave size)
 constexpr int kFst = 12;  // staged doubles per element: f_e = vol P D

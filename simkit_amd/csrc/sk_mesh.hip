@@ -6,7 +6,8 @@
 // ascending element order. Each Hessian evaluation is then two launches:
 //   1. element pass: one lane per tet computes F, the 9x9 Hessian, its
 //      vol-weighted PSD projection and the 10 upper 3x3 blocks of
-//      K_e = B_e^T H+ B_e, staged per element (kKst doubles);
+//      K_e = B_e^T H+ B_e, staged per element (kKstR values, blocks padded
+//      to 16 bytes);
 //   2. block gather: one lane per CSR block sums its contributions in list
 //      order and writes the 3 row segments of the block. No two lanes write the
 //      same value and the summation order is fixed, so values are
@@ -162,7 +163,7 @@ __global__ __launch_bounds__(kBlock) void elem_hess_kernel(ElemArgs a) {
 #pragma unroll
   for (int k = 0; k < 12; ++k) D[k] = (R)Dd[k];
   // K_e[3i+p][3j+q] = sum_{c,c'} D[c][i] H[3p+c][3q+c'] D[c'][j], upper blocks i <= j.
-  R* out = reinterpret_cast<R*>(a.Kst) + (int64_t)kKst * e;
+  R* out = reinterpret_cast<R*>(a.Kst) + (int64_t)kKstR<R> * e;
 #pragma unroll
   for (int p = 0; p < 3; ++p)
 #pragma unroll
@@ -178,7 +179,7 @@ __global__ __launch_bounds__(kBlock) void elem_hess_kernel(ElemArgs a) {
       for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = i; j < 4; ++j)
-          out[ublock(i, j) * 9 + 3 * p + q] = D[i] * G[0][j] + D[4 + i] * G[1][j] + D[8 + i] * G[2][j];
+          out[ublock(i, j) * kBlkStride<R> + 3 * p + q] = D[i] * G[0][j] + D[4 + i] * G[1][j] + D[8 + i] * G[2][j];
     }
 }
 
@@ -253,6 +254,24 @@ struct GatherHArgs {
   const int32_t* pid;
 };
 
+// A staged block: FP64 in five 16-byte loads (16-byte aligned, padded to 10),
+// FP32 in nine 4-byte loads.
+template <typename R>
+__device__ __forceinline__ void load_staged_block(const R* __restrict__ s, R (&B)[kBlkStride<R>]) {
+  if constexpr (sizeof(R) == 8) {
+    const double2* v = reinterpret_cast<const double2*>(s);
+#pragma unroll
+    for (int t = 0; t < 5; ++t) {
+      const double2 d = v[t];
+      B[2 * t] = d.x;
+      B[2 * t + 1] = d.y;
+    }
+  } else {
+#pragma unroll
+    for (int m = 0; m < 9; ++m) B[m] = s[m];
+  }
+}
+
 // One lane per block: sums the staged contributions in list order (precision
 // R), adds the lumped mass / h^2 on diagonal blocks, writes the block in CSR,
 // BSR or a sliced order and, for diagonal blocks, the inverse of the
@@ -272,17 +291,21 @@ __global__ __launch_bounds__(kBlock) void gather_hess_kernel(GatherHArgs a) {
     const int32_t pk = a.clist[k];
     const int i = (pk >> 2) & 3, j = pk & 3;
     const int64_t e = pk >> 4;
-    if (i <= j) {
-      const R* s = Kst + kKst * e + 9 * ublock(i, j);
-#pragma unroll
-      for (int m = 0; m < 9; ++m) acc[m] += s[m];
-    } else {
-      const R* s = Kst + kKst * e + 9 * ublock(j, i);
-#pragma unroll
-      for (int p = 0; p < 3; ++p)
-#pragma unroll
-        for (int q = 0; q < 3; ++q) acc[3 * p + q] += s[3 * q + p];
-    }
+    // the staged upper block (min(i,j), max(i,j)), used transposed when i > j:
+    // one load path for every lane, the transpose by selects
+    const bool tr = i > j;
+    const R* s = Kst + kKstR<R> * e + kBlkStride<R> * ublock(tr ? j : i, tr ? i : j);
+    R B[kBlkStride<R>];
+    load_staged_block<R>(s, B);
+    acc[0] += B[0];
+    acc[4] += B[4];
+    acc[8] += B[8];
+    acc[1] += tr ? B[3] : B[1];
+    acc[3] += tr ? B[1] : B[3];
+    acc[2] += tr ? B[6] : B[2];
+    acc[6] += tr ? B[2] : B[6];
+    acc[5] += tr ? B[7] : B[5];
+    acc[7] += tr ? B[5] : B[7];
   }
   const int32_t v = a.brow[b], w = a.bcol[b];
   const int32_t b0 = a.bptr[v], deg = a.bptr[v + 1] - b0, r = (int32_t)(b - b0);
