@@ -737,16 +737,17 @@ __global__ __launch_bounds__(kBlock) void bpcg_update_kernel(BlockPcgArgs<T> a) 
 // aggregate I (launched after a kAgg vertex pass of the same grid): y_I =
 // (E^-1 c)_I, the aggregate's rigid motion (u, w) = T_I y_I is added to z on
 // its free DOFs, and c_I . y_I is added to the workgroup's r.z partial, so
-// the next launch's r.z is r . (B^-1 r + W E^-1 W^T r). The stop flag is read
-// at entry and tested after the first barrier; nothing is written once set.
+// the next launch's r.z is r . (B^-1 r + W E^-1 W^T r). No block of this
+// launch writes the stop flag (only the SpMV and update launches before it
+// do), so every thread reads the same final value at entry: a launch past
+// convergence returns before streaming E^-1 and writes nothing.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void bpcg_coarse_kernel(BlockPcgArgs<T> a) {
-  __shared__ int s_done;
   __shared__ double lds[6 * kBlock / kWave];
   __shared__ double s_y[6], s_uw[6], s_cy[6];
   const CoarseView<T>& cv = a.cv;
   const int I = blockIdx.x;
-  if (threadIdx.x == 0) s_done = a.st->done;
+  if (a.st->done) return;
   const int c0 = cv.agg_col[I], k = cv.agg_col[I + 1] - c0;
   const int32_t t0 = cv.agg_ptr[I], t1 = cv.agg_ptr[I + 1];
   double trow[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // thread q < 6: row q of T_I
@@ -806,7 +807,6 @@ __global__ __launch_bounds__(kBlock) void bpcg_coarse_kernel(BlockPcgArgs<T> a) 
     }
   }
   block_sum<6>(y, lds);
-  if (s_done) return;
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int q = 0; q < 6; ++q) s_y[q] = y[q];
