@@ -1211,7 +1211,9 @@ void launch_spmv(const BlockPcgArgs<T>& a, const sk_mesh* m, hipStream_t st) {
 // replacement: B.r holds the replaced residual, iterations continue from
 // index k0 with the kept search direction, and x accumulates the correction
 // from 0. Stops at ||r|| <= rtol ||r(start of this run)||; *iters is the
-// total iteration index reached.
+// total iteration index reached. iters = nullptr: returns once the last
+// launches are enqueued, without waiting for them; the caller reads the
+// result with pcg_result after its next synchronization of the stream.
 template <typename T>
 int run_block_pcg(sk_sim* S, const PcgBufs<T>& B, const T* b, double rtol, int64_t max_iter, hipStream_t st,
                   int64_t* iters, int* done, int64_t k0 = 0) {
@@ -1265,10 +1267,19 @@ int run_block_pcg(sk_sim* S, const PcgBufs<T>& B, const T* b, double rtol, int64
       if (k0 > 0 && hi < k0) stop = true;                     // continuation did not start
     }
   }
+  if (!iters) return SK_OK;
   SK_HIP_CHECK(hipEventSynchronize(S->chunk_ev[(c - 1) & 1]));
   *iters = h->iter;
   *done = h->done;
   return SK_OK;
+}
+
+// The result of a run_block_pcg launched without waiting, once the stream
+// has been synchronized since.
+void pcg_result(const sk_sim* S, int64_t* iters, int* done) {
+  const volatile PcgState* h = S->st_host;
+  *iters = h->iter;
+  *done = h->done;
 }
 
 // Mixed-precision solve of A dx = b (b FP64 in S->g, dx FP64 in S->dx). One
@@ -1299,19 +1310,24 @@ int run_mixed_solve(sk_sim* S, hipStream_t st, int64_t* iters, int* refines, int
   for (int seg = 0; seg < S->prm.max_refine && rr > target && bb > 0.0; ++seg) {
     // The right-hand side may alias the residual r32: the init launch reads
     // each entry before the same thread overwrites it.
+    // The run is not waited for: the refinement launches follow it in the
+    // stream, and its result is read after the residual's readback (on a
+    // breakdown the step fails, so the correction accumulated meanwhile is moot).
     int d = 0;
-    SK_TRY(run_block_pcg<float>(S, B, B.r, S->prm.inner_rtol, S->prm.pcg_max_iter, st, &k, &d, seg == 0 ? 0 : k));
+    SK_TRY(run_block_pcg<float>(S, B, B.r, S->prm.inner_rtol, S->prm.pcg_max_iter, st, nullptr, nullptr,
+                                seg == 0 ? 0 : k));
     ++*refines;
-    if (d == 2 || d == 3) {
-      *iters = k;
-      *done = d;
-      return SK_OK;
-    }
     ir_accumulate_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, B.x, S->dx.p, seg == 0 ? 1 : 0);
     ra.x = S->dx.p;
     ir_residual_kernel<<<gr, kBlock, 0, st>>>(ra);
     finalize_sum_kernel<1><<<1, kBlock, 0, st>>>(ra.partials, gr, rr_dev);
     SK_TRY(read_scalars(m, rr_dev, 1, st));
+    pcg_result(S, &k, &d);
+    if (d == 2 || d == 3) {
+      *iters = k;
+      *done = d;
+      return SK_OK;
+    }
     rr = m->pinned[0];
     if (!std::isfinite(rr)) {
       *iters = k;
@@ -1562,18 +1578,21 @@ int step_core(sk_sim* S, double c_mass, const double* x_guess, double* x_next, s
     axpy_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, -2.0, S->g.p, S->g.p);
     int64_t it = 0;
     int done = 0, refines = 0;
+    // The FP64 solve is not waited for: the slope's launches follow it in the
+    // stream and its result is read after the slope's readback.
     if (mixed) SK_TRY(run_mixed_solve(S, st, &it, &refines, &done));
-    else SK_TRY(run_block_pcg<double>(S, bufs64(S), S->g.p, P.pcg_rtol, P.pcg_max_iter, st, &it, &done));
+    else SK_TRY(run_block_pcg<double>(S, bufs64(S), S->g.p, P.pcg_rtol, P.pcg_max_iter, st, nullptr, nullptr));
     axpy_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, -2.0, S->g.p, S->g.p);
+    if (P.timing) SK_HIP_CHECK(hipEventRecord(S->ev[2], st));
+    dot_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, S->g.p, S->dx.p, m->red.partials.p);
+    finalize_sum_kernel<1><<<1, kBlock, 0, st>>>(m->red.partials.p, grid_for(n3), m->red.result.p);
+    SK_TRY(read_scalars(m, m->red.result.p, 1, st));
+    if (!mixed) pcg_result(S, &it, &done);
     inf.pcg_iters += it;
     inf.refine_steps += refines;
     if (done == 2) inf.status = SK_ERR_PCG_BREAKDOWN;
     if (done == 3) inf.status = SK_ERR_NONFINITE;
     if (inf.status != SK_OK) break;
-    if (P.timing) SK_HIP_CHECK(hipEventRecord(S->ev[2], st));
-    dot_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, S->g.p, S->dx.p, m->red.partials.p);
-    finalize_sum_kernel<1><<<1, kBlock, 0, st>>>(m->red.partials.p, grid_for(n3), m->red.result.p);
-    SK_TRY(read_scalars(m, m->red.result.p, 1, st));
     const double slope = m->pinned[0];
     double t = 1.0;
     bool accepted = false;
