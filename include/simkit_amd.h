@@ -37,7 +37,7 @@ enum sk_status {
   SK_ERR_PCG_BREAKDOWN = -4, /* p^T A p <= 0: system not positive definite */
   SK_ERR_PCG_MAXIT = -5,     /* PCG hit its iteration limit before rtol */
   SK_ERR_NONFINITE = -6,     /* NaN/Inf in a reduction */
-  SK_ERR_INVERTED = -7,      /* reserved: inverted element count reported */
+  SK_ERR_INVERTED = -7,      /* sk_sim_step: more elements with det F <= 0 at x_next than sk_sim_params.max_inverted */
   SK_ERR_NEWTON_MAXIT = -8,  /* Newton hit its iteration limit before the gradient tolerance */
   SK_ERR_UNSUPPORTED = -9,
   SK_ERR_EIG_MAXIT = -10
@@ -216,6 +216,9 @@ typedef struct sk_sim_params {
   int pcg_chunk;         /* PCG iterations enqueued between convergence checks (the check of one chunk
                             waits until the next is enqueued); 0 = default (8 block-Jacobi; two-level: 4
                             FP64, 2 mixed) */
+  int max_inverted;      /* < 0: no limit; otherwise a step whose x_next has more elements with det F <= 0
+                            returns SK_ERR_INVERTED (x_next is still written). The reference's densities
+                            accept inverted elements, so the default is no limit. */
 } sk_sim_params;
 
 typedef struct sk_step_info {
@@ -234,6 +237,9 @@ typedef struct sk_step_info {
   double guess_dE;       /* sk_sim_step_guess: E(x_guess) - E(x~), FP64 */
   double guess_g_norm;   /* sk_sim_step_guess: ||g_free(x_guess)|| when the guess was taken, else 0 */
   int stalled;           /* 1: stopped because the update fell below position resolution (see sk_sim_step) */
+  int inverted_elems;    /* elements with det F <= 0 at x_next */
+  int64_t polar_fp64_elems; /* SK_PREC_MIXED, ARAP/FCR: element Hessians evaluated in FP64 because
+                               det F < 1e-3, summed over the step's Newton iterations */
 } sk_step_info;
 
 /* mu, lam, vol: per element; mass: per DOF (3nv, M = kron(M_lumped, I3));
@@ -450,6 +456,16 @@ int sk_sell_order(const int32_t* T, int64_t nv, int64_t nt, int32_t* order);
  * renumbered in this order gives every aggregate a contiguous vertex range,
  * which the two-level kernels then read with unit stride. */
 int sk_coarse_order(const double* X, int64_t nv, int per_agg, int32_t* order, int* n_agg, int32_t* agg_ptr);
+/* Host-side vertex numbering of the states, no counterpart in the reference:
+ * order_host (nv entries, a permutation) says that the simulator's mesh
+ * vertex t is the caller's vertex order_host[t]. Every state the step entry
+ * points read (x_curr, x_prev, x_prev2, x_prev3, x_bc, x_guess) and x_next are
+ * then in the caller's numbering; the step's first and last kernels permute
+ * them. The mesh, mass, f_ext and fixed given to sk_sim_create are in the
+ * simulator's numbering. NULL restores the identity. Lets a caller keep its
+ * own vertex order while the simulator numbers the mesh in the order its
+ * kernels read it (sk_sell_order, sk_coarse_order). */
+int sk_sim_set_order(sk_sim* sim, const int32_t* order_host);
 /* Aggregates, coarse unknowns, whether E^-1 has been formed, and the time it took (ms). */
 int sk_sim_coarse_info(const sk_sim* sim, int* n_agg, int* nc, int* built, double* build_ms);
 

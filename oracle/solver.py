@@ -20,6 +20,8 @@ two explicit harness choices that the reference leaves to its callers:
   cancelling two ~1e5 sums whose rounding (~1e-11) exceeds the 1e-12 slack.
 """
 
+import time
+
 import numpy as np
 import scipy.sparse as sps
 import scipy.sparse.linalg as spla
@@ -156,6 +158,14 @@ class BEProblem:
         self.pcg_rtol = pcg_rtol
         self.integrator = integrator
         self.c = {"be": 1.0, "bdf2": 9.0 / 4.0}[integrator]
+        # optional per-phase wall times (seconds), accumulated by step() when set to a dict
+        self.timers = None
+
+    def _tick(self, key, t0):
+        t1 = time.perf_counter()
+        if self.timers is not None:
+            self.timers[key] = self.timers.get(key, 0.0) + (t1 - t0)
+        return t1
 
     # -- incremental potential pieces -------------------------------------
     def potential_gradient(self, x, xt):
@@ -188,7 +198,9 @@ class BEProblem:
             x[self.fixed] = x_curr[self.fixed]
         fr = self.free
         info = {"newton": 0, "ls": 0, "pcg": 0, "status": "ok", "stalled": False}
+        t0 = time.perf_counter()
         g = self.potential_gradient(x, xt)
+        t0 = self._tick("gradient", t0)
         g0 = np.linalg.norm(g[fr])
         stalled = False
         for _ in range(self.newton_max_iter):
@@ -196,7 +208,9 @@ class BEProblem:
             if gn <= self.newton_rtol * g0 or gn == 0.0 or stalled:
                 info["stalled"] = bool(stalled and gn > self.newton_rtol * g0)
                 break
+            t0 = time.perf_counter()
             H = self.potential_hessian(x)
+            t0 = self._tick("hessian", t0)
             dx = np.zeros_like(x)
             if self.linsolve == "spsolve":
                 Hff = H[fr][:, fr]
@@ -204,6 +218,7 @@ class BEProblem:
             else:
                 dx, it = block_jacobi_pcg(H, -g, fr, rtol=self.pcg_rtol)
                 info["pcg"] += it
+            t0 = self._tick("solve", t0)
             slope = g[fr] @ dx[fr]
             t = 1.0
             accepted = False
@@ -213,6 +228,7 @@ class BEProblem:
                     accepted = True
                     break
                 t *= 0.5
+            t0 = self._tick("line_search", t0)
             info["newton"] += 1
             if not accepted:
                 info["status"] = "line_search_failed"
@@ -221,6 +237,7 @@ class BEProblem:
             # an update below the position resolution (4 eps max |x|) cannot improve x further
             stalled = np.abs(t * dx[fr]).max() <= 4.0 * np.finfo(float).eps * np.abs(x[fr]).max()
             g = self.potential_gradient(x, xt)
+            self._tick("gradient", t0)
         else:
             info["status"] = "newton_max_iter"
         return (x, info) if return_info else x

@@ -72,6 +72,7 @@ class SkSimParams(ctypes.Structure):
         ("spmv_blocks", ctypes.c_int),
         ("update_blocks", ctypes.c_int),
         ("pcg_chunk", ctypes.c_int),
+        ("max_inverted", ctypes.c_int),
     ]
 
 
@@ -92,6 +93,8 @@ class SkStepInfo(ctypes.Structure):
         ("guess_dE", ctypes.c_double),
         ("guess_g_norm", ctypes.c_double),
         ("stalled", ctypes.c_int),
+        ("inverted_elems", ctypes.c_int),
+        ("polar_fp64_elems", ctypes.c_int64),
     ]
 
     def as_dict(self):
@@ -210,6 +213,7 @@ SIGNATURES = {
     "sk_coarse_order": [_P, _I64, _INT, _P, _P, _P],
     "sk_sell_order": [_P, _I64, _I64, _P],
     "sk_sim_coarse_info": [_P, _P, _P, _P, _P],
+    "sk_sim_set_order": [_P, _P],
 }
 _RESTYPES = {"sk_last_error": ctypes.c_char_p, "sk_status_string": ctypes.c_char_p, "sk_mlp_num_params": ctypes.c_int64}
 

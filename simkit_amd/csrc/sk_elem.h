@@ -38,6 +38,10 @@ __device__ __forceinline__ void make_F(const double (&xe)[12], const double (&D)
       F[3 * i + c] = xe[i] * D[4 * c] + xe[3 + i] * D[4 * c + 1] + xe[6 + i] * D[4 * c + 2] + xe[9 + i] * D[4 * c + 3];
 }
 
+__device__ __forceinline__ double det3(const double (&F)[9]) {
+  return F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) + F[2] * (F[3] * F[7] - F[4] * F[6]);
+}
+
 __device__ __forceinline__ void add_F0(const double* __restrict__ F0, int64_t e, double (&F)[9]) {
   if (F0) {
 #pragma unroll

@@ -144,9 +144,14 @@ int launch_elem_grad(sk_mesh* m, int material, const double* x, const double* F0
 // buffer reinterpreted), otherwise FP64.
 // active / pid (optional, together): skip elements whose problem (pid of the
 // element's first vertex) has active == 0 (batched problems).
+// fp64_count (optional): += elements whose FP32 Hessian of a polar density
+// (ARAP, FCR) was evaluated in FP64 because det F < 1e-3.
 int launch_elem_hess(sk_mesh* m, int material, const double* x, const double* F0, const double* mu,
                      const double* lam, const double* vol, int psd, bool fp32, hipStream_t st,
-                     const int* active = nullptr, const int32_t* pid = nullptr);
+                     const int* active = nullptr, const int32_t* pid = nullptr,
+                     unsigned long long* fp64_count = nullptr);
+// *result (device) = number of elements with det F(x) <= 0.
+int launch_count_inverted(sk_mesh* m, const double* x, double* result, hipStream_t st);
 // Gathers staged element forces into g; with mass != nullptr adds
 // m (x - xt) * inv_h2 - fext, zeroes fixed DOFs and reduces ||g||^2 into red.result[0].
 int launch_gather_grad(sk_mesh* m, double* g, const double* mass, const double* x, const double* xt,
@@ -206,6 +211,13 @@ constexpr int kQuadC = 64, kQuadVW = 4;
 int launch_gather_hess_quad(sk_mesh* m, void* values, const double* mass, double inv_h2, const uint8_t* fixed,
                             void* binv, const int32_t* pos, const int32_t* row_pos, bool fp32, hipStream_t st,
                             const int* active = nullptr, const int32_t* pid = nullptr);
+// Jacobi-preconditioned CG on a general CSR matrix (sk_pcg_csr) with
+// caller-provided scratch: work holds csr_pcg_scratch_bytes(n) bytes,
+// 64-byte aligned; nothing in it needs clearing. Launches at most maxit
+// iterations; returns the sk_pcg_csr statuses.
+size_t csr_pcg_scratch_bytes(int64_t n);
+int run_csr_pcg(int64_t n, const int32_t* indptr, const int32_t* indices, const double* values, const double* b,
+                double* x, double rtol, int64_t maxit, int64_t* iters, double* relres, void* work, hipStream_t st);
 // Copies n doubles from device to the mesh's pinned buffer and waits.
 int read_scalars(sk_mesh* m, const double* dev, int n, hipStream_t st);
 

@@ -96,6 +96,7 @@ struct ElemArgs {
   int psd;
   const int* active;    // optional (batched problems): skip elements whose problem is inactive
   const int32_t* pid;   // problem of each vertex, with `active`
+  unsigned long long* fp64_count;  // optional: += elements whose FP32 Hessian took the FP64 polar branch
 };
 
 template <class M>
@@ -122,24 +123,30 @@ __global__ __launch_bounds__(kBlock) void elem_grad_kernel(ElemArgs a) {
 // path). F is formed in FP64 from the FP64 positions and rounded to R; in FP32
 // the densities that use the polar decomposition (ARAP, FCR) still evaluate
 // elements with det F < 1e-3 in FP64, where the FP32 SVD of a near-singular F
-// loses the rotation.
+// loses the rotation. Returns whether that FP64 branch was taken.
 template <template <typename> class M, typename R>
-__device__ __forceinline__ void elem_hessian_eval(const double (&F)[9], double mu, double lam, R (&H)[45]) {
+__device__ __forceinline__ bool elem_hessian_eval(const double (&F)[9], double mu, double lam, R (&H)[45]) {
   if (sizeof(R) == 4 && M<R>::kPolar) {
-    const double det = F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) +
-                       F[2] * (F[3] * F[7] - F[4] * F[6]);
-    if (det < 1e-3) {
+    if (det3(F) < 1e-3) {
       double Hd[45];
       M<double>::hessian(F, mu, lam, Hd);
 #pragma unroll
       for (int k = 0; k < 45; ++k) H[k] = (R)Hd[k];
-      return;
+      return true;
     }
   }
   R Fr[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) Fr[k] = (R)F[k];
   M<R>::hessian(Fr, (R)mu, (R)lam, H);
+  return false;
+}
+
+// Adds the number of lanes of the wave with `flag` set to *count: one atomic
+// per wave, by the lowest flagged lane (integer sums do not depend on order).
+__device__ __forceinline__ void wave_count(bool flag, unsigned long long* count) {
+  const unsigned long long m = __ballot(flag);
+  if (flag && (int)__lane_id() == __ffsll((long long)m) - 1) atomicAdd(count, (unsigned long long)__popcll(m));
 }
 
 template <template <typename> class M, int PSD, typename R>
@@ -154,7 +161,8 @@ __global__ __launch_bounds__(kBlock) void elem_hess_kernel(ElemArgs a) {
   make_F(xe, Dd, F);
   add_F0(a.F0, e, F);
   R H[45];
-  elem_hessian_eval<M, R>(F, a.mu[e], a.lam ? a.lam[e] : 0.0, H);
+  const bool fp64_polar = elem_hessian_eval<M, R>(F, a.mu[e], a.lam ? a.lam[e] : 0.0, H);
+  if (sizeof(R) == 4 && M<R>::kPolar && a.fp64_count) wave_count(fp64_polar, a.fp64_count);
   const R vol = (R)a.vol[e];
 #pragma unroll
   for (int k = 0; k < 45; ++k) H[k] *= vol;
@@ -355,6 +363,23 @@ __global__ __launch_bounds__(kBlock) void gather_hess_kernel(GatherHArgs a) {
   }
 }
 
+// Per-block partial counts of elements with det F(x) <= 0 (inverted or
+// collapsed), summed by a finalize launch.
+__global__ __launch_bounds__(kBlock) void count_inverted_kernel(ElemView m, const double* __restrict__ x,
+                                                                double* partials) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double c[1] = {0.0};
+  if (e < m.nt) {
+    int32_t v[4];
+    double D[12], xe[12], F[9];
+    load_elem(m, e, v, D);
+    gather_x(x, v, xe);
+    make_F(xe, D, F);
+    c[0] = det3(F) <= 0.0 ? 1.0 : 0.0;
+  }
+  store_block_partials<1>(c, partials);
+}
+
 __global__ void csr_pattern_kernel(int64_t nv, int64_t nblk, const int32_t* bptr, const int32_t* bcol,
                                    const int32_t* brow, int32_t* indptr, int32_t* indices) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -428,7 +453,7 @@ int launch_elem_energy(sk_mesh* m, int material, const double* x, const double* 
 
 int launch_elem_grad(sk_mesh* m, int material, const double* x, const double* F0, const double* mu,
                      const double* lam, const double* vol, hipStream_t st) {
-  ElemArgs a{view(m), x, F0, mu, lam, vol, m->fst.p, m->Kst.p, 0, nullptr, nullptr};
+  ElemArgs a{view(m), x, F0, mu, lam, vol, m->fst.p, m->Kst.p, 0, nullptr, nullptr, nullptr};
   const int g = grid_for(m->nt);
   SK_REQUIRE(dispatch_material(material, [&](auto tag) {
                elem_grad_kernel<typename decltype(tag)::template type<double>><<<g, kBlock, 0, st>>>(a);
@@ -453,10 +478,10 @@ static void hess_dispatch_prec(bool fp32, int psd, int g, hipStream_t st, const 
 
 int launch_elem_hess(sk_mesh* m, int material, const double* x, const double* F0, const double* mu,
                      const double* lam, const double* vol, int psd, bool fp32, hipStream_t st, const int* active,
-                     const int32_t* pid) {
+                     const int32_t* pid, unsigned long long* fp64_count) {
   SK_REQUIRE(psd == SK_PSD_NONE || psd == SK_PSD_PROJ || psd == SK_PSD_ABS, "invalid psd method");
   SK_REQUIRE(!active == !pid, "launch_elem_hess: active and pid go together");
-  ElemArgs a{view(m), x, F0, mu, lam, vol, m->fst.p, m->Kst.p, psd, active, pid};
+  ElemArgs a{view(m), x, F0, mu, lam, vol, m->fst.p, m->Kst.p, psd, active, pid, fp64_count};
   const int g = grid_for(m->nt);
   bool known = true;
   switch (material) {
@@ -468,6 +493,15 @@ int launch_elem_hess(sk_mesh* m, int material, const double* x, const double* F0
     default: known = false;
   }
   SK_REQUIRE(known, "unknown material");
+  SK_HIP_CHECK(hipGetLastError());
+  return SK_OK;
+}
+
+int launch_count_inverted(sk_mesh* m, const double* x, double* result, hipStream_t st) {
+  const int g = grid_for(m->nt);
+  SK_REQUIRE(g <= Reducer::kMaxBlocks, "mesh too large for the inverted-element count");
+  count_inverted_kernel<<<g, kBlock, 0, st>>>(view(m), x, m->red.partials.p);
+  finalize_sum_kernel<1><<<1, kBlock, 0, st>>>(m->red.partials.p, g, result);
   SK_HIP_CHECK(hipGetLastError());
   return SK_OK;
 }

@@ -998,16 +998,24 @@ __global__ __launch_bounds__(kBlock) void cpcg_update_kernel(CsrPcgArgs a) {
 // ---------------------------------------------------------------------------
 // Backward-Euler helpers
 // ---------------------------------------------------------------------------
+// Caller's index of internal DOF k: with a vertex order (internal vertex t is
+// the caller's vertex order[t]) states cross the boundary permuted here, in
+// the kernels that read and write them.
+__device__ __forceinline__ int64_t caller_dof(const int32_t* order, int64_t k) {
+  return order ? 3 * (int64_t)order[k / 3] + k % 3 : k;
+}
+
 // x~ = x_c + h ((x_c - x_p) / h) (kinetic.py:87-90); x0 = x~ on free DOFs,
 // prescribed (or current) values on fixed DOFs.
 __global__ void be_init_kernel(int64_t n, const double* xc, const double* xp, const double* xbc,
-                               const uint8_t* fixed, double h, double* x, double* xt) {
+                               const uint8_t* fixed, double h, const int32_t* order, double* x, double* xt) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  const double c = xc[k];
-  const double t = c + h * ((c - xp[k]) / h);
+  const int64_t q = caller_dof(order, k);
+  const double c = xc[q];
+  const double t = c + h * ((c - xp[q]) / h);
   xt[k] = t;
-  x[k] = (fixed && fixed[k]) ? (xbc ? xbc[k] : c) : t;
+  x[k] = (fixed && fixed[k]) ? (xbc ? xbc[q] : c) : t;
 }
 
 // BDF2 inertial target (kinetic.py bdf2_target / velocity_bdf2, in the
@@ -1015,23 +1023,34 @@ __global__ void be_init_kernel(int64_t n, const double* xc, const double* xp, co
 //   v_c = (3 x_c - 4 x_p + x_p2) / (2h),  v_p = (3 x_p - 4 x_p2 + x_p3) / (2h)
 //   x~ = (4/3) x_c - (1/3) x_p + (8h/9) v_c - (2h/9) v_p.
 __global__ void bdf2_init_kernel(int64_t n, const double* xc, const double* xp, const double* xp2, const double* xp3,
-                                 const double* xbc, const uint8_t* fixed, double h, double* x, double* xt) {
+                                 const double* xbc, const uint8_t* fixed, double h, const int32_t* order, double* x,
+                                 double* xt) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  const double c = xc[k], p = xp[k], p2 = xp2[k], p3 = xp3[k];
+  const int64_t q = caller_dof(order, k);
+  const double c = xc[q], p = xp[q], p2 = xp2[q], p3 = xp3[q];
   const double vc = (3.0 * c - 4.0 * p + p2) / (2.0 * h);
   const double vp = (3.0 * p - 4.0 * p2 + p3) / (2.0 * h);
   const double t = (4.0 / 3.0) * c - (1.0 / 3.0) * p + (8.0 * h / 9.0) * vc - (2.0 * h / 9.0) * vp;
   xt[k] = t;
-  x[k] = (fixed && fixed[k]) ? (xbc ? xbc[k] : c) : t;
+  x[k] = (fixed && fixed[k]) ? (xbc ? xbc[q] : c) : t;
 }
 
 // dx = guess - x on free DOFs, 0 on fixed DOFs (which keep their prescribed
 // values whichever start point is taken).
-__global__ void guess_dx_kernel(int64_t n, const double* guess, const double* x, const uint8_t* fixed, double* dx) {
+__global__ void guess_dx_kernel(int64_t n, const double* guess, const double* x, const uint8_t* fixed,
+                                const int32_t* order, double* dx) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  dx[k] = (fixed && fixed[k]) ? 0.0 : guess[k] - x[k];
+  dx[k] = (fixed && fixed[k]) ? 0.0 : guess[caller_dof(order, k)] - x[k];
+}
+
+// out (caller's numbering) = x (internal numbering).
+__global__ void store_state_kernel(int64_t n, const double* __restrict__ x, const int32_t* order,
+                                   double* __restrict__ out) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  out[caller_dof(order, k)] = x[k];
 }
 
 // Largest of each of NV values over a block (fixed tree), result in thread 0.
@@ -1103,6 +1122,77 @@ __global__ __launch_bounds__(kBlock) void dot_kernel(int64_t n, const double* a,
   store_block_partials<1>(v, partials);
 }
 
+// ---------------------------------------------------------------------------
+// Generic CSR PCG on caller-provided scratch
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int kPcgChunk = 16;  // iterations enqueued between the generic PCG's convergence checks
+
+int spmv_grid(int64_t rows) {
+  const int64_t g = (rows + (kBlock / kRowLanes) - 1) / (kBlock / kRowLanes);
+  return (int)(g < kSpmvBlocks ? (g < 1 ? 1 : g) : kSpmvBlocks);
+}
+
+constexpr size_t kStateBytes = 64;  // PcgState, padded so the doubles after it stay 64-byte aligned
+static_assert(sizeof(PcgState) <= kStateBytes, "PcgState outgrew its scratch slot");
+}  // namespace
+
+// Scratch layout: the PcgState, then dinv, r, z, two search directions and
+// A p (n doubles each), then the reduction partials (2 gu + gs doubles).
+size_t csr_pcg_scratch_bytes(int64_t n) {
+  return kStateBytes + sizeof(double) * (6 * (size_t)n + 2 * (size_t)grid_for(n) + spmv_grid(n));
+}
+
+// Every scratch entry the kernels read is written first by the init launch,
+// so the scratch need not be cleared. At most maxit iterations are launched.
+int run_csr_pcg(int64_t n, const int32_t* indptr, const int32_t* indices, const double* values, const double* b,
+                double* x, double rtol, int64_t maxit, int64_t* iters, double* relres, void* work, hipStream_t st) {
+  const int gu = grid_for(n);
+  const int gs = spmv_grid(n);
+  PcgState* state = reinterpret_cast<PcgState*>(work);
+  double* dinv = reinterpret_cast<double*>(reinterpret_cast<char*>(work) + kStateBytes);
+  double* r = dinv + n;
+  double* z = r + n;
+  double* p0 = z + n;
+  double* p1 = p0 + n;
+  double* Ap = p1 + n;
+  double* partials = Ap + n;
+  CsrPcgArgs a{n, indptr, indices, values, dinv, b, x, r, z, p1, p0, Ap, state, rtol * rtol,
+               partials, gu, partials + 2 * gu, gs, 0};
+  cpcg_init_kernel<<<gu, kBlock, 0, st>>>(a);
+  SK_HIP_CHECK(hipGetLastError());
+  PcgState hs{};
+  int64_t launched = 0;
+  for (;;) {
+    for (int k = 0; k < kPcgChunk && launched < maxit; ++k, ++launched) {
+      a.k = (int)launched;
+      a.pin = (launched & 1) ? p1 : p0;
+      a.pout = (launched & 1) ? p0 : p1;
+      cpcg_spmv_kernel<<<gs, kBlock, 0, st>>>(a);
+      cpcg_update_kernel<<<gu, kBlock, 0, st>>>(a);
+    }
+    SK_HIP_CHECK(hipGetLastError());
+    SK_HIP_CHECK(hipMemcpyAsync(&hs, state, sizeof(PcgState), hipMemcpyDeviceToHost, st));
+    SK_HIP_CHECK(hipStreamSynchronize(st));
+    if (hs.done || launched >= maxit) break;
+  }
+  if (iters) *iters = hs.iter;
+  if (relres) *relres = hs.bb > 0.0 ? std::sqrt(hs.rr / hs.bb) : 0.0;
+  if (hs.done == 2) {
+    set_error("sk_pcg_csr: p^T A p <= 0 (matrix not positive definite)");
+    return SK_ERR_PCG_BREAKDOWN;
+  }
+  if (hs.done == 3) {
+    set_error("sk_pcg_csr: non-finite residual");
+    return SK_ERR_NONFINITE;
+  }
+  if (hs.done == 0) {
+    set_error("sk_pcg_csr: iteration limit reached before rtol");
+    return SK_ERR_PCG_MAXIT;
+  }
+  return SK_OK;
+}
+
 }  // namespace sk
 
 using namespace sk;
@@ -1119,6 +1209,8 @@ struct sk_sim {
   DevBuf<float> vals32, binv32, x32, r32, z32, p0_32, p1_32, Ap32;
   DevBuf<PcgState> st;
   PcgState* st_host = nullptr;  // pinned, coherent: the PCG kernels write iter and done through to it
+  DevBuf<unsigned long long> fp64_count;  // element Hessians that took the FP64 polar branch this step
+  DevBuf<int32_t> order;  // optional: internal vertex t is the caller's vertex order[t] (sk_sim_set_order)
   hipEvent_t chunk_ev[2] = {};
   hipEvent_t ev[6] = {};
   bool have_events = false;
@@ -1147,12 +1239,6 @@ int pcg_chunk(const sk_sim* S) {
   if (S->prm.pcg_chunk > 0) return S->prm.pcg_chunk;
   if (S->two_level()) return S->mixed() ? 2 : 4;
   return 8;
-}
-constexpr int kPcgChunk = 16;  // sk_pcg_csr
-
-int spmv_grid(int64_t rows) {
-  const int64_t g = (rows + (kBlock / kRowLanes) - 1) / (kBlock / kRowLanes);
-  return (int)(g < kSpmvBlocks ? (g < 1 ? 1 : g) : kSpmvBlocks);
 }
 
 // One SELL chunk per wave; the grid is rounded up to a multiple of 8 so the
@@ -1244,7 +1330,7 @@ int run_block_pcg(sk_sim* S, const PcgBufs<T>& B, const T* b, double rtol, int64
   int64_t launched = k0;
   int c = 0;
   for (bool stop = false; !stop; ++c) {
-    for (int k = 0; k < chunk; ++k, ++launched) {
+    for (int k = 0; k < chunk && launched < max_iter; ++k, ++launched) {
       a.k = (int)launched;
       a.restart = (k0 > 0 && launched == k0) ? 1 : 0;
       a.pin = (launched & 1) ? B.p1 : B.p0;
@@ -1263,9 +1349,9 @@ int run_block_pcg(sk_sim* S, const PcgBufs<T>& B, const T* b, double rtol, int64
       SK_HIP_CHECK(hipEventSynchronize(S->chunk_ev[(c - 1) & 1]));
       const int hd = h->done, hi = h->iter;
       if (hd || hi >= max_iter) stop = true;
-      if (launched > max_iter + 2 * chunk) stop = true;  // device state not advancing
-      if (k0 > 0 && hi < k0) stop = true;                     // continuation did not start
+      if (k0 > 0 && hi < k0) stop = true;  // continuation did not start
     }
+    if (launched >= max_iter) stop = true;  // no launches past the iteration limit
   }
   if (!iters) return SK_OK;
   SK_HIP_CHECK(hipEventSynchronize(S->chunk_ev[(c - 1) & 1]));
@@ -1360,55 +1446,10 @@ int sk_pcg_csr(int64_t n, const int32_t* indptr, const int32_t* indices, const d
   if (relres) *relres = 0.0;
   if (n == 0) return SK_OK;
   SK_REQUIRE(indptr && indices && values && b && x, "sk_pcg_csr: null pointer");
-  hipStream_t st = as_stream(s);
-  DevBuf<double> dinv, r, z, p0, p1, Ap, partials;
-  DevBuf<PcgState> state;
-  SK_TRY(dinv.alloc(n));
-  SK_TRY(r.alloc(n));
-  SK_TRY(z.alloc(n));
-  SK_TRY(p0.alloc(n));
-  SK_TRY(p1.alloc(n));
-  SK_TRY(Ap.alloc(n));
-  const int gu = grid_for(n);
-  const int gs = spmv_grid(n);
-  SK_REQUIRE(gu <= Reducer::kMaxBlocks, "sk_pcg_csr: system too large");
-  SK_TRY(partials.alloc(2 * (size_t)gu + gs));
-  SK_TRY(state.alloc(1));
-  CsrPcgArgs a{n, indptr, indices, values, dinv.p, b, x, r.p, z.p, p1.p, p0.p, Ap.p, state.p, rtol * rtol,
-               partials.p, gu, partials.p + 2 * gu, gs, 0};
-  cpcg_init_kernel<<<gu, kBlock, 0, st>>>(a);
-  SK_HIP_CHECK(hipGetLastError());
-  PcgState hs{};
-  int64_t launched = 0;
-  for (;;) {
-    for (int k = 0; k < kPcgChunk; ++k, ++launched) {
-      a.k = (int)launched;
-      a.pin = (launched & 1) ? p1.p : p0.p;
-      a.pout = (launched & 1) ? p0.p : p1.p;
-      cpcg_spmv_kernel<<<gs, kBlock, 0, st>>>(a);
-      cpcg_update_kernel<<<gu, kBlock, 0, st>>>(a);
-    }
-    SK_HIP_CHECK(hipGetLastError());
-    SK_HIP_CHECK(hipMemcpyAsync(&hs, state.p, sizeof(PcgState), hipMemcpyDeviceToHost, st));
-    SK_HIP_CHECK(hipStreamSynchronize(st));
-    if (hs.done || hs.iter >= maxit) break;
-    if (launched > maxit + kPcgChunk) break;  // device state not advancing
-  }
-  if (iters) *iters = hs.iter;
-  if (relres) *relres = hs.bb > 0.0 ? std::sqrt(hs.rr / hs.bb) : 0.0;
-  if (hs.done == 2) {
-    set_error("sk_pcg_csr: p^T A p <= 0 (matrix not positive definite)");
-    return SK_ERR_PCG_BREAKDOWN;
-  }
-  if (hs.done == 3) {
-    set_error("sk_pcg_csr: non-finite residual");
-    return SK_ERR_NONFINITE;
-  }
-  if (hs.done == 0) {
-    set_error("sk_pcg_csr: iteration limit reached before rtol");
-    return SK_ERR_PCG_MAXIT;
-  }
-  return SK_OK;
+  SK_REQUIRE(grid_for(n) <= Reducer::kMaxBlocks, "sk_pcg_csr: system too large");
+  DevBuf<char> work;
+  SK_TRY(work.alloc(csr_pcg_scratch_bytes(n)));
+  return run_csr_pcg(n, indptr, indices, values, b, x, rtol, maxit, iters, relres, work.p, as_stream(s));
 }
 
 int sk_sim_create(sk_mesh* mesh, const sk_sim_params* p, const double* mu, const double* lam, const double* vol,
@@ -1459,6 +1500,7 @@ int sk_sim_create(sk_mesh* mesh, const sk_sim_params* p, const double* mu, const
       SK_TRY(S->binv.alloc(9 * (size_t)mesh->nv));
     }
     SK_TRY(S->st.alloc(1));
+    SK_TRY(S->fp64_count.alloc(1));
     SK_HIP_CHECK(hipHostMalloc((void**)&S->st_host, sizeof(PcgState), hipHostMallocCoherent));
     *S->st_host = PcgState{};
     for (auto& e : S->ev) SK_HIP_CHECK(hipEventCreate(&e));
@@ -1503,6 +1545,23 @@ int sk_sim_set_coarse(sk_sim* S, const double* X, int per_agg) {
   return s;
 }
 
+int sk_sim_set_order(sk_sim* S, const int32_t* order_host) {
+  SK_REQUIRE(S, "sk_sim_set_order: null simulator");
+  if (!order_host) {
+    S->order.release();
+    return SK_OK;
+  }
+  const int64_t nv = S->mesh->nv;
+  std::vector<uint8_t> seen(nv, 0);
+  for (int64_t t = 0; t < nv; ++t) {
+    const int32_t v = order_host[t];
+    SK_REQUIRE(v >= 0 && v < nv && !seen[v], "sk_sim_set_order: order is not a permutation of the vertices");
+    seen[v] = 1;
+  }
+  std::vector<int32_t> h(order_host, order_host + nv);
+  return S->order.upload(h);
+}
+
 int sk_sim_coarse_info(const sk_sim* S, int* n_agg, int* nc, int* built, double* build_ms) {
   SK_REQUIRE(S, "sk_sim_coarse_info: null simulator");
   if (n_agg) *n_agg = S->coarse.n_agg;
@@ -1533,6 +1592,7 @@ int step_core(sk_sim* S, double c_mass, const double* x_guess, double* x_next, s
   const uint8_t* fixed = S->fixed.p;
   sk_step_info inf{};
   inf.status = SK_OK;
+  SK_HIP_CHECK(hipMemsetAsync(S->fp64_count.p, 0, sizeof(unsigned long long), st));
 
   SK_TRY(launch_elem_grad(m, P.material, S->x.p, nullptr, S->mu.p, lam, S->vol.p, st));
   SK_TRY(launch_gather_grad(m, S->g.p, S->mass.p, S->x.p, S->xt.p, fext, fixed, inv_h2, st));
@@ -1543,7 +1603,7 @@ int step_core(sk_sim* S, double c_mass, const double* x_guess, double* x_next, s
   inf.g0_norm = g0;
   if (!std::isfinite(g0)) inf.status = SK_ERR_NONFINITE;
   if (x_guess && inf.status == SK_OK) {
-    guess_dx_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, x_guess, S->x.p, fixed, S->dx.p);
+    guess_dx_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, x_guess, S->x.p, fixed, S->order.p, S->dx.p);
     SK_TRY(launch_elem_energy(m, P.material, S->x.p, nullptr, S->dx.p, 1.0, S->mu.p, lam, S->vol.p, nullptr,
                               S->mass.p, S->xt.p, fext, fixed, inv_h2, st));
     SK_TRY(read_scalars(m, m->red.result.p, 1, st));
@@ -1568,7 +1628,8 @@ int step_core(sk_sim* S, double c_mass, const double* x_guess, double* x_next, s
     }
     if (inf.newton_iters >= P.newton_max_iter) break;
     if (P.timing) SK_HIP_CHECK(hipEventRecord(S->ev[0], st));
-    SK_TRY(launch_elem_hess(m, P.material, S->x.p, nullptr, S->mu.p, lam, S->vol.p, P.psd, mixed, st));
+    SK_TRY(launch_elem_hess(m, P.material, S->x.p, nullptr, S->mu.p, lam, S->vol.p, P.psd, mixed, st, nullptr,
+                            nullptr, S->fp64_count.p));
     if (mixed)
       SK_TRY(launch_gather_hess(m, S->vals32.p, S->mass.p, inv_h2, fixed, S->binv32.p, kLayoutSELL, true, st));
     else
@@ -1634,7 +1695,17 @@ int step_core(sk_sim* S, double c_mass, const double* x_guess, double* x_next, s
   if (inf.status == SK_OK && !converged) inf.status = SK_ERR_NEWTON_MAXIT;
   inf.g_norm = gn;
   inf.stalled = (converged && stalled && !(gn <= P.newton_rtol * g0)) ? 1 : 0;
-  SK_HIP_CHECK(hipMemcpyAsync(x_next, S->x.p, n3 * sizeof(double), hipMemcpyDeviceToDevice, st));
+  // element counts: det F <= 0 at x_next (a double count in result[4]) and the FP64 polar branch
+  SK_TRY(launch_count_inverted(m, S->x.p, m->red.result.p + 4, st));
+  SK_HIP_CHECK(hipMemcpyAsync(m->pinned + 1, S->fp64_count.p, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                              st));
+  SK_TRY(read_scalars(m, m->red.result.p + 4, 1, st));
+  inf.inverted_elems = (int)m->pinned[0];
+  inf.polar_fp64_elems = (int64_t)*reinterpret_cast<const unsigned long long*>(m->pinned + 1);
+  if (inf.status == SK_OK && P.max_inverted >= 0 && inf.inverted_elems > P.max_inverted)
+    inf.status = SK_ERR_INVERTED;
+  store_state_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, S->x.p, S->order.p, x_next);
+  SK_HIP_CHECK(hipGetLastError());
   if (info) *info = inf;
   return SK_OK;
 }
@@ -1648,7 +1719,8 @@ int sk_sim_step(sk_sim* S, const double* x_curr, const double* x_prev, const dou
   SK_REQUIRE(S && x_curr && x_prev && x_next, "sk_sim_step: null argument");
   hipStream_t st = as_stream(s);
   const int64_t n3 = 3 * S->mesh->nv;
-  be_init_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, x_curr, x_prev, x_bc, S->fixed.p, S->prm.h, S->x.p, S->xt.p);
+  be_init_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, x_curr, x_prev, x_bc, S->fixed.p, S->prm.h, S->order.p,
+                                                   S->x.p, S->xt.p);
   SK_HIP_CHECK(hipGetLastError());
   return step_core(S, 1.0, nullptr, x_next, info, st);
 }
@@ -1658,7 +1730,8 @@ int sk_sim_step_guess(sk_sim* S, const double* x_curr, const double* x_prev, con
   SK_REQUIRE(S && x_curr && x_prev && x_next, "sk_sim_step_guess: null argument");
   hipStream_t st = as_stream(s);
   const int64_t n3 = 3 * S->mesh->nv;
-  be_init_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, x_curr, x_prev, x_bc, S->fixed.p, S->prm.h, S->x.p, S->xt.p);
+  be_init_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, x_curr, x_prev, x_bc, S->fixed.p, S->prm.h, S->order.p,
+                                                   S->x.p, S->xt.p);
   SK_HIP_CHECK(hipGetLastError());
   return step_core(S, 1.0, x_guess, x_next, info, st);
 }
@@ -1669,7 +1742,7 @@ int sk_sim_step_bdf2(sk_sim* S, const double* x_curr, const double* x_prev, cons
   hipStream_t st = as_stream(s);
   const int64_t n3 = 3 * S->mesh->nv;
   bdf2_init_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, x_curr, x_prev, x_prev2, x_prev3, x_bc, S->fixed.p,
-                                                     S->prm.h, S->x.p, S->xt.p);
+                                                     S->prm.h, S->order.p, S->x.p, S->xt.p);
   SK_HIP_CHECK(hipGetLastError());
   return step_core(S, 9.0 / 4.0, nullptr, x_next, info, st);
 }

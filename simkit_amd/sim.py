@@ -34,7 +34,8 @@ class BackwardEulerSim:
                  gravity=0.0, psd=True, newton_rtol=1e-6, newton_max_iter=100, pcg_rtol=1e-8,
                  pcg_max_iter=100000, ls_alpha=0.01, ls_beta=0.5, ls_max_iter=100, ls_threshold=1e-12,
                  timing=False, precision="fp64", inner_rtol=1e-2, max_refine=50, spmv_blocks=0,
-                 update_blocks=0, preconditioner="bjacobi", coarse_vertices=216, pcg_chunk=0, renumber=True):
+                 update_blocks=0, preconditioner="bjacobi", coarse_vertices=216, pcg_chunk=0, renumber=True,
+                 max_inverted=-1):
         """precision: "fp64" (every stage FP64) or "mixed" (FP32 element Hessian, PSD projection,
         assembled values and inner PCG to inner_rtol, wrapped in FP64 iterative refinement to
         pcg_rtol; gradient, line-search energies and the polar SVD of ARAP elements with
@@ -48,7 +49,10 @@ class BackwardEulerSim:
         them (two-level: aggregate by aggregate, so that each aggregate is a contiguous range, and
         in the sliced-ELL order within an aggregate; block-Jacobi: the sliced-ELL row order, so
         that the SpMV's row stores are contiguous);
-        states passed in and returned keep the caller's vertex order."""
+        states passed in and returned keep the caller's vertex order (the library permutes them in the
+        step's first and last kernels, sk_sim_set_order).
+        max_inverted: < 0 no limit; otherwise a step that ends with more elements of det F <= 0 reports
+        status SK_ERR_INVERTED (info["inverted_elems"] always holds the count)."""
         if preconditioner not in ("bjacobi", "two_level"):
             raise ValueError("preconditioner must be 'bjacobi' or 'two_level'")
         X = np.asarray(X, dtype=np.float64)
@@ -93,8 +97,6 @@ class BackwardEulerSim:
                 pos = np.empty_like(order)
                 pos[order] = np.arange(self.nv, dtype=np.int32)
                 self._order, self._pos = order, pos
-                self._order_dev = torch.as_tensor(order.astype(np.int64), device="cuda")
-                self._pos_dev = torch.as_tensor(pos.astype(np.int64), device="cuda")
         Xi = X if self._order is None else np.ascontiguousarray(X[self._order])
         Ti = T if self._order is None else self._pos[T]
         self.h = float(h)
@@ -140,6 +142,7 @@ class BackwardEulerSim:
         p.spmv_blocks = spmv_blocks
         p.update_blocks = update_blocks
         p.pcg_chunk = pcg_chunk
+        p.max_inverted = int(max_inverted)
         self.params = p
         self.precision = precision
         h_ = ctypes.c_void_p()
@@ -147,31 +150,19 @@ class BackwardEulerSim:
                   ptr(self.mass), ptr(self.f_ext), ptr(self.fixed), ctypes.byref(h_))
         self._h = h_
         self._fin = weakref.finalize(self, _destroy, h_.value)
+        if self._order is not None:
+            # states cross the boundary in the caller's numbering; the library permutes them
+            oc = np.ascontiguousarray(self._order, dtype=np.int32)
+            _lib.call("sk_sim_set_order", self._h, oc.ctypes.data_as(ctypes.c_void_p))
         self.X_dev = dev(X.reshape(-1))
         self.preconditioner = preconditioner
         if preconditioner == "two_level":
             Xc = np.ascontiguousarray(Xi, dtype=np.float64)
             _lib.call("sk_sim_set_coarse", self._h, Xc.ctypes.data_as(ctypes.c_void_p), int(coarse_vertices))
 
-    def _to_internal(self, x):
-        if x is None or self._order is None:
-            return x
-        return x.view(-1, 3).index_select(0, self._order_dev).reshape(-1)
-
     def _out(self, like, out):
-        """Device buffer for the library's result (internal order)."""
-        if self._order is None and out is not None:
-            return out
-        return torch.empty_like(like)
-
-    def _to_caller(self, xi, out):
-        if self._order is None:
-            return xi
-        x = xi.view(-1, 3).index_select(0, self._pos_dev).reshape(-1)
-        if out is not None:
-            out.copy_(x)
-            return out
-        return x
+        """Device buffer for the library's result (caller's numbering)."""
+        return torch.empty_like(like) if out is None else out
 
     def coarse_info(self):
         """Aggregates, coarse unknowns, whether the coarse inverse has been formed and its build time (ms)."""
@@ -184,29 +175,27 @@ class BackwardEulerSim:
     def step(self, x_curr, x_prev, x_bc=None, out=None):
         """One BE step on device tensors (3nv,). Returns (x_next, info dict)."""
         info = _lib.SkStepInfo()
-        xc, xp, xb = self._to_internal(x_curr), self._to_internal(x_prev), self._to_internal(x_bc)
-        x_next = self._out(xc, out)
-        _lib.call("sk_sim_step", self._h, ptr(xc), ptr(xp), ptr(xb), ptr(x_next), ctypes.byref(info), stream())
-        return self._to_caller(x_next, out), info.as_dict()
+        x_next = self._out(x_curr, out)
+        _lib.call("sk_sim_step", self._h, ptr(x_curr), ptr(x_prev), ptr(x_bc), ptr(x_next), ctypes.byref(info),
+                  stream())
+        return x_next, info.as_dict()
 
     def step_guess(self, x_curr, x_prev, x_guess, x_bc=None, out=None):
         """BE step that starts Newton from x_guess when its incremental potential is lower than at
         x~ (info["guess_used"]), with the stopping test still relative to ||g(x~)||."""
         info = _lib.SkStepInfo()
-        xc, xp, xb = self._to_internal(x_curr), self._to_internal(x_prev), self._to_internal(x_bc)
-        xg = self._to_internal(x_guess)
-        x_next = self._out(xc, out)
-        _lib.call("sk_sim_step_guess", self._h, ptr(xc), ptr(xp), ptr(xb), ptr(xg), ptr(x_next),
+        x_next = self._out(x_curr, out)
+        _lib.call("sk_sim_step_guess", self._h, ptr(x_curr), ptr(x_prev), ptr(x_bc), ptr(x_guess), ptr(x_next),
                   ctypes.byref(info), stream())
-        return self._to_caller(x_next, out), info.as_dict()
+        return x_next, info.as_dict()
 
     def step_bdf2(self, x_curr, x_prev, x_prev2, x_prev3, x_bc=None, out=None):
         """One BDF2 step (c = 9/4, BDF2 target) on device tensors."""
         info = _lib.SkStepInfo()
-        h = [self._to_internal(v) for v in (x_curr, x_prev, x_prev2, x_prev3, x_bc)]
-        x_next = self._out(h[0], out)
-        _lib.call("sk_sim_step_bdf2", self._h, *[ptr(v) for v in h], ptr(x_next), ctypes.byref(info), stream())
-        return self._to_caller(x_next, out), info.as_dict()
+        x_next = self._out(x_curr, out)
+        _lib.call("sk_sim_step_bdf2", self._h, *[ptr(v) for v in (x_curr, x_prev, x_prev2, x_prev3, x_bc)],
+                  ptr(x_next), ctypes.byref(info), stream())
+        return x_next, info.as_dict()
 
     def rollout(self, steps, bc=None, x0=None, keep=None, integrator="be"):
         """`steps` steps from rest (x_prev = x_curr = X, or x0). bc(n, t) -> device x_bc for step n
@@ -240,12 +229,16 @@ def _destroy(h):
 
 def top_face_sinusoid(X, top_vertices, amplitude=0.1, period=1.0, axis=0):
     """bc(n, t) for BackwardEulerSim.rollout: top vertices at X + amplitude sin(2 pi t / period) e_axis,
-    every other fixed vertex at X. Built from two device vectors (rest positions and a unit mask)."""
+    every other fixed vertex at X. Built from two device vectors (rest positions and a unit mask)
+    combined by the library's axpby kernel into a fresh buffer per call."""
     Xd = dev(np.asarray(X, np.float64).reshape(-1))
     mask = np.zeros(Xd.numel())
     mask[3 * np.asarray(top_vertices) + axis] = 1.0
     maskd = dev(mask)
 
     def bc(n, t):
-        return torch.add(Xd, maskd, alpha=float(amplitude * np.sin(2.0 * np.pi * t / period)))
+        out = torch.empty_like(Xd)
+        _lib.call("sk_axpby", Xd.numel(), 1.0, ptr(Xd), float(amplitude * np.sin(2.0 * np.pi * t / period)),
+                  ptr(maskd), ptr(out), stream())
+        return out
     return bc

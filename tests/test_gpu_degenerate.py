@@ -1,0 +1,311 @@
+"""GPU parity on degenerate deformation gradients and failure statuses.
+
+* The reference's own collapse / stretch cases for stable neo-Hookean
+  (tests/test_stable_neo_hookean.py:110-176: a unit tet shrunk to F = 0,
+  squashed flat, and one vertex dragged to 1e6) through the self-contained,
+  x and element tiers, against the oracle at 1e-10 (per element, FP64).
+* The 1e-8 clamp of dR/dF (simkit/rotation_gradient.py:65-67): F whose
+  singular-value pair sums lie below 1e-8 (also negative, for inverted F)
+  through the rotation gradient and the ARAP / FCR element Hessians, against
+  the oracle at 1e-10 and against the closed form t_k^T (dR/dF) t_k =
+  2 / max(s_i + s_j, 1e-8) of the twist directions.
+* A mesh started with inverted tets, stepped by ARAP and FCR in FP64 and in
+  mixed precision, against the FP64 oracle (direct solve) at 1e-6 per step;
+  the mixed path's FP64 polar branch (det F < 1e-3) must have been taken.
+* Status codes reached on the device: PCG breakdown on an indefinite system
+  (psd off, compressed state) and a non-finite system with a finite free
+  gradient, through the FP64 and mixed solves; the simulator's next
+  clean step is bitwise identical to a fresh simulator's. SK_ERR_INVERTED with
+  a fixed inverted layer, and SK_ERR_PCG_MAXIT from the generic PCG (exactly
+  maxit iterations).
+"""
+
+import numpy as np
+import pytest
+
+from conftest import per_item_rel_err, rel_err
+from oracle import fem, solver
+from oracle.mesh import unit_cube
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+def _unit_tet():
+    """tests/test_stable_neo_hookean.py:_unit_tet (mu = lam = 1)."""
+    X = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]])
+    return X, np.array([[0, 1, 2, 3]]), 1.0, 1.0
+
+
+def _F_of(X, T, U):
+    D = fem.tet_operator(X, T)
+    return fem.deformation_gradient(D, T, U)
+
+
+def _within(a, b, floor):
+    """Per-element error of a against b on max(|b_e|, floor) (floor keeps F = 0, where H = 0, comparable)."""
+    a = np.asarray(a, float).reshape(len(b), -1)
+    b = np.asarray(b, float).reshape(len(b), -1)
+    scale = np.maximum(np.abs(b).max(axis=1), floor)
+    return float((np.abs(a - b).max(axis=1) / scale).max())
+
+
+def test_snh_collapse_and_stretch_reference_cases():
+    from simkit_amd.energies import stable_neo_hookean as snh
+    X, T, mu, lam = _unit_tet()
+    shrink = [s * X for s in (1.0, 0.5, 1e-1, 1e-3, 1e-6, 0.0)]
+    shear = []
+    for s in (1.0, 0.5, 1e-1, 1e-3, 1e-6, 0.0):
+        U = X.copy()
+        U[:, 2] *= s
+        shear.append(U)
+    stretch = []
+    for s in (1.0, 10.0, 1e3, 1e6):
+        U = X.copy()
+        U[-1] *= s
+        stretch.append(U)
+    D = fem.tet_operator(X, T)
+    J = fem.deformation_jacobian(D, T, 4)
+    vol = fem.tet_volumes(X, T)
+    for states in (shrink, shear, stretch):
+        E = np.array([snh.stable_neo_hookean_energy(X, T, mu, lam, U) for U in states])
+        Eo = np.array([fem.energy_x("snh", U.reshape(-1), J, [mu], [lam], vol) for U in states])
+        assert np.all(np.isfinite(E))
+        assert np.abs(E - Eo).max() <= TOL * np.abs(Eo).max()
+        F = np.concatenate([_F_of(X, T, U) for U in states])
+        mus, lams = np.full(len(F), mu), np.full(len(F), lam)
+        assert _within(snh.stable_neo_hookean_energy_element_F(F, mus, lams),
+                       fem.snh_energy_element_F(F, mus, lams), mu) < TOL
+        assert _within(snh.stable_neo_hookean_gradient_element_F(F, mus, lams),
+                       fem.snh_gradient_element_F(F, mus, lams), mu) < TOL
+        assert _within(snh.stable_neo_hookean_hessian_element_F(F, mus, lams),
+                       fem.snh_hessian_element_F(F, mus, lams), mu) < TOL
+        for U in states:
+            g = snh.stable_neo_hookean_gradient(X, T, mu, lam, U)
+            go = fem.gradient_x("snh", U.reshape(-1), J, [mu], [lam], vol)
+            assert np.abs(g.ravel() - go.ravel()).max() <= TOL * max(np.abs(go).max(), mu)
+            for psd in (True, False):
+                Q = snh.stable_neo_hookean_hessian(X, T, mu, lam, U, psd=psd).toarray()
+                Qo = fem.hessian_x("snh", U.reshape(-1), J, [mu], [lam], vol, psd=psd).toarray()
+                assert np.abs(Q - Qo).max() <= TOL * max(np.abs(Qo).max(), mu)
+    # the reference's assertions (test_stable_neo_hookean.py:132-170)
+    Es = np.array([snh.stable_neo_hookean_energy(X, T, mu, lam, U) for U in stretch])
+    assert np.all(np.diff(Es) > 0) and Es[-1] > 1e6 * abs(Es[0]) + 1.0
+
+
+def _clamp_cases(seed=0):
+    """F = R diag(s) with singular-value pair sums below the 1e-8 clamp, some inverted."""
+    rng = np.random.default_rng(seed)
+    sigmas = [(1.0, 3e-9, 2e-9), (1.0, 3e-9, -2e-9), (1.0, 1e-9, -2e-9), (2e-9, 1.0, 4e-9),
+              (1.0, 1e-9, 0.0), (0.7, 1.2e-8, 5e-9), (1.5, 8e-9, 1.1e-8), (1.0, 2e-8, 1e-8)]
+    Fs, S = [], []
+    for s in sigmas:
+        q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+        q *= np.sign(np.linalg.det(q))
+        Fs.append(q @ np.diag(s))
+        S.append(s)
+    return np.array(Fs), np.array(S)
+
+
+def test_rotation_gradient_clamp():
+    import simkit_amd as sk
+    F, s = _clamp_cases()
+    K = sk.rotation_gradient_F(F)
+    assert per_item_rel_err(K, fem.rotation_gradient_F(F)) < TOL
+    # closed form: with F = R diag(s) (V = I), t_k = vec(R T_k)/sqrt2 and t_k^T K t_k = 2/max(s_i+s_j, 1e-8)
+    # (the signed-SVD convention puts sign det F on the smallest singular value, as svd_rv.py:42-51)
+    for e in range(len(F)):
+        _, ss, _ = fem.svd_rv(F[e:e + 1])
+        R = fem.polar_rotation(F[e:e + 1])[0]
+        U, _, V = fem.svd_rv(F[e:e + 1])
+        for Tk, i, j in fem._TWISTS:
+            t = (U[0] @ Tk @ V[0].T).reshape(9) / np.sqrt(2.0)
+            want = 2.0 / max(ss[0, i] + ss[0, j], 1e-8)
+            # roundoff of the quadratic form is relative to the largest entry of K
+            assert abs(t @ K[e] @ t - want) <= 1e-10 * max(want, np.abs(K[e]).max()), (e, i, j)
+        assert np.allclose(R @ R.T, np.eye(3), atol=1e-12)
+    # a removed clamp would give 2/(s1+s2) = 4e8 instead of 2e8 for the first case
+    U, ss, V = fem.svd_rv(F[:1])
+    t = (U[0] @ fem._TWISTS[1][0] @ V[0].T).reshape(9) / np.sqrt(2.0)
+    assert abs(t @ K[0] @ t - 2e8) <= 1e-2
+
+
+def test_arap_fcr_element_tier_at_the_clamp():
+    from simkit_amd.energies import arap, fcr
+    F, _ = _clamp_cases(1)
+    mu = np.full(len(F), 2e5)
+    lam = np.full(len(F), 1e6)
+    assert per_item_rel_err(arap.arap_hessian_element_F(F, mu), fem.arap_hessian_element_F(F, mu)) < TOL
+    assert per_item_rel_err(arap.arap_gradient_element_F(F, mu), fem.arap_gradient_element_F(F, mu)) < TOL
+    assert per_item_rel_err(fcr.fcr_hessian_element_F(F, mu, lam), fem.fcr_hessian_element_F(F, mu, lam)) < TOL
+    assert per_item_rel_err(fcr.fcr_gradient_element_F(F, mu, lam), fem.fcr_gradient_element_F(F, mu, lam)) < TOL
+    # near-inverted and inverted F (det in [-0.1, 1e-3]): the range of the mixed path's FP64 polar branch
+    rng = np.random.default_rng(2)
+    G = []
+    while len(G) < 400:
+        A = np.eye(3) + 0.6 * rng.standard_normal((3, 3))
+        d = np.linalg.det(A)
+        if -0.1 <= d <= 1e-3:
+            G.append(A)
+    G = np.array(G)
+    m = np.full(len(G), 3e5)
+    assert per_item_rel_err(arap.arap_hessian_element_F(G, m), fem.arap_hessian_element_F(G, m)) < TOL
+    assert per_item_rel_err(fcr.fcr_hessian_element_F(G, m, 4 * m), fem.fcr_hessian_element_F(G, m, 4 * m)) < TOL
+
+
+def _tangled_cube(n=6, seed=0):
+    """unit_cube(n) with 6 interior vertices pushed through their neighbours: 37 inverted tets."""
+    X, T = unit_cube(n)
+    rng = np.random.default_rng(seed)
+    interior = np.where((X.min(1) > 0) & (X.max(1) < 1))[0]
+    x0 = X.copy()
+    for v in rng.choice(interior, 6, replace=False):
+        x0[v] += rng.choice([-1, 1], 3) * np.array([1.3, 1.1, 0.0]) / n
+    return X, T, x0
+
+
+@pytest.mark.parametrize("mat", ["arap", "fcr"])
+def test_tangled_mesh_mixed_and_fp64_match_oracle(mat):
+    """Inverted tets untangled by ARAP / FCR: FP64 and mixed-precision trajectories against the FP64
+    oracle with a direct solve, 1e-6 relative per step; in mixed precision the near-inverted
+    elements (det F < 1e-3) take the FP64 polar branch."""
+    import torch
+    from simkit_amd.sim import BackwardEulerSim
+    X, T, x0 = _tangled_cube()
+    vol0 = fem.tet_volumes(X, T)
+    inv0 = int((fem.tet_volumes(x0, T) / vol0 <= 0).sum())
+    assert inv0 > 0
+    bottom = np.where(X[:, 1] == 0)[0]
+    mu, lam = fem.ympr_to_lame(1e6, 0.45)
+    steps = 4
+    ref = solver.BEProblem(X, T, mu, lam, 1000.0, 1 / 60, bottom, material=mat, gravity=-9.8,
+                           linsolve="spsolve").rollout(steps, x0=x0.reshape(-1))
+    for prec in ("fp64", "mixed"):
+        sim = BackwardEulerSim(X, T, mu, lam, rho=1000.0, h=1 / 60, material=mat, fixed_vertices=bottom,
+                               gravity=-9.8, precision=prec, pcg_rtol=1e-8)
+        kept, infos = sim.rollout(steps, x0=x0.reshape(-1))
+        print(prec, [(i["newton_iters"], i["polar_fp64_elems"], i["inverted_elems"]) for i in infos], flush=True)
+        assert all(i["status"] == 0 for i in infos), infos
+        for s in range(1, steps + 1):
+            err = np.abs(kept[s] - ref[s - 1]).max() / np.abs(ref[s - 1]).max()
+            assert err < 1e-6, (prec, s, err)
+            assert infos[s - 1]["inverted_elems"] == int((fem.tet_volumes(kept[s].reshape(-1, 3), T) / vol0
+                                                          <= 0).sum())
+        if prec == "mixed":
+            assert infos[0]["polar_fp64_elems"] >= inv0
+        else:
+            assert all(i["polar_fp64_elems"] == 0 for i in infos)
+    assert torch.cuda.is_available()
+
+
+def test_inverted_status_with_a_fixed_inverted_layer():
+    from simkit_amd import _lib
+    from simkit_amd.sim import BackwardEulerSim
+    n = 4
+    X, T = unit_cube(n)
+    low = np.where(X[:, 1] <= 1.0 / n + 1e-12)[0]          # bottom two vertex layers, held fixed
+    x0 = X.copy()
+    layer1 = np.where(np.abs(X[:, 1] - 1.0 / n) < 1e-12)[0]
+    x0[layer1, 1] = -0.5 / n                                # pushed below layer 0: that cell layer is inverted
+    vol0 = fem.tet_volumes(X, T)
+    n_inv = int((fem.tet_volumes(x0, T) / vol0 <= 0).sum())
+    assert n_inv > 0
+    mu, lam = fem.ympr_to_lame(1e5, 0.4)
+    for limit, want in ((-1, 0), (n_inv, 0), (0, _lib.SK_ERR_INVERTED)):
+        sim = BackwardEulerSim(X, T, mu, lam, rho=1000.0, h=1 / 60, fixed_vertices=low, gravity=-9.8,
+                               max_inverted=limit)
+        kept, infos = sim.rollout(1, x0=x0.reshape(-1))
+        assert infos[0]["status"] == want, (limit, infos[0])
+        got = int((fem.tet_volumes(kept[1].reshape(-1, 3), T) / vol0 <= 0).sum())
+        assert infos[0]["inverted_elems"] == got >= n_inv
+
+
+def _cube10_sim(**kw):
+    from simkit_amd.sim import BackwardEulerSim
+    X, T = unit_cube(10)
+    mu, lam = fem.ympr_to_lame(1e6, 0.45)
+    return X, BackwardEulerSim(X, T, mu, lam, rho=1000.0, h=1 / 60, fixed_vertices=np.where(X[:, 1] == 0)[0],
+                               gravity=-9.8, **kw)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_pcg_breakdown_on_indefinite_system_then_clean_step(prec):
+    """psd off on a cube compressed to half size: the system is indefinite (the oracle's block-Jacobi PCG
+    meets p^T A p <= 0 within two iterations), so the device PCG reports a breakdown through its
+    deferred result; the same simulator's next step from rest equals a fresh simulator's bitwise."""
+    import torch
+    from simkit_amd import _lib
+    from simkit_amd._device import dev
+    X, sim = _cube10_sim(psd=False, precision=prec)
+    half = dev(0.5 * X.reshape(-1))
+    _, info = sim.step(half, half)
+    assert info["status"] == _lib.SK_ERR_PCG_BREAKDOWN, info
+    x = sim.X_dev.clone()
+    a, ia = sim.step(x, x)
+    _, fresh = _cube10_sim(psd=False, precision=prec)
+    b, ib = fresh.step(x, x)
+    assert ia["status"] == 0 and torch.equal(a, b)
+    assert {k: v for k, v in ia.items() if not k.startswith("ms_")} == \
+        {k: v for k, v in ib.items() if not k.startswith("ms_")}
+
+
+def _layer_fixed_sim(**kw):
+    from simkit_amd.sim import BackwardEulerSim
+    n = 4
+    X, T = unit_cube(n)
+    low = np.where(X[:, 1] <= 1.0 / n + 1e-12)[0]          # bottom two vertex layers, held fixed
+    mu, lam = fem.ympr_to_lame(1e6, 0.45)
+    return X, BackwardEulerSim(X, T, mu, lam, rho=1000.0, h=1 / 60, fixed_vertices=low, gravity=-9.8, **kw)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_nonfinite_system_with_finite_gradient_then_clean_step(prec):
+    """A vertex of the fixed bottom layer at 1e200: only elements whose four vertices are all fixed
+    see it, so the free gradient stays finite (fixed DOFs are eliminated), while those elements'
+    Hessians, and so the fixed rows' block-Jacobi inverses, are NaN. The solve reports
+    SK_ERR_NONFINITE from its deferred PCG result (FP64, and the mixed path's FP32 run and FP64
+    residual), and the same simulator's next step from rest equals a fresh simulator's bitwise."""
+    import torch
+    from simkit_amd import _lib
+    from simkit_amd._device import dev
+    X, sim = _layer_fixed_sim(precision=prec)
+    assert X[0, 1] == 0.0
+    far = X.copy()
+    far[0] = 1e200
+    fd = dev(far.reshape(-1))
+    _, info = sim.step(fd, fd)
+    assert np.isfinite(info["g0_norm"]) and info["status"] == _lib.SK_ERR_NONFINITE, info
+    x = sim.X_dev.clone()
+    a, ia = sim.step(x, x)
+    _, fresh = _layer_fixed_sim(precision=prec)
+    b, ib = fresh.step(x, x)
+    assert ia["status"] == 0 and torch.equal(a, b)
+    assert {k: v for k, v in ia.items() if not k.startswith("ms_")} == \
+        {k: v for k, v in ib.items() if not k.startswith("ms_")}
+
+
+def test_generic_pcg_iteration_limit():
+    """SK_ERR_PCG_MAXIT from the generic PCG (the spsolve replacement): exactly maxit iterations run,
+    and the iterate is textbook Jacobi-preconditioned CG's after maxit steps."""
+    from simkit_amd.linsolve import pcg_solve
+    X, T = unit_cube(5)
+    p = solver.BEProblem(X, T, 3e5, 2e6, 1000.0, 1 / 60, [], gravity=-9.8)
+    H = p.potential_hessian(X.reshape(-1) + 0.01)
+    b = np.random.default_rng(1).standard_normal(H.shape[0])
+    dinv = 1.0 / H.diagonal()
+    x = np.zeros_like(b)
+    r = b.copy()
+    z = dinv * r
+    q = z.copy()
+    for it in range(20):
+        with pytest.warns(RuntimeWarning, match="iterations reached"):
+            xg, info = pcg_solve(H, b, rtol=1e-14, maxit=it + 1, return_info=True)
+        Aq = H @ q
+        a = (r @ z) / (q @ Aq)
+        x = x + a * q
+        rn = r - a * Aq
+        zn = dinv * rn
+        q = zn + (rn @ zn) / (r @ z) * q
+        r, z = rn, zn
+        assert info["iters"] == it + 1 and info["relres"] > 1e-14
+        assert rel_err(xg, x) < 1e-10, (it, rel_err(xg, x))

@@ -167,6 +167,23 @@ def test_arap_twist_eigenstructure():
         assert np.allclose(ev, np.sort(np.concatenate([tw, np.full(6, mu)])), rtol=1e-10, atol=1e-6)
 
 
+def test_rotation_gradient_clamp_spectrum():
+    """Where s_i + s_j < 1e-8 (rotation_gradient.py:65-67 clamps it to 1e-8, also when negative for
+    inverted F) dR/dF keeps the twist eigenstructure: eigenvalues 2/max(s_i + s_j, 1e-8) on the three
+    twists, 0 on the other six directions (eigvalsh, independent of the outer-product form)."""
+    rng = np.random.default_rng(8)
+    for sig in ((1.0, 3e-9, 2e-9), (1.0, 3e-9, -2e-9), (1.0, 1e-9, -2e-9), (1.0, 1e-9, 0.0), (0.7, 1.2e-8, 5e-9)):
+        q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+        q *= np.sign(np.linalg.det(q))
+        F = (q @ np.diag(sig))[None]
+        K = fem.rotation_gradient_F(F)[0]
+        s = np.sort(np.abs(sig))[::-1] * np.array([1, 1, np.sign(np.prod(sig)) or 1.0])
+        want = np.sort(np.concatenate([np.zeros(6), [2 / max(s[0] + s[1], 1e-8), 2 / max(s[1] + s[2], 1e-8),
+                                                     2 / max(s[0] + s[2], 1e-8)]]))
+        ev = np.sort(np.linalg.eigvalsh(K))
+        assert np.allclose(ev, want, rtol=1e-9, atol=1e-6 * want.max()), (sig, ev, want)
+
+
 def test_psd_invariants():
     rng = np.random.default_rng(8)
     A = rng.standard_normal((20, 9, 9))
