@@ -188,6 +188,17 @@ int sk_kinetic_bdf2(int64_t n, const double* x, const double* x_curr, const doub
 int sk_pcg_csr(int64_t n, const int32_t* indptr, const int32_t* indices, const double* values, const double* b,
                double* x, double rtol, int64_t maxit, int64_t* iters, double* relres, sk_stream_t stream);
 
+/* sk_pcg_csr without any allocation: the same kernels on a static device
+ * workspace (one per GPU, 25.2 MB, held until the process ends) instead of
+ * buffers allocated per call; n <= 524288 (larger n returns SK_ERR_INVALID).
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under stream capture, serving loops, harnesses that forbid
+ * hipMalloc). All calls on one GPU share its workspace, so calls that overlap
+ * on one GPU race; results equal sk_pcg_csr's bit for bit. */
+int sk_pcg_csr_noalloc(int64_t n, const int32_t* indptr, const int32_t* indices, const double* values,
+                       const double* b, double* x, double rtol, int64_t maxit, int64_t* iters, double* relres,
+                       sk_stream_t stream);
+
 /* ------------------------------------------------------------------------ */
 /* Device-resident backward-Euler simulator                                  */
 /* ------------------------------------------------------------------------ */

@@ -181,6 +181,7 @@ SIGNATURES = {
     "sk_kinetic": [_I64, _P, _P, _P, _D, _D, _P, _P, _P, ctypes.POINTER(_D), _P, _P, _P, _P],
     "sk_kinetic_bdf2": [_I64, _P, _P, _P, _P, _P, _D, _P, _P, _P, ctypes.POINTER(_D), _P, _P, _P, _P],
     "sk_pcg_csr": [_I64, _P, _P, _P, _P, _P, _D, _I64, ctypes.POINTER(_I64), ctypes.POINTER(_D), _P],
+    "sk_pcg_csr_noalloc": [_I64, _P, _P, _P, _P, _P, _D, _I64, ctypes.POINTER(_I64), ctypes.POINTER(_D), _P],
     "sk_sim_create": [_P, ctypes.POINTER(SkSimParams), _P, _P, _P, _P, _P, _P, ctypes.POINTER(_P)],
     "sk_sim_destroy": [_P],
     "sk_sim_step": [_P, _P, _P, _P, _P, ctypes.POINTER(SkStepInfo), _P],

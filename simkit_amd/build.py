@@ -14,7 +14,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsimkit_amd.so")
 SOURCES = ["sk_element.hip", "sk_mesh.hip", "sk_solver.hip", "sk_coarse.hip", "sk_aux.hip", "sk_modal.hip",
-           "sk_mlp.hip", "sk_batch.hip"]
+           "sk_mlp.hip", "sk_batch.hip", "sk_noalloc.hip"]
 ARCH = os.environ.get("SIMKIT_AMD_ARCH", "gfx950")
 
 

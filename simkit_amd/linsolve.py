@@ -15,7 +15,9 @@ from . import _lib
 from ._device import I32, dev, empty, host, ptr, stream
 
 
-def pcg_solve(H, b, rtol=1e-12, maxit=None, return_info=False):
+def pcg_solve(H, b, rtol=1e-12, maxit=None, return_info=False, allocate=True):
+    """allocate=False runs the library's allocation-free entry point (sk_pcg_csr_noalloc: a static
+    device workspace, at most 524,288 unknowns); the answer is the same bit for bit."""
     H = sps.csr_matrix(H)
     n = H.shape[0]
     if H.shape[0] != H.shape[1]:
@@ -31,8 +33,9 @@ def pcg_solve(H, b, rtol=1e-12, maxit=None, return_info=False):
     x = empty(n)
     iters = ctypes.c_int64()
     relres = ctypes.c_double()
-    status = _lib.load().sk_pcg_csr(n, ptr(ip), ptr(ix), ptr(vals), ptr(bd), ptr(x), float(rtol), int(maxit),
-                                    ctypes.byref(iters), ctypes.byref(relres), stream())
+    entry = _lib.load().sk_pcg_csr if allocate else _lib.load().sk_pcg_csr_noalloc
+    status = entry(n, ptr(ip), ptr(ix), ptr(vals), ptr(bd), ptr(x), float(rtol), int(maxit), ctypes.byref(iters),
+                   ctypes.byref(relres), stream())
     if status == _lib.SK_ERR_PCG_BREAKDOWN:
         raise np.linalg.LinAlgError("Hessian is not positive definite (CG breakdown); use psd=True")
     if status == _lib.SK_ERR_PCG_MAXIT:
