@@ -87,7 +87,8 @@ int sk_element_F(int material, const double* F, const double* mu, const double* 
                  double* P, double* H, sk_stream_t stream);
 
 /* Batched eigenvalue projection of n symmetric d x d matrices (lower triangle
- * read), method SK_PSD_PROJ (floor 1e-6) or SK_PSD_ABS. d in 1..12.
+ * read), method SK_PSD_PROJ (floor 1e-6) or SK_PSD_ABS; any d >= 1 (d <= 12 in
+ * registers, larger d in device scratch).
  * Replaces psd_project.py:12-47. */
 int sk_psd_project(const double* H, int64_t n, int d, int method, double* out, sk_stream_t stream);
 

@@ -2,7 +2,7 @@
 // gradients, SNH/ARAP densities on user F arrays, batched PSD projection and
 // the polar SVD / rotation gradient. One lane owns one element (or matrix);
 // all per-element linear algebra stays in registers.
-#include "sk_common.h"
+#include "sk_internal.h"
 #include "sk_materials.h"
 
 #include <mutex>
@@ -116,6 +116,68 @@ __global__ void psd_project_kernel(const double* __restrict__ Hin, int64_t n, in
   for (int i = 0; i < N; ++i)
 #pragma unroll
     for (int j = 0; j < N; ++j) o[i * N + j] = a[Sym<N>::idx(i, j)];
+}
+
+// Any other size: the same cyclic Jacobi (rotation rule and stopping test of
+// jacobi_eigh) with the matrix and its eigenvectors in global memory, one lane
+// per matrix: A and V are n d^2 scratch each, the result is V diag(s') V^T.
+__global__ void psd_project_general_kernel(const double* __restrict__ Hin, int64_t n, int d, int method,
+                                           double* __restrict__ A, double* __restrict__ V,
+                                           double* __restrict__ Hout) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int64_t o = (int64_t)d * d * e;
+  const double* h = Hin + o;
+  double* a = A + o;
+  double* v = V + o;
+  for (int i = 0; i < d; ++i)
+    for (int j = 0; j < d; ++j) {
+      a[i * d + j] = i >= j ? h[i * d + j] : h[j * d + i];  // lower triangle, as numpy.linalg.eigh
+      v[i * d + j] = i == j ? 1.0 : 0.0;
+    }
+  for (int sweep = 0; sweep < 30; ++sweep) {
+    double off = 0.0, dia = 0.0;
+    for (int p = 0; p < d; ++p) {
+      dia += a[p * d + p] * a[p * d + p];
+      for (int q = p + 1; q < d; ++q) off += a[p * d + q] * a[p * d + q];
+    }
+    if (!(off > 1e-32 * dia)) break;
+    for (int p = 0; p < d - 1; ++p)
+      for (int q = p + 1; q < d; ++q) {
+        const double apq = a[p * d + q], app = a[p * d + p], aqq = a[q * d + q];
+        const double dd = aqq - app;
+        const double rt = sqrt(dd * dd + 4.0 * apq * apq);
+        double t = 2.0 * apq / (fabs(dd) + rt);
+        t = dd < 0.0 ? -t : t;
+        t = (apq == 0.0 || !(rt > 0.0)) ? 0.0 : t;
+        const double c = 1.0 / sqrt(1.0 + t * t), sn = t * c, hh = t * apq;
+        a[p * d + p] = app - hh;
+        a[q * d + q] = aqq + hh;
+        a[p * d + q] = a[q * d + p] = 0.0;
+        for (int r = 0; r < d; ++r) {
+          if (r == p || r == q) continue;
+          const double g = a[r * d + p], k = a[r * d + q];
+          a[r * d + p] = a[p * d + r] = c * g - sn * k;
+          a[r * d + q] = a[q * d + r] = sn * g + c * k;
+        }
+        for (int r = 0; r < d; ++r) {
+          const double g = v[r * d + p], k = v[r * d + q];
+          v[r * d + p] = c * g - sn * k;
+          v[r * d + q] = sn * g + c * k;
+        }
+      }
+  }
+  for (int k = 0; k < d; ++k) {
+    const double ev = a[k * d + k];
+    a[k * d + k] = method == SK_PSD_ABS ? fabs(ev) : method == SK_PSD_PROJ ? (ev < 1e-6 ? 1e-6 : ev) : ev;
+  }
+  double* out = Hout + o;
+  for (int i = 0; i < d; ++i)
+    for (int j = 0; j < d; ++j) {
+      double acc = 0.0;
+      for (int k = 0; k < d; ++k) acc += v[i * d + k] * a[k * d + k] * v[j * d + k];
+      out[i * d + j] = acc;
+    }
 }
 
 // Signed polar SVD per 3x3: U, s (3), V, R = U V^T (svd_rv.py:33-51, polar_svd.py:209-221).
@@ -238,11 +300,21 @@ int sk_psd_project(const double* H, int64_t n, int d, int method, double* out, s
     case 4: psd_project_kernel<4><<<g, 64, 0, st>>>(H, n, method, out); break;
     case 5: psd_project_kernel<5><<<g, 64, 0, st>>>(H, n, method, out); break;
     case 6: psd_project_kernel<6><<<g, 64, 0, st>>>(H, n, method, out); break;
+    case 7: psd_project_kernel<7><<<g, 64, 0, st>>>(H, n, method, out); break;
+    case 8: psd_project_kernel<8><<<g, 64, 0, st>>>(H, n, method, out); break;
     case 9: psd_project_kernel<9><<<g, 64, 0, st>>>(H, n, method, out); break;
+    case 10: psd_project_kernel<10><<<g, 64, 0, st>>>(H, n, method, out); break;
+    case 11: psd_project_kernel<11><<<g, 64, 0, st>>>(H, n, method, out); break;
     case 12: psd_project_kernel<12><<<g, 64, 0, st>>>(H, n, method, out); break;
-    default:
-      set_error("sk_psd_project: supported matrix sizes are 1-6, 9 and 12");
-      return SK_ERR_UNSUPPORTED;
+    default: {
+      SK_REQUIRE(d >= 1, "sk_psd_project: matrix size must be positive");
+      DevBuf<double> A, V;
+      SK_TRY(A.alloc((size_t)n * d * d));
+      SK_TRY(V.alloc((size_t)n * d * d));
+      psd_project_general_kernel<<<g, 64, 0, st>>>(H, n, d, method, A.p, V.p, out);
+      SK_HIP_CHECK(hipGetLastError());
+      SK_HIP_CHECK(hipStreamSynchronize(st));  // the scratch is freed on return
+    }
   }
   SK_HIP_CHECK(hipGetLastError());
   return SK_OK;

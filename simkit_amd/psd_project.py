@@ -12,7 +12,7 @@ def psd_project(H, method="proj"):
     """Floor eigenvalues at 1e-6 ('proj') or take |.| ('abs'); returns (n, d, d).
 
     Like the reference (numpy.linalg.eigh), only the lower triangle is read and
-    a 2D input is promoted to a batch of one. Matrix sizes 1..12 are supported.
+    a 2D input is promoted to a batch of one. Any matrix size is supported (up to 12 in registers).
     """
     H = np.asarray(H, dtype=np.float64)
     if H.ndim == 2:

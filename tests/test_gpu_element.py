@@ -68,7 +68,7 @@ def test_psd_project_vs_oracle(ref_element):
     d = ref_element
     assert per_item_rel_err(sk.psd_project(d["snh_H"]), d["snh_H_psd"]) < TOL
     rng = np.random.default_rng(14)
-    for dim in (2, 3, 9, 12):
+    for dim in (1, 2, 3, 7, 8, 9, 10, 11, 12, 13, 20):   # > 12: the scratch-memory Jacobi
         A = rng.standard_normal((300, dim, dim))
         A = A + A.transpose(0, 2, 1)
         for method in ("proj", "abs"):
@@ -80,10 +80,14 @@ def test_psd_project_vs_oracle(ref_element):
     F = random_F(15, 2000, 0.35, inverted=50)
     H = fem.snh_hessian_element_F(F, 3.4e5, 3.1e6) * 1.2e-6
     assert per_item_rel_err(sk.psd_project(H), fem.psd_project(H)) < TOL
-    # lower triangle only (as numpy.linalg.eigh)
-    B = A.copy()
-    B[:, np.triu_indices(12, 1)[0], np.triu_indices(12, 1)[1]] = 7.0
-    assert per_item_rel_err(sk.psd_project(B), fem.psd_project(A)) < TOL
+    # lower triangle only (as numpy.linalg.eigh), in registers (12) and in scratch memory (20)
+    for dim in (12, 20):
+        A = rng.standard_normal((100, dim, dim))
+        A = A + A.transpose(0, 2, 1)
+        B = A.copy()
+        iu = np.triu_indices(dim, 1)
+        B[:, iu[0], iu[1]] = 7.0
+        assert per_item_rel_err(sk.psd_project(B), fem.psd_project(A)) < TOL
 
 
 def test_rest_measures_and_kinematics():
