@@ -259,53 +259,64 @@ def run_path(args, X, T, bottom, top, mu, lam, name):
     return res
 
 
-def learned_leg(cells, steps, warmup):
-    """Learned warm start + FP32/FP64-refinement path (two-level PCG) on the same trajectory as the
-    headline, with the committed model for this mesh (trained on random top-face sinusoids, so this
-    trajectory is held out). None when no model exists for `cells`."""
+def learned_leg(cells, steps, warmup, preconditioner="bjacobi"):
+    """Learned warm start (boundary lift + the network's modal correction) on the FP32 + FP64-refinement
+    path, on the same trajectory as the headline, with the committed model for this mesh (trained on
+    random top-face sinusoids, so this trajectory is held out); also the lift without the network, to
+    isolate the network's share. None when no model exists for `cells`."""
     import torch
     from simkit_amd import learned
     model = learned.load_model(cells)
     if model is None:
         return None
     t0 = time.perf_counter()
-    sim, coords = learned.cube_problem(cells, precision="mixed", k=model.k, preconditioner="two_level")
+    sim, coords = learned.cube_problem(cells, precision="mixed", k=model.k, preconditioner=preconditioner)
     setup = time.perf_counter() - t0
     A, f = np.array([0.1, 0.0, 0.0]), np.array([1.0, 1.0, 1.0])   # d(t) = (0.1 sin 2 pi t, 0, 0)
-    ro = learned.Rollout(sim, coords, model)
-    ro.run(A, f, warmup)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    out = ro.run(A, f, steps, keep_last=True)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    infos = out["infos"]
-    used = [i for i in infos if i["guess_used"]]
-    res = {"steps_per_s": steps / dt, "ms_per_step": 1e3 * dt / steps,
-           "newton_iters": sum(i["newton_iters"] for i in infos), "pcg_iters": sum(i["pcg_iters"] for i in infos),
-           "guess_accepted": len(used) / len(infos),
-           "guess_g_ratio": float(np.exp(np.mean([np.log(i["guess_g_norm"] / i["g0_norm"]) for i in used])))
-           if used else None,
-           "setup_s_modes_and_lift": round(setup, 2), "modes": coords.modal_info,
-           "x_final": out["x"].cpu().numpy()}
-    log(f"bench[learned]: {dt:.2f}s, {res['steps_per_s']:.3f} steps/s, newton {res['newton_iters']}, "
-        f"pcg {res['pcg_iters']}, guess taken {res['guess_accepted']:.2f}")
-    return res
+    res = {}
+    for start in ("learned", "lift"):
+        ro = learned.Rollout(sim, coords, model if start == "learned" else None, start=start)
+        ro.run(A, f, warmup)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = ro.run(A, f, steps, keep_last=True)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        infos = out["infos"]
+        used = [i for i in infos if i["guess_used"]]
+        res[start] = {"steps_per_s": steps / dt, "ms_per_step": 1e3 * dt / steps,
+                      "newton_iters": sum(i["newton_iters"] for i in infos),
+                      "pcg_iters": sum(i["pcg_iters"] for i in infos),
+                      "guess_accepted": len(used) / len(infos),
+                      "guess_g_ratio": float(np.exp(np.mean([np.log(i["guess_g_norm"] / i["g0_norm"])
+                                                             for i in used]))) if used else None,
+                      "x_final": out["x"].cpu().numpy()}
+        log(f"bench[{start}/{preconditioner}]: {dt:.2f}s, {res[start]['steps_per_s']:.3f} steps/s, newton "
+            f"{res[start]['newton_iters']}, pcg {res[start]['pcg_iters']}, guess taken {res[start]['guess_accepted']:.2f}")
+    out = res["learned"]
+    out["setup_s_modes_and_lift"] = round(setup, 2)
+    out["modes"] = coords.modal_info
+    out["lift_only"] = {k: v for k, v in res["lift"].items() if k != "x_final"}
+    out["network_speedup_over_lift"] = round(out["steps_per_s"] / res["lift"]["steps_per_s"], 3)
+    return out
 
 
 def warmstart_eval_leg(cells, episodes, steps):
-    """Learned warm start on its own workload: held-out random sinusoids of the driven cube, plain
-    vs warm-started, with the committed model."""
+    """Learned warm start on its own workload: held-out random sinusoids of the driven cube from the plain
+    start x~, the boundary lift alone and the lift plus the network, with the committed model."""
     from simkit_amd import learned
     model = learned.load_model(cells)
     if model is None:
         return None
-    sim, coords = learned.cube_problem(cells, precision="mixed", k=model.k, preconditioner="two_level")
+    sim, coords = learned.cube_problem(cells, precision="mixed", k=model.k)
     res = learned.evaluate(sim, coords, model, episodes, steps, seed=2)
+    for v in res.values():
+        if isinstance(v, dict):
+            v.pop("newton_per_step_list", None)
     log(f"bench[warmstart{cells}]: speedup {res['speedup']:.3f}, newton/step {res['plain']['newton_per_step']:.2f}"
-        f" -> {res['learned']['newton_per_step']:.2f}")
+        f" -> lift {res['lift']['newton_per_step']:.2f} -> learned {res['learned']['newton_per_step']:.2f}")
     return {"workload": f"driven cube {cells}^3 cells, random top-face sinusoids (held out), stable neo-Hookean, "
-                        f"FP32 + FP64 refinement, two-level PCG", **res}
+                        f"FP32 + FP64 refinement, block-Jacobi PCG", **res}
 
 
 def mixed_jittered_leg(cells, steps):
@@ -519,18 +530,18 @@ def main():
                                               "aggregates (6 rigid motions each), coarse operator from the first "
                                               "Newton system")
     if not args.no_learned and head_name == "fp64":
-        lr = learned_leg(args.cells, args.steps, args.warmup)
-        if lr is not None:
+        for pc, key, ref in (("bjacobi", "learned_fp32", "mixed"), ("two_level", "learned_fp32_two_level",
+                                                                     "mixed_two_level")):
+            lr = learned_leg(args.cells, args.steps, args.warmup, pc)
+            if lr is None:
+                continue
             dev = float(np.abs(lr["x_final"] - head["x_final"]).max() / np.abs(head["x_final"]).max())
             lr.pop("x_final")
-            out["learned_fp32"] = {"value": round(lr["steps_per_s"], 4), "preconditioner": "two_level",
-                                   "speedup_vs_fp64": round(lr["steps_per_s"] / head["steps_per_s"], 3),
-                                   "speedup_vs_fp32_refined": round(lr["steps_per_s"] / results["mixed"]["steps_per_s"], 3)
-                                   if "mixed" in results else None,
-                                   "speedup_vs_fp32_two_level": round(
-                                       lr["steps_per_s"] / results["mixed_two_level"]["steps_per_s"], 3)
-                                   if "mixed_two_level" in results else None,
-                                   "final_position_rel_dev_vs_fp64": dev, **lr}
+            out[key] = {"value": round(lr["steps_per_s"], 4), "preconditioner": pc,
+                        "speedup_vs_fp64": round(lr["steps_per_s"] / head["steps_per_s"], 3),
+                        f"speedup_vs_{ref}": round(lr["steps_per_s"] / results[ref]["steps_per_s"], 3)
+                        if ref in results else None,
+                        "final_position_rel_dev_vs_fp64": dev, **lr}
         ws = warmstart_eval_leg(32, args.warmstart_episodes, 100)
         if ws is not None:
             out["learned_warm_start_32"] = ws

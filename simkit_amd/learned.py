@@ -13,13 +13,24 @@ mu = lam = 1, lumped mass with rho = 1, K + 1e-6 M) and L (3nv x 3) the static
 linear-elastic responses to a unit displacement of the driven face along
 each axis (the "lift", so that X + L d meets the boundary values).
 
-The model is an MLP (simkit_amd.mlp, BF16 matrix cores) that maps
-[q_curr, q_prev, 12 boundary-motion parameters] to the correction
-dq = q_next - q~, where q~ are the modal coordinates of the plain guess
-x~ = 2 x_curr - x_prev. The warm start is x~ + Phi dq: the plain guess with
-its modal part replaced by the prediction. sk_sim_step_guess takes it only
-when its incremental potential is lower than x~'s, and Newton still stops at
-||g|| <= rtol ||g(x~)||, so the answer meets the same tolerance either way.
+The warm start has two parts:
+
+* the boundary lift: x~ = 2 x_curr - x_prev extrapolates the driven face to
+  2 d_n - d_{n-1}, while the step prescribes d_{n+1}; the jump
+  dd = d_{n+1} - 2 d_n + d_{n-1} is what dominates the gradient at x~ (a
+  strain spike in the layer of elements under the face). x^ = x~ + L dd
+  spreads it through the mesh with the static response; this is linear
+  algebra, not learned;
+* the network: an MLP (simkit_amd.mlp, BF16 matrix cores) maps
+  [q_curr, q_prev, 12 boundary-motion parameters] to the modal correction
+  dq = q_next - q^, where q^ are the modal coordinates of x^. The guess is
+  x^ + Phi dq.
+
+sk_sim_step_guess takes the guess only when its incremental potential is
+lower than x~'s, and Newton still stops at ||g|| <= rtol ||g(x~)||, so the
+answer meets the same tolerance either way. Rollouts run with start "plain"
+(x~, the reference's start), "lift" (x^) or "learned" (x^ + Phi dq), so that
+the network's own share of the gain is measured apart from the lift's.
 
 The 12 parameters of a top-face motion d(t) = A sin(2 pi f t) (per axis) at
 the step's end time t are [A (3), f (3), sin(2 pi f t) (3), cos(2 pi f t) (3)].
@@ -181,15 +192,26 @@ class WarmStartModel:
 
 class Rollout:
     """Steps a BackwardEulerSim through a driven-face sinusoid, tracking modal
-    coordinates; with a model, every step is warm-started by it."""
+    coordinates. start: "plain" (Newton from x~), "lift" (from x^ = x~ + L dd when its
+    potential is lower) or "learned" (from x^ + Phi dq, dq predicted by the model); the default
+    is "learned" with a model, else "plain"."""
 
-    def __init__(self, sim, coords, model=None):
+    def __init__(self, sim, coords, model=None, start=None):
         self.sim, self.c, self.model = sim, coords, model
+        self.start = start or ("learned" if model is not None else "plain")
+        if self.start not in ("plain", "lift", "learned"):
+            raise ValueError("start must be 'plain', 'lift' or 'learned'")
+        if self.start == "learned" and model is None:
+            raise ValueError("start='learned' needs a model")
         n3 = 3 * coords.nv
         z = lambda: torch.empty(n3, dtype=torch.float64, device="cuda")
-        self.xt, self.guess, self.lift_next = z(), z(), z()
-        self.qt = torch.empty(coords.k, dtype=torch.float64, device="cuda")
+        self.xt, self.xhat, self.guess, self.lift_next = z(), z(), z(), z()
+        self.qhat = torch.empty(coords.k, dtype=torch.float64, device="cuda")
         self.feat = torch.empty(2 * coords.k + 12, dtype=torch.float64, device="cuda")
+        # per-step host scalars (12 motion parameters, the face jump dd, the face displacement d) in
+        # one pinned buffer; each step synchronizes the stream, so the buffer is free again at the next
+        self._host = torch.empty(18, dtype=torch.float64, pin_memory=True)
+        self._dev = torch.empty(18, dtype=torch.float64, device="cuda")
 
     def run(self, A, f, steps, record=False, keep_last=False):
         c, sim = self.c, self.sim
@@ -199,31 +221,43 @@ class Rollout:
         q_prev = torch.zeros(c.k, dtype=torch.float64, device="cuda")
         q_curr = torch.zeros_like(q_prev)
         feats, targets, infos = [], [], []
+        d_prev = d_curr = motion_d(A, f, 0.0)          # from rest: x_prev = x_curr, face at d(0)
         for n in range(steps):
             t = (n + 1) * h
             d = motion_d(A, f, t)
-            c.lift(d, out=self.lift_next)                      # also the boundary values
-            self.feat[:c.k].copy_(q_curr)
-            self.feat[c.k:2 * c.k].copy_(q_prev)
-            self.feat[2 * c.k:].copy_(torch.as_tensor(motion_params(A, f, t)))
-            if self.model is not None or record:
+            self._host[:12] = torch.from_numpy(motion_params(A, f, t))
+            self._host[12:15] = torch.from_numpy(d - (2.0 * d_curr - d_prev))
+            self._host[15:] = torch.from_numpy(d)
+            self._dev.copy_(self._host, non_blocking=True)
+            _expand(c.L, self._dev[15:], c.X, self.lift_next)  # X + L d: also the boundary values
+            if self.start != "plain" or record:
                 _axpby(2.0, x_curr, -1.0, x_prev, self.xt)
-                c.project(self.xt, self.lift_next, out=self.qt)
-            if self.model is not None:
-                dq = self.model.predict(self.feat)
-                c.add_modes(self.xt, dq, out=self.guess)
-                x_next, info = sim.step_guess(x_curr, x_prev, self.guess, self.lift_next)
-            else:
+                _expand(c.L, self._dev[12:15], self.xt, self.xhat)  # x^ = x~ + L dd
+            if self.start == "learned" or record:
+                self.feat[:c.k].copy_(q_curr)
+                self.feat[c.k:2 * c.k].copy_(q_prev)
+                self.feat[2 * c.k:].copy_(self._dev[:12])
+            if record:
+                c.project(self.xhat, self.lift_next, out=self.qhat)
+            if self.start == "plain":
                 x_next, info = sim.step(x_curr, x_prev, self.lift_next)
+            else:
+                if self.start == "learned":
+                    c.add_modes(self.xhat, self.model.predict(self.feat), out=self.guess)
+                    g = self.guess
+                else:
+                    g = self.xhat
+                x_next, info = sim.step_guess(x_curr, x_prev, g, self.lift_next)
             if info["status"] != 0:
                 raise RuntimeError(f"step {n} failed: {info}")
             q_next = c.project(x_next, self.lift_next)
             if record:
                 feats.append(self.feat.clone())
-                targets.append(q_next - self.qt)
+                targets.append(q_next - self.qhat)
             infos.append(info)
             x_prev, x_curr = x_curr, x_next
             q_prev, q_curr = q_curr, q_next
+            d_prev, d_curr = d_curr, d
         out = {"infos": infos}
         if record:
             out["F"] = torch.stack(feats)
@@ -270,7 +304,7 @@ def cube_problem(n, E=1e6, nu=0.45, rho=1000.0, h=1.0 / 60.0, precision="mixed",
 def generate(sim, coords, episodes, steps, seed, log=None):
     """Solver-generated training data: `episodes` random top-face sinusoids from rest."""
     rng = np.random.default_rng(seed)
-    ro = Rollout(sim, coords)
+    ro = Rollout(sim, coords, start="lift")
     F, D = [], []
     for ep in range(episodes):
         A, f = random_motion(rng)
@@ -284,14 +318,15 @@ def generate(sim, coords, episodes, steps, seed, log=None):
     return torch.cat(F), torch.cat(D)
 
 
-def evaluate(sim, coords, model, episodes, steps, seed):
-    """Held-out episodes run plain and warm-started; Newton/PCG counts, wall time and agreement."""
+def evaluate(sim, coords, model, episodes, steps, seed, starts=("plain", "lift", "learned")):
+    """Held-out episodes run from each start ("plain": x~; "lift": x~ + L dd; "learned": the lift plus
+    the network's modal correction): Newton/PCG counts, wall time and agreement with the plain run."""
     rng = np.random.default_rng(seed)
     motions = [random_motion(rng) for _ in range(episodes)]
     res = {}
     finals = {}
-    for name, mdl in (("plain", None), ("learned", model)):
-        ro = Rollout(sim, coords, mdl)
+    for name in starts:
+        ro = Rollout(sim, coords, model if name == "learned" else None, start=name)
         ro.run(*motions[0], 2)                                 # warm-up
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -306,18 +341,28 @@ def evaluate(sim, coords, model, episodes, steps, seed):
         res[name] = {"steps": n, "seconds": dt, "steps_per_s": n / dt,
                      "newton_per_step": sum(i["newton_iters"] for i in infos) / n,
                      "pcg_per_step": sum(i["pcg_iters"] for i in infos) / n,
-                     "ls_per_step": sum(i["ls_trials"] for i in infos) / n}
-        if name == "learned":
+                     "ls_per_step": sum(i["ls_trials"] for i in infos) / n,
+                     "newton_per_step_list": [int(i["newton_iters"]) for i in infos]}
+        if name != "plain":
             used = [i for i in infos if i["guess_used"]]
             res[name]["guess_accepted"] = len(used) / n
             # gradient norm at the taken guess relative to the plain start x~ (geometric mean)
             res[name]["guess_g_ratio"] = float(np.exp(np.mean([np.log(i["guess_g_norm"] / i["g0_norm"])
                                                                for i in used]))) if used else None
         finals[name] = xs
-    dev_ = max(np.abs(a - b).max() / np.abs(b).max() for a, b in zip(finals["learned"], finals["plain"]))
-    res["final_position_rel_dev"] = float(dev_)
-    res["speedup"] = res["learned"]["steps_per_s"] / res["plain"]["steps_per_s"]
-    res["newton_reduction"] = res["plain"]["newton_per_step"] / res["learned"]["newton_per_step"]
+    for name in starts:
+        if name != "plain" and "plain" in finals:
+            res[name]["final_position_rel_dev"] = float(max(np.abs(a - b).max() / np.abs(b).max()
+                                                            for a, b in zip(finals[name], finals["plain"])))
+    if "plain" in res and "learned" in res:
+        res["speedup"] = res["learned"]["steps_per_s"] / res["plain"]["steps_per_s"]
+        res["newton_reduction"] = res["plain"]["newton_per_step"] / res["learned"]["newton_per_step"]
+        res["final_position_rel_dev"] = res["learned"]["final_position_rel_dev"]
+    if "lift" in res and "learned" in res:
+        # the network's own share: learned against the lifted start without it
+        res["network_speedup_over_lift"] = res["learned"]["steps_per_s"] / res["lift"]["steps_per_s"]
+        res["network_newton_reduction_over_lift"] = (res["lift"]["newton_per_step"]
+                                                     / res["learned"]["newton_per_step"])
     return res
 
 
