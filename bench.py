@@ -191,9 +191,12 @@ def cpu_baseline(X, T, bottom, top, mu, lam, gpu_infos, steps, pcg_sample, gpu10
     }
 
 
-# solver paths: name -> (precision, PCG preconditioner)
-PATHS = {"fp64": ("fp64", "bjacobi"), "mixed": ("mixed", "bjacobi"),
-         "fp64_two_level": ("fp64", "two_level"), "mixed_two_level": ("mixed", "two_level")}
+# solver paths: name -> (precision, PCG preconditioner, PCG form, PSD eigen-solver)
+PATHS = {"fp64": ("fp64", "bjacobi", "classic", "closed_form"), "mixed": ("mixed", "bjacobi", "classic", "closed_form"),
+         "fp64_pipelined": ("fp64", "bjacobi", "pipelined", "closed_form"),
+         "fp64_jacobi": ("fp64", "bjacobi", "classic", "jacobi"), "mixed_jacobi": ("mixed", "bjacobi", "classic", "jacobi"),
+         "fp64_two_level": ("fp64", "two_level", "classic", "closed_form"),
+         "mixed_two_level": ("mixed", "two_level", "classic", "closed_form")}
 
 
 def run_path(args, X, T, bottom, top, mu, lam, name):
@@ -201,11 +204,12 @@ def run_path(args, X, T, bottom, top, mu, lam, name):
     import torch
     from simkit_amd import _lib
     from simkit_amd.sim import BackwardEulerSim, top_face_sinusoid
-    precision, pc = PATHS[name]
+    precision, pc, form, psd_solver = PATHS[name]
     t_setup = time.perf_counter()
     sim = BackwardEulerSim(X, T, mu, lam, rho=1000.0, h=1.0 / 60.0, material="snh",
                            fixed_vertices=np.concatenate([bottom, top]), pcg_rtol=1e-8, newton_rtol=1e-6,
-                           timing=True, precision=precision, inner_rtol=args.inner_rtol, preconditioner=pc)
+                           timing=True, precision=precision, inner_rtol=args.inner_rtol, preconditioner=pc,
+                           pcg=form, psd_solver=psd_solver)
     bc = top_face_sinusoid(X, top, 0.1, 1.0)
     torch.cuda.synchronize()
     log(f"bench[{name}]: setup {time.perf_counter() - t_setup:.1f}s; warmup {args.warmup} steps")
@@ -254,7 +258,8 @@ def run_path(args, X, T, bottom, top, mu, lam, name):
         "coarse": sim.coarse_info() if pc == "two_level" else None,
     }
     log(f"bench[{name}]: {elapsed:.2f}s, {res['steps_per_s']:.3f} steps/s, newton {res['newton_iters']}, "
-        f"pcg {res['pcg_iters']}, spmv {spmv_ms * 1e3:.1f} us")
+        f"pcg {res['pcg_iters']}, spmv {spmv_ms * 1e3:.1f} us, element pass {hess_ms * 1e3:.1f} us, "
+        f"phases {res['phase_ms_total']}")
     del sim
     return res
 

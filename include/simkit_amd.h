@@ -143,7 +143,9 @@ int sk_mesh_energy(sk_mesh* mesh, int material, const double* x, const double* F
 int sk_mesh_gradient(sk_mesh* mesh, int material, const double* x, const double* F0, const double* mu, const double* lam,
                      const double* vol, double* g, sk_stream_t stream);
 
-/* CSR values (nnz, pattern of sk_mesh_csr_pattern) of sum_e B_e^T psd(vol_e H_e) B_e.
+/* CSR values (nnz, pattern of sk_mesh_csr_pattern) of sum_e B_e^T psd(vol_e H_e) B_e
+ * (stable neo-Hookean: psd from the Hessian's closed-form eigensystem, other densities and
+ * near-singular F by the 9x9 Jacobi).
  * psd: SK_PSD_NONE / SK_PSD_PROJ / SK_PSD_ABS. Replaces stable_neo_hookean_hessian_x
  * (:506-538) / arap_hessian_x (arap.py:295-325) including psd_project and the
  * block_diag / J.T @ H @ J assembly. */
@@ -231,6 +233,14 @@ typedef struct sk_sim_params {
   int max_inverted;      /* < 0: no limit; otherwise a step whose x_next has more elements with det F <= 0
                             returns SK_ERR_INVERTED (x_next is still written). The reference's densities
                             accept inverted elements, so the default is no limit. */
+  int pcg_variant;       /* block-Jacobi PCG: 0 classic (two launches per iteration: A p, then the vector
+                            update), 1 pipelined (one launch per iteration, Ghysels-Vanroose recurrences on
+                            A B^-1 formed once per Newton system), SK_PREC_FP64 only: the two-level
+                            preconditioner and the mixed path (whose residual replacement breaks the
+                            pipelined recurrences in FP32) run the classic form */
+  int psd_jacobi;        /* 1: PSD projection by the in-register 9x9 Jacobi for every density; 0: from the
+                            closed-form Hessian eigensystem where the density has one (stable neo-Hookean;
+                            elements with F near singular still take the Jacobi) */
 } sk_sim_params;
 
 typedef struct sk_step_info {

@@ -73,6 +73,8 @@ class SkSimParams(ctypes.Structure):
         ("update_blocks", ctypes.c_int),
         ("pcg_chunk", ctypes.c_int),
         ("max_inverted", ctypes.c_int),
+        ("pcg_variant", ctypes.c_int),
+        ("psd_jacobi", ctypes.c_int),
     ]
 
 

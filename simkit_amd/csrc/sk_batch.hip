@@ -1203,7 +1203,7 @@ int hessian_phase(sk_batch* B, bool only_active) {
   const int* act = only_active ? B->active.p : nullptr;
   const int32_t* pid = only_active ? B->pid.p : nullptr;
   SK_TRY(launch_elem_hess(m, B->prm.material, B->x.p, nullptr, B->mu.p, B->lam.p, B->vol.p, B->prm.psd, true,
-                          B->st, act, pid));
+                          B->st, act, pid, nullptr, true));
   return launch_gather_hess_quad(m, B->vals32.p, B->mass.p, 1.0 / (B->prm.h * B->prm.h), B->fixed.p,
                                  B->binv32.p, B->wpos.p, B->wrow.p, true, B->st, act, pid);
 }

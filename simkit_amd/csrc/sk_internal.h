@@ -126,6 +126,7 @@ struct sk_mesh {
   int64_t lpad = 0;
   sk::DevBuf<double> Kst;     // [nt][90] staged element stiffness blocks
   sk::DevBuf<double> fst;     // [nt][12] staged element forces
+  sk::DevBuf<uint8_t> eflag;  // [nt] element pass: 1 where the closed-form eigensystem was not used
   sk::Reducer red;
   double* pinned = nullptr;   // 16 host-pinned doubles for scalar readback
 };
@@ -146,10 +147,14 @@ int launch_elem_grad(sk_mesh* m, int material, const double* x, const double* F0
 // element's first vertex) has active == 0 (batched problems).
 // fp64_count (optional): += elements whose FP32 Hessian of a polar density
 // (ARAP, FCR) was evaluated in FP64 because det F < 1e-3.
+// closed_form: densities with a closed-form Hessian eigensystem (stable
+// neo-Hookean) modify eigenvalues from it instead of the 9x9 Jacobi when a
+// PSD method is given (two launches; elements whose F is too close to
+// singular for the SVD fall back to the Jacobi in the second).
 int launch_elem_hess(sk_mesh* m, int material, const double* x, const double* F0, const double* mu,
                      const double* lam, const double* vol, int psd, bool fp32, hipStream_t st,
                      const int* active = nullptr, const int32_t* pid = nullptr,
-                     unsigned long long* fp64_count = nullptr);
+                     unsigned long long* fp64_count = nullptr, bool closed_form = false);
 // *result (device) = number of elements with det F(x) <= 0.
 int launch_count_inverted(sk_mesh* m, const double* x, double* result, hipStream_t st);
 // Gathers staged element forces into g; with mass != nullptr adds

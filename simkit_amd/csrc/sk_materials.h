@@ -58,6 +58,7 @@ __device__ __forceinline__ void cofactor_form_hessian(const R (&F)[9], const R (
 template <typename R>
 struct SNH {
   static constexpr bool kPolar = false;  // no polar decomposition in the density
+  static constexpr bool kAnalyticEig = true;  // eigenvalue modification by hessian_psd_analytic
   struct Scal {
     R IC, J, alpha, cof[9];
   };
@@ -89,6 +90,85 @@ struct SNH {
     const R den = s.IC + R(1);
     cofactor_form_hessian(F, s.cof, mu * s.IC / den, R(2) * mu / (den * den), lam, lam * (s.J - s.alpha), H);
   }
+  // vol H with its eigenvalues e modified as psd_project does (method 1: max(e, floor); 2: |e|), from
+  // the Hessian's closed-form eigensystem in the signed SVD F = U diag(s) V^T (U, V rotations):
+  // with H = a I + b vecF vecF^T + c vec(cof) vec(cof)^T + d dvec(cof)/dvecF,
+  //   twist  vec(U (e_i e_j^T - e_j e_i^T) V^T)/sqrt2: a + d s_k,
+  //   flip   vec(U (e_i e_j^T + e_j e_i^T) V^T)/sqrt2: a - d s_k   ({i, j, k} = {0, 1, 2}),
+  //   scaling vec(U diag(w) V^T), w an eigenvector of the 3x3 block
+  //   a I + b s s^T + c t t^T + d S, t = (s1 s2, s0 s2, s0 s1), S_ij = s_k (i != j), S_ii = 0.
+  // H itself is formed as in hessian() and only the modified modes are added, (m(e) - e) v v^T.
+  // Returns false (H not written) when the smallest singular value is below 1e-2 of the largest:
+  // the SVD (Jacobi on F^T F) then resolves U and V too coarsely for FP64 parity, and the caller
+  // projects by the 9x9 Jacobi instead.
+  static __device__ __forceinline__ bool hessian_psd_analytic(const R (&F)[9], R mu, R lam, R vol, int method,
+                                                              R floor_, R (&H)[45]) {
+    R U[9], sv[3], V[9];
+    svd3_signed(F, U, sv, V);
+    const R amax = fmax(fabs(sv[0]), fmax(fabs(sv[1]), fabs(sv[2])));
+    const R amin = fmin(fabs(sv[0]), fmin(fabs(sv[1]), fabs(sv[2])));
+    if (!(amin >= R(1e-2) * amax)) return false;  // also for F = 0 and NaN
+    const Scal s = scalars(F, mu, lam);
+    const R den = s.IC + R(1);
+    const R a = mu * s.IC / den, b = R(2) * mu / (den * den), c = lam, d = lam * (s.J - s.alpha);
+    cofactor_form_hessian(F, s.cof, a, b, c, d, H);
+#pragma unroll
+    for (int k = 0; k < 45; ++k) H[k] *= vol;
+    auto modified = [&](R e) { return method == 2 ? fabs(e) : (e < floor_ ? floor_ : e); };
+    auto add_mode = [&](R cm, const R (&v)[9]) {
+#pragma unroll
+      for (int p = 0; p < 9; ++p)
+#pragma unroll
+        for (int q = p; q < 9; ++q) H[Sym<9>::idx(p, q)] += cm * v[p] * v[q];
+    };
+    const R inv_sqrt2 = R(0.70710678118654752440);
+    const int pi[3] = {0, 1, 0}, pj[3] = {1, 2, 2}, pk[3] = {2, 0, 1};
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const int i = pi[m], j = pj[m], k = pk[m];
+#pragma unroll
+      for (int sgn = 0; sgn < 2; ++sgn) {  // 0: twist, 1: flip
+        const R e = vol * (sgn == 0 ? a + d * sv[k] : a - d * sv[k]);
+        const R cm = modified(e) - e;
+        if (cm != R(0)) {
+          R v[9];
+#pragma unroll
+          for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) {
+              const R t1 = U[r * 3 + i] * V[cc * 3 + j], t2 = U[r * 3 + j] * V[cc * 3 + i];
+              v[r * 3 + cc] = (sgn == 0 ? t1 - t2 : t1 + t2) * inv_sqrt2;
+            }
+          add_mode(cm, v);
+        }
+      }
+    }
+    const R t3[3] = {sv[1] * sv[2], sv[0] * sv[2], sv[0] * sv[1]};
+    R A3[6], W[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = i; j < 3; ++j)
+        A3[Sym<3>::idx(i, j)] =
+            vol * ((i == j ? a : d * sv[3 - i - j]) + b * sv[i] * sv[j] + c * t3[i] * t3[j]);
+    jacobi_eigh<3, R>(A3, W, 20);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const R e = A3[Sym<3>::idx(m, m)];
+      const R cm = modified(e) - e;
+      if (cm != R(0)) {
+        R v[9];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int cc = 0; cc < 3; ++cc)
+            v[r * 3 + cc] = W[0 * 3 + m] * U[r * 3 + 0] * V[cc * 3 + 0] + W[1 * 3 + m] * U[r * 3 + 1] * V[cc * 3 + 1] +
+                            W[2 * 3 + m] * U[r * 3 + 2] * V[cc * 3 + 2];
+        add_mode(cm, v);
+      }
+    }
+    return true;
+  }
 };
 
 // ARAP: psi = mu/2 ||F - R||^2, P = mu (F - R), H = mu (I - dR/dF) with R the
@@ -97,6 +177,7 @@ struct SNH {
 // (arap.py:86-145, polar_svd.py:209-218, rotation_gradient.py:42-72).
 template <typename R>
 struct ARAP {
+  static constexpr bool kAnalyticEig = false;
   static constexpr bool kPolar = true;
   static __device__ __forceinline__ void rotation(const R (&U)[9], const R (&V)[9], R (&Rm)[9]) {
 #pragma unroll
@@ -158,6 +239,7 @@ struct ARAP {
 // (linear_elasticity.py: energy, gradient and hessian element tiers).
 template <typename R>
 struct Linear {
+  static constexpr bool kAnalyticEig = false;
   static constexpr bool kPolar = false;
   static __device__ __forceinline__ R energy(const R (&F)[9], R mu, R lam) {
     R acc = 0;
@@ -200,6 +282,7 @@ struct Linear {
 //   H = mu I + lam vec(cof) vec(cof)^T + (lam (J - 1) - mu) dvec(cof)/dvecF
 template <typename R>
 struct MMNH {
+  static constexpr bool kAnalyticEig = false;
   static constexpr bool kPolar = false;
   static __device__ __forceinline__ R energy(const R (&F)[9], R mu, R lam) {
     R cof[9];
@@ -233,6 +316,7 @@ struct MMNH {
 //   H = 2 mu (I - dR/dF) + lam vec(cof) vec(cof)^T + lam (J - 1) dvec(cof)/dvecF
 template <typename R>
 struct FCR {
+  static constexpr bool kAnalyticEig = false;
   static constexpr bool kPolar = true;
   static __device__ __forceinline__ R energy(const R (&F)[9], R mu, R lam) {
     R cof[9];

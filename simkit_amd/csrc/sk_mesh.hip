@@ -97,6 +97,7 @@ struct ElemArgs {
   const int* active;    // optional (batched problems): skip elements whose problem is inactive
   const int32_t* pid;   // problem of each vertex, with `active`
   unsigned long long* fp64_count;  // optional: += elements whose FP32 Hessian took the FP64 polar branch
+  uint8_t* flags;       // closed-form eigensystem passes: 1 where the element falls back to the Jacobi pass
 };
 
 template <class M>
@@ -149,11 +150,18 @@ __device__ __forceinline__ void wave_count(bool flag, unsigned long long* count)
   if (flag && (int)__lane_id() == __ffsll((long long)m) - 1) atomicAdd(count, (unsigned long long)__popcll(m));
 }
 
-template <template <typename> class M, int PSD, typename R>
+// kMode 0: Hessian, vol weight and 9x9 Jacobi PSD projection for every
+// element. kMode 1 (densities with a closed-form eigensystem, PSD on): the
+// eigenvalue modification from that eigensystem; elements it declines are
+// flagged and left to kMode 2, which runs the kMode 0 path on flagged
+// elements only. Splitting the two paths over two kernels keeps the
+// closed-form kernel's registers (and occupancy) free of the 9x9 Jacobi's.
+template <template <typename> class M, int PSD, typename R, int kMode = 0>
 __global__ __launch_bounds__(kBlock) void elem_hess_kernel(ElemArgs a) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= a.m.nt) return;
   if (a.active && !a.active[a.pid[a.m.T[e]]]) return;
+  if (kMode == 2 && !a.flags[e]) return;
   int32_t v[4];
   double Dd[12], xe[12], F[9];
   load_elem(a.m, e, v, Dd);
@@ -161,12 +169,22 @@ __global__ __launch_bounds__(kBlock) void elem_hess_kernel(ElemArgs a) {
   make_F(xe, Dd, F);
   add_F0(a.F0, e, F);
   R H[45];
-  const bool fp64_polar = elem_hessian_eval<M, R>(F, a.mu[e], a.lam ? a.lam[e] : 0.0, H);
-  if (sizeof(R) == 4 && M<R>::kPolar && a.fp64_count) wave_count(fp64_polar, a.fp64_count);
-  const R vol = (R)a.vol[e];
+  if constexpr (kMode == 1) {
+    R Fr[9];
 #pragma unroll
-  for (int k = 0; k < 45; ++k) H[k] *= vol;
-  if (PSD != SK_PSD_NONE) psd_project_sym<9, R>(H, PSD, R(1e-6));
+    for (int k = 0; k < 9; ++k) Fr[k] = (R)F[k];
+    const bool ok = M<R>::hessian_psd_analytic(Fr, (R)a.mu[e], a.lam ? (R)a.lam[e] : R(0), (R)a.vol[e], PSD,
+                                               R(1e-6), H);
+    a.flags[e] = ok ? 0 : 1;
+    if (!ok) return;
+  } else {
+    const bool fp64_polar = elem_hessian_eval<M, R>(F, a.mu[e], a.lam ? a.lam[e] : 0.0, H);
+    if (sizeof(R) == 4 && M<R>::kPolar && a.fp64_count) wave_count(fp64_polar, a.fp64_count);
+    const R vol = (R)a.vol[e];
+#pragma unroll
+    for (int k = 0; k < 45; ++k) H[k] *= vol;
+    if (PSD != SK_PSD_NONE) psd_project_sym<9, R>(H, PSD, R(1e-6));
+  }
   R D[12];
 #pragma unroll
   for (int k = 0; k < 12; ++k) D[k] = (R)Dd[k];
@@ -465,6 +483,18 @@ int launch_elem_grad(sk_mesh* m, int material, const double* x, const double* F0
 
 template <template <typename> class M, typename R>
 static void hess_dispatch(int psd, int g, hipStream_t st, const ElemArgs& a) {
+  if constexpr (M<R>::kAnalyticEig) {
+    if (a.flags && psd != SK_PSD_NONE) {
+      if (psd == SK_PSD_PROJ) {
+        elem_hess_kernel<M, SK_PSD_PROJ, R, 1><<<g, kBlock, 0, st>>>(a);
+        elem_hess_kernel<M, SK_PSD_PROJ, R, 2><<<g, kBlock, 0, st>>>(a);
+      } else {
+        elem_hess_kernel<M, SK_PSD_ABS, R, 1><<<g, kBlock, 0, st>>>(a);
+        elem_hess_kernel<M, SK_PSD_ABS, R, 2><<<g, kBlock, 0, st>>>(a);
+      }
+      return;
+    }
+  }
   if (psd == SK_PSD_PROJ) elem_hess_kernel<M, SK_PSD_PROJ, R><<<g, kBlock, 0, st>>>(a);
   else if (psd == SK_PSD_ABS) elem_hess_kernel<M, SK_PSD_ABS, R><<<g, kBlock, 0, st>>>(a);
   else elem_hess_kernel<M, SK_PSD_NONE, R><<<g, kBlock, 0, st>>>(a);
@@ -478,10 +508,11 @@ static void hess_dispatch_prec(bool fp32, int psd, int g, hipStream_t st, const 
 
 int launch_elem_hess(sk_mesh* m, int material, const double* x, const double* F0, const double* mu,
                      const double* lam, const double* vol, int psd, bool fp32, hipStream_t st, const int* active,
-                     const int32_t* pid, unsigned long long* fp64_count) {
+                     const int32_t* pid, unsigned long long* fp64_count, bool closed_form) {
   SK_REQUIRE(psd == SK_PSD_NONE || psd == SK_PSD_PROJ || psd == SK_PSD_ABS, "invalid psd method");
   SK_REQUIRE(!active == !pid, "launch_elem_hess: active and pid go together");
-  ElemArgs a{view(m), x, F0, mu, lam, vol, m->fst.p, m->Kst.p, psd, active, pid, fp64_count};
+  ElemArgs a{view(m), x, F0, mu, lam, vol, m->fst.p, m->Kst.p, psd, active, pid, fp64_count,
+             closed_form ? m->eflag.p : nullptr};
   const int g = grid_for(m->nt);
   bool known = true;
   switch (material) {
@@ -705,6 +736,7 @@ static int build_mesh(const int32_t* T, int64_t nv, int64_t nt, sk_mesh* m) {
   SK_TRY(m->D.alloc(12 * (size_t)nt));
   SK_TRY(m->Kst.alloc((size_t)kKst * nt));
   SK_TRY(m->fst.alloc((size_t)kFst * nt));
+  SK_TRY(m->eflag.alloc((size_t)nt));
   SK_TRY(m->red.init());
   SK_HIP_CHECK(hipHostMalloc((void**)&m->pinned, 16 * sizeof(double), hipHostMallocDefault));
   return SK_OK;
@@ -829,7 +861,7 @@ int sk_mesh_hessian(sk_mesh* m, int material, const double* x, const double* F0,
   SK_REQUIRE(m && x && mu && vol && values, "sk_mesh_hessian: null argument");
   SK_REQUIRE(!material_needs_lam(material) || lam, "sk_mesh_hessian: this material needs lam");
   hipStream_t st = as_stream(s);
-  SK_TRY(launch_elem_hess(m, material, x, F0, mu, lam, vol, psd, false, st));
+  SK_TRY(launch_elem_hess(m, material, x, F0, mu, lam, vol, psd, false, st, nullptr, nullptr, nullptr, true));
   SK_TRY(launch_gather_hess(m, values, nullptr, 0.0, nullptr, nullptr, kLayoutCSR, false, st));
   return SK_OK;
 }

@@ -895,6 +895,327 @@ __global__ void ir_accumulate_kernel(int64_t n, const float* d, double* x, int f
 }
 
 // ---------------------------------------------------------------------------
+// Pipelined block-Jacobi PCG: one launch per iteration.
+//
+// CG needs two grid-wide reductions per iteration (p.Ap, then r.z), so the
+// classic form above runs two dependent launches. The pipelined recurrences
+// of Ghysels & Vanroose (Parallel Computing 40, 2014), with the block-Jacobi
+// inverse B^-1 folded into the matrix as Ah = A B^-1 (block columns scaled
+// once per Newton system), need one reduction phase of three dot products:
+//   u = B^-1 r (never stored), w = Ah r = A u,
+//   gamma = r.u, delta = w.u, beta = gamma / gamma_prev,
+//   alpha = gamma / (delta - beta gamma / alpha_prev)   (= gamma / p.Ap),
+//   t = Ah w + beta t   (= A B^-1 s),  s = w + beta s   (= A p),
+//   p = u + beta p,  x += alpha p,  r -= alpha s,  w -= alpha t.
+// Launch k reduces (gamma_k, delta_k, r.r) written by launch k - 1, forms
+// alpha_k and beta_k, multiplies Ah w_k (w is double-buffered: the product
+// gathers w_k at neighbouring rows that other workgroups update in the same
+// launch), updates its own rows and writes the partials of iteration k + 1.
+// Exact arithmetic gives the classic iterates; the stopping test is on the
+// recurrence residual as in the classic form.
+// ---------------------------------------------------------------------------
+constexpr int kPipeBlock = 512;                   // 8 waves: fewer partials for every prologue to sum
+constexpr int kPipeWaves = kPipeBlock / kWave;
+
+template <typename T>
+struct PipeArgs {
+  int64_t nv;
+  SellView sv;
+  const T* vhat;       // Ah = A B^-1, the SELL layout of the matrix
+  const T* binv;       // fixed-DOF-masked B^-1, 9 per vertex
+  const uint8_t* fixed;
+  const T* b;          // right-hand side (init)
+  T *x, *r, *s, *t, *p;
+  const T* win;        // w_k
+  T* wout;             // w_k+1
+  PcgState* st;
+  double* ab;          // alpha of launch k at ab[k & 1]
+  double rtol2;
+  const double* pin;   // [3][np] gamma, delta, r.r partials of the previous launch
+  double* pout;        // [3][gridDim.x] this launch's
+  int np;
+  int k;
+  int restart;         // first iteration after a residual replacement (tolerance from this residual)
+  PcgState* hst;
+};
+
+// Sum over a kPipeBlock-thread block of NV values (fixed order: wave trees,
+// then a tree over the wave sums in wave 0), result in every thread. The second
+// stage keeps one value per lane rather than all 16 x NV in one thread.
+template <int NV>
+__device__ __forceinline__ void pipe_block_sum(double (&v)[NV]) {
+  __shared__ double lds[NV * kPipeWaves];
+  __shared__ double res[NV];
+  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) lds[k * kPipeWaves + wid] = v[k];
+  }
+  __syncthreads();
+  if (wid == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      const double t = wave_sum(lane < kPipeWaves ? lds[k * kPipeWaves + lane] : 0.0);
+      if (lane == 0) res[k] = t;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = res[k];
+}
+
+// The wave's chunk entry (chunk, slots, offset) in the XCD-balanced order; chunk -1: none.
+__device__ __forceinline__ int4 pipe_chunk(const SellView& sv) {
+  const int wid = threadIdx.x / kWave;
+  const int64_t pos = (int64_t)(blockIdx.x / 8) * kPipeWaves + wid;
+  int4 e = sv.cmeta[(blockIdx.x % 8) * sv.lpad + (pos < sv.lpad ? pos : sv.lpad - 1)];
+  if (pos >= sv.lpad) e.x = -1;
+  return e;
+}
+
+// y = (Ah v)_row for the lane's row, v(w) read through `get`; summed over the
+// row's kSellLanes lanes and returned to all of them.
+template <typename T, typename G>
+__device__ __forceinline__ void pipe_row_product(const SellView& sv, const T* __restrict__ vals, int4 e, int i, int j,
+                                                 G get, T (&y)[3]) {
+  const int32_t deg = e.y;
+  const int64_t off = e.z;
+#pragma unroll 1
+  for (int32_t sl = j; sl < deg; sl += kSellLanes) {
+    const int64_t g = off + (int64_t)sl * kSellC;
+    const int64_t w = 3 * (int64_t)sv.scol[g + i];
+    T v[3];
+    get(w, v);
+    T A[9];
+    load_slot<T, kSellC, 1>(vals, g, i, A);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) y[q] += A[3 * q] * v[0] + A[3 * q + 1] * v[1] + A[3 * q + 2] * v[2];
+  }
+#pragma unroll
+  for (int o = kSellC; o < kWave; o <<= 1) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) y[q] += __shfl_xor(y[q], o, kWave);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void pipe_apply_binv(const T* __restrict__ binv, int64_t v, const T (&r)[3], T (&u)[3]) {
+  const T* B = binv + 9 * v;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) u[q] = B[3 * q] * r[0] + B[3 * q + 1] * r[1] + B[3 * q + 2] * r[2];
+}
+
+template <typename T>
+__device__ __forceinline__ void pipe_store_partials(const PipeArgs<T>& a, double (&v)[3]) {
+  pipe_block_sum<3>(v);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) a.pout[q * gridDim.x + blockIdx.x] = v[q];
+  }
+}
+
+// Start (or restart after a residual replacement) of a pipelined run: r = b
+// masked (init) or the replaced residual already in r (restart), w = Ah r,
+// partials of gamma = r.B^-1 r, delta = w.B^-1 r and r.r. The init also
+// zeroes x, s, t, p; a restart keeps s, t, p (the search direction) and
+// zeroes x (the correction accumulator).
+template <typename T>
+__global__ __launch_bounds__(kPipeBlock) void ppcg_start_kernel(PipeArgs<T> a) {
+  const int lane = threadIdx.x % kWave, i = lane % kSellC, j = lane / kSellC;
+  const int4 e = pipe_chunk(a.sv);
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (e.x >= 0) {
+    const T* __restrict__ src = a.restart ? a.r : a.b;
+    const uint8_t* __restrict__ fx = a.fixed;
+    T y[3] = {T(0), T(0), T(0)};
+    pipe_row_product<T>(a.sv, a.vhat, e, i, j, [&](int64_t w, T (&v)[3]) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) v[q] = (fx && fx[w + q]) ? T(0) : src[w + q];
+    }, y);
+    const int32_t row = a.sv.sell_row[(int64_t)e.x * kSellC + i];
+    if (row >= 0 && j == 0) {
+      const int64_t v0 = 3 * (int64_t)row;
+      T r[3], u[3];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const bool f = fx && fx[v0 + q];
+        r[q] = f ? T(0) : src[v0 + q];
+        y[q] = f ? T(0) : y[q];
+      }
+      pipe_apply_binv(a.binv, row, r, u);
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const int64_t k = v0 + q;
+        a.r[k] = r[q];
+        a.wout[k] = y[q];
+        a.x[k] = T(0);
+        if (!a.restart) {
+          a.s[k] = T(0);
+          a.t[k] = T(0);
+          a.p[k] = T(0);
+        }
+        acc[0] += (double)r[q] * (double)u[q];
+        acc[1] += (double)y[q] * (double)u[q];
+        acc[2] += (double)r[q] * (double)r[q];
+      }
+    }
+  }
+  pipe_store_partials(a, acc);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.st->done = 0;
+    if (!a.restart) a.st->iter = 0;
+    if (a.hst) {
+      a.hst->done = 0;
+      if (!a.restart) a.hst->iter = 0;
+    }
+  }
+}
+
+// Iteration k of a pipelined run (see above).
+template <typename T>
+__global__ __launch_bounds__(kPipeBlock, 8) void ppcg_iter_kernel(PipeArgs<T> a) {
+  __shared__ int s_done;
+  __shared__ double s_red[3 * kPipeWaves];
+  __shared__ double s_ab[3];
+  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave, i = lane % kSellC, j = lane / kSellC;
+  // prologue loads: the stop flag, this thread's partials, alpha and gamma of launch k - 1
+  const int done0 = a.st->done;
+  double part[3] = {0.0, 0.0, 0.0};
+  for (int q = (int)threadIdx.x; q < a.np; q += kPipeBlock) {
+    part[0] += a.pin[q];
+    part[1] += a.pin[a.np + q];
+    part[2] += a.pin[2 * a.np + q];
+  }
+  const double alpha_prev = a.k > 0 ? a.ab[(a.k - 1) & 1] : 1.0;
+  const double gamma_prev = a.k > 0 ? a.st->rz[(a.k - 1) & 1] : 1.0;
+  const int4 e = pipe_chunk(a.sv);
+  if (threadIdx.x == 0) s_done = done0;  // thread 0's read decides for the block
+  // fixed-order block reduction of the partials (as pipe_block_sum)
+#pragma unroll
+  for (int q = 0; q < 3; ++q) part[q] = wave_sum(part[q]);
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) s_red[q * kPipeWaves + wid] = part[q];
+  }
+  __syncthreads();
+  if (s_done != 0) return;
+  if (wid == 0) {
+    double tot[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) tot[q] = wave_sum(lane < kPipeWaves ? s_red[q * kPipeWaves + lane] : 0.0);
+    if (lane == 0) {
+      const double g = tot[0], d = tot[1], rr = tot[2];
+      const bool fresh = a.k == 0 || a.restart;
+      const double tol2 = fresh ? a.rtol2 * rr : a.st->tol2;
+      int dn = 0;
+      double alpha = 0.0, beta = 0.0;
+      if (!(isfinite(g) && isfinite(d) && isfinite(rr))) {
+        dn = 3;
+      } else if (!(rr > tol2)) {
+        dn = 1;
+      } else {
+        beta = a.k > 0 ? g / gamma_prev : 0.0;
+        const double den = a.k > 0 ? d - beta * g / alpha_prev : d;  // p.Ap
+        if (!(den > 0.0)) dn = isfinite(den) ? 2 : 3;
+        else alpha = g / den;
+      }
+      s_ab[0] = alpha;
+      s_ab[1] = beta;
+      s_ab[2] = dn;
+      if (blockIdx.x == 0) {
+        a.st->rz[a.k & 1] = g;
+        a.ab[a.k & 1] = alpha;
+        a.st->rr = rr;
+        a.st->pAp = alpha > 0.0 ? g / alpha : d;
+        if (fresh) a.st->tol2 = tol2;
+        if (a.k == 0) a.st->bb = rr;
+        if (dn) {
+          a.st->done = dn;
+          if (a.hst) a.hst->done = dn;
+        } else {
+          a.st->iter = a.k + 1;
+          if (a.hst) a.hst->iter = a.k + 1;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (s_ab[2] != 0.0) return;
+  const T alpha = (T)s_ab[0], beta = (T)s_ab[1];
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (e.x >= 0) {
+    const T* __restrict__ win = a.win;
+    T y[3] = {T(0), T(0), T(0)};
+    pipe_row_product<T>(a.sv, a.vhat, e, i, j, [&](int64_t w, T (&v)[3]) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) v[q] = win[w + q];
+    }, y);
+    const int32_t row = a.sv.sell_row[(int64_t)e.x * kSellC + i];
+    if (row >= 0 && j == 0) {
+      const int64_t v0 = 3 * (int64_t)row;
+      T r[3], u[3];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) r[q] = a.r[v0 + q];
+      pipe_apply_binv(a.binv, row, r, u);
+      T rn[3], wn[3];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const int64_t k = v0 + q;
+        const bool f = a.fixed && a.fixed[k];
+        const T tq = (f ? T(0) : y[q]) + beta * a.t[k];
+        const T wq = win[k];
+        const T sq = wq + beta * a.s[k];
+        const T pq = u[q] + beta * a.p[k];
+        a.t[k] = tq;
+        a.s[k] = sq;
+        a.p[k] = pq;
+        a.x[k] += alpha * pq;
+        rn[q] = r[q] - alpha * sq;
+        wn[q] = wq - alpha * tq;
+        a.r[k] = rn[q];
+        a.wout[k] = wn[q];
+      }
+      T un[3];
+      pipe_apply_binv(a.binv, row, rn, un);
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        acc[0] += (double)rn[q] * (double)un[q];
+        acc[1] += (double)wn[q] * (double)un[q];
+        acc[2] += (double)rn[q] * (double)rn[q];
+      }
+    }
+  }
+  pipe_store_partials(a, acc);
+}
+
+// Ah = A B^-1 per slot: block (row v, column w) times the masked inverse of
+// w's diagonal block, in the SELL layout of A (padding slots stay zero).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void precond_scale_kernel(int64_t nslot, const int32_t* __restrict__ scol,
+                                                              const T* __restrict__ A, const T* __restrict__ binv,
+                                                              T* __restrict__ Ah) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nslot) return;
+  const int64_t g = q & ~(int64_t)(kSellC - 1);
+  const int i = (int)(q - g);
+  const int32_t w = scol[q];
+  T a[9];
+  load_slot<T, kSellC, 1>(A, g, i, a);
+  T B[9];
+#pragma unroll
+  for (int m = 0; m < 9; ++m) B[m] = w >= 0 ? binv[9 * (int64_t)w + m] : T(0);
+  T* o = Ah + 9 * g + i;
+#pragma unroll
+  for (int pq = 0; pq < 3; ++pq)
+#pragma unroll
+    for (int qq = 0; qq < 3; ++qq)
+      o[(3 * pq + qq) * kSellC] = a[3 * pq] * B[qq] + a[3 * pq + 1] * B[3 + qq] + a[3 * pq + 2] * B[6 + qq];
+}
+
+// ---------------------------------------------------------------------------
 // Generic CSR operator with scalar Jacobi preconditioner.
 // ---------------------------------------------------------------------------
 struct CsrPcgArgs {
@@ -1211,12 +1532,21 @@ struct sk_sim {
   PcgState* st_host = nullptr;  // pinned, coherent: the PCG kernels write iter and done through to it
   DevBuf<unsigned long long> fp64_count;  // element Hessians that took the FP64 polar branch this step
   DevBuf<int32_t> order;  // optional: internal vertex t is the caller's vertex order[t] (sk_sim_set_order)
+  // pipelined PCG (pcg_variant 1): Ah = A B^-1 in the matrix's layout, the p vector, alpha of the last two launches
+  DevBuf<double> vhat, pp;
+  DevBuf<float> vhat32, pp32;
+  DevBuf<double> ab;
   hipEvent_t chunk_ev[2] = {};
   hipEvent_t ev[6] = {};
   bool have_events = false;
   sk::CoarseSpace coarse;  // two-level preconditioner (coarse.n_agg > 0 when enabled)
   bool mixed() const { return prm.precision == SK_PREC_MIXED; }
   bool two_level() const { return coarse.n_agg > 0; }
+  // The mixed path keeps the classic form: its FP64 residual replacement breaks the pipelined
+  // recurrences' identity p.Ap = delta - beta gamma / alpha_prev in FP32 (the replaced residual is
+  // no longer r_prev - alpha s_prev to the recurrences' accuracy; a numpy emulation of the FP32 run
+  // breaks down at the second replacement).
+  bool pipelined() const { return prm.pcg_variant == 1 && !two_level() && !mixed(); }
 };
 
 namespace {
@@ -1360,6 +1690,68 @@ int run_block_pcg(sk_sim* S, const PcgBufs<T>& B, const T* b, double rtol, int64
   return SK_OK;
 }
 
+// Pipelined block-Jacobi PCG (ppcg_*_kernel) on the system whose Ah = A B^-1
+// the caller has formed (precond_scale_kernel), with the same contract as
+// run_block_pcg: x0 = 0 (k0 = 0) or the continuation after a residual
+// replacement (k0 > 0: r holds the replaced residual, the search direction is
+// kept, x restarts from 0); iters = nullptr returns without waiting.
+template <typename T>
+int run_pipe_pcg(sk_sim* S, const PcgBufs<T>& B, const T* vhat, T* pvec, const T* b, double rtol,
+                 int64_t max_iter, hipStream_t st, int64_t* iters, int* done, int64_t k0 = 0) {
+  sk_mesh* m = S->mesh;
+  const int grid = (int)(8 * ((m->lpad + kPipeWaves - 1) / kPipeWaves));
+  SK_REQUIRE(6 * (int64_t)grid <= 8 * (int64_t)Reducer::kMaxBlocks, "mesh too large for the PCG reductions");
+  double* part = m->red.partials.p;
+  // vectors: s in z, t in Ap, w double-buffered in p0 / p1
+  PipeArgs<T> a{m->nv, sell_view(m), vhat, B.binv, S->fixed.p, b, B.x, B.r, B.z, B.Ap, pvec, nullptr, nullptr,
+                S->st.p, S->ab.p, rtol * rtol, nullptr, nullptr, grid, 0, 0, S->st_host};
+  auto wbuf = [&](int64_t k) { return (k & 1) ? B.p1 : B.p0; };
+  auto pbuf = [&](int64_t k) { return part + 3 * (size_t)grid * (k & 1); };
+  a.restart = k0 > 0 ? 1 : 0;
+  a.wout = wbuf(k0);
+  a.pout = pbuf(k0);
+  ppcg_start_kernel<T><<<grid, kPipeBlock, 0, st>>>(a);
+  SK_HIP_CHECK(hipGetLastError());
+  const int chunk = pcg_chunk(S);
+  const volatile PcgState* h = S->st_host;
+  int64_t launched = k0;
+  int c = 0;
+  for (bool stop = false; !stop; ++c) {
+    for (int k = 0; k < chunk && launched < max_iter; ++k, ++launched) {
+      a.k = (int)launched;
+      a.restart = (k0 > 0 && launched == k0) ? 1 : 0;
+      a.win = wbuf(launched);
+      a.wout = wbuf(launched + 1);
+      a.pin = pbuf(launched);
+      a.pout = pbuf(launched + 1);
+      ppcg_iter_kernel<T><<<grid, kPipeBlock, 0, st>>>(a);
+    }
+    SK_HIP_CHECK(hipGetLastError());
+    SK_HIP_CHECK(hipEventRecord(S->chunk_ev[c & 1], st));
+    if (c > 0) {
+      SK_HIP_CHECK(hipEventSynchronize(S->chunk_ev[(c - 1) & 1]));
+      const int hd = h->done, hi = h->iter;
+      if (hd || hi >= max_iter) stop = true;
+      if (k0 > 0 && hi < k0) stop = true;  // continuation did not start
+    }
+    if (launched >= max_iter) stop = true;  // no launches past the iteration limit
+  }
+  if (!iters) return SK_OK;
+  SK_HIP_CHECK(hipEventSynchronize(S->chunk_ev[(c - 1) & 1]));
+  *iters = h->iter;
+  *done = h->done;
+  return SK_OK;
+}
+
+// Ah = A B^-1 of the assembled system (pipelined PCG), once per Newton system.
+template <typename T>
+int form_precond_matrix(sk_sim* S, const T* A, const T* binv, T* Ah, hipStream_t st) {
+  sk_mesh* m = S->mesh;
+  precond_scale_kernel<T><<<grid_for(m->nslot), kBlock, 0, st>>>(m->nslot, m->scol.p, A, binv, Ah);
+  SK_HIP_CHECK(hipGetLastError());
+  return SK_OK;
+}
+
 // The result of a run_block_pcg launched without waiting, once the stream
 // has been synchronized since.
 void pcg_result(const sk_sim* S, int64_t* iters, int* done) {
@@ -1400,8 +1792,12 @@ int run_mixed_solve(sk_sim* S, hipStream_t st, int64_t* iters, int* refines, int
     // stream, and its result is read after the residual's readback (on a
     // breakdown the step fails, so the correction accumulated meanwhile is moot).
     int d = 0;
-    SK_TRY(run_block_pcg<float>(S, B, B.r, S->prm.inner_rtol, S->prm.pcg_max_iter, st, nullptr, nullptr,
-                                seg == 0 ? 0 : k));
+    if (S->pipelined())
+      SK_TRY(run_pipe_pcg<float>(S, B, S->vhat32.p, S->pp32.p, B.r, S->prm.inner_rtol, S->prm.pcg_max_iter, st,
+                                 nullptr, nullptr, seg == 0 ? 0 : k));
+    else
+      SK_TRY(run_block_pcg<float>(S, B, B.r, S->prm.inner_rtol, S->prm.pcg_max_iter, st, nullptr, nullptr,
+                                  seg == 0 ? 0 : k));
     ++*refines;
     ir_accumulate_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, B.x, S->dx.p, seg == 0 ? 1 : 0);
     ra.x = S->dx.p;
@@ -1468,6 +1864,8 @@ int sk_sim_create(sk_mesh* mesh, const sk_sim_params* p, const double* mu, const
              "sk_sim_create: mixed precision needs 0 < inner_rtol < 1 and max_refine > 0");
   SK_REQUIRE(p->spmv_blocks >= 0 && p->update_blocks >= 0, "sk_sim_create: launch widths must be >= 0");
   SK_REQUIRE(p->pcg_chunk >= 0 && p->pcg_chunk <= 1024, "sk_sim_create: pcg_chunk must be in [0, 1024]");
+  SK_REQUIRE(p->pcg_variant == 0 || p->pcg_variant == 1, "sk_sim_create: pcg_variant must be 0 or 1");
+  SK_REQUIRE(p->psd_jacobi == 0 || p->psd_jacobi == 1, "sk_sim_create: psd_jacobi must be 0 or 1");
   sk_sim* S = new sk_sim();
   S->mesh = mesh;
   S->prm = *p;
@@ -1498,6 +1896,16 @@ int sk_sim_create(sk_mesh* mesh, const sk_sim_params* p, const double* mu, const
       SK_TRY(S->vals.alloc(9 * (size_t)mesh->nslot));
       SK_HIP_CHECK(hipMemset(S->vals.p, 0, 9 * (size_t)mesh->nslot * sizeof(double)));
       SK_TRY(S->binv.alloc(9 * (size_t)mesh->nv));
+    }
+    if (p->pcg_variant == 1) {
+      if (S->mixed()) {
+        SK_TRY(S->vhat32.alloc(9 * (size_t)mesh->nslot));
+        SK_TRY(S->pp32.alloc(n3));
+      } else {
+        SK_TRY(S->vhat.alloc(9 * (size_t)mesh->nslot));
+        SK_TRY(S->pp.alloc(n3));
+      }
+      SK_TRY(S->ab.alloc(4));
     }
     SK_TRY(S->st.alloc(1));
     SK_TRY(S->fp64_count.alloc(1));
@@ -1629,11 +2037,15 @@ int step_core(sk_sim* S, double c_mass, const double* x_guess, double* x_next, s
     if (inf.newton_iters >= P.newton_max_iter) break;
     if (P.timing) SK_HIP_CHECK(hipEventRecord(S->ev[0], st));
     SK_TRY(launch_elem_hess(m, P.material, S->x.p, nullptr, S->mu.p, lam, S->vol.p, P.psd, mixed, st, nullptr,
-                            nullptr, S->fp64_count.p));
+                            nullptr, S->fp64_count.p, P.psd_jacobi == 0));
     if (mixed)
       SK_TRY(launch_gather_hess(m, S->vals32.p, S->mass.p, inv_h2, fixed, S->binv32.p, kLayoutSELL, true, st));
     else
       SK_TRY(launch_gather_hess(m, S->vals.p, S->mass.p, inv_h2, fixed, S->binv.p, kLayoutSELL, false, st));
+    if (S->pipelined()) {
+      if (mixed) SK_TRY(form_precond_matrix<float>(S, S->vals32.p, S->binv32.p, S->vhat32.p, st));
+      else SK_TRY(form_precond_matrix<double>(S, S->vals.p, S->binv.p, S->vhat.p, st));
+    }
     if (P.timing) SK_HIP_CHECK(hipEventRecord(S->ev[1], st));
     // Newton direction: H dx = -g on the free DOFs. The RHS is negated in place.
     axpy_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, -2.0, S->g.p, S->g.p);
@@ -1642,6 +2054,9 @@ int step_core(sk_sim* S, double c_mass, const double* x_guess, double* x_next, s
     // The FP64 solve is not waited for: the slope's launches follow it in the
     // stream and its result is read after the slope's readback.
     if (mixed) SK_TRY(run_mixed_solve(S, st, &it, &refines, &done));
+    else if (S->pipelined())
+      SK_TRY(run_pipe_pcg<double>(S, bufs64(S), S->vhat.p, S->pp.p, S->g.p, P.pcg_rtol, P.pcg_max_iter, st,
+                                  nullptr, nullptr));
     else SK_TRY(run_block_pcg<double>(S, bufs64(S), S->g.p, P.pcg_rtol, P.pcg_max_iter, st, nullptr, nullptr));
     axpy_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, -2.0, S->g.p, S->g.p);
     if (P.timing) SK_HIP_CHECK(hipEventRecord(S->ev[2], st));
@@ -1752,6 +2167,35 @@ int sk_sim_spmv_reps(sk_sim* S, int reps, sk_stream_t s) {
   hipStream_t st = as_stream(s);
   // Per-step scratch vectors only; rtol 0 so every launch runs iteration 0 in
   // full. Right-hand side: the last gradient (FP64) or the last FP32 residual.
+  // The pipelined form runs its iteration kernel: one start launch, then reps
+  // consecutive iterations (rtol 0: none stops early on an SPD system).
+  if (S->pipelined()) {
+    sk_mesh* m = S->mesh;
+    const int grid = (int)(8 * ((m->lpad + kPipeWaves - 1) / kPipeWaves));
+    double* part = m->red.partials.p;
+    auto run = [&](auto B, auto* vhat, auto* pvec, const auto* b) {
+      using T = std::remove_const_t<std::remove_pointer_t<decltype(vhat)>>;
+      PipeArgs<T> a{m->nv, sell_view(m), vhat, B.binv, S->fixed.p, b, B.x, B.r, B.z, B.Ap, pvec, nullptr,
+                    B.p0, S->st.p, S->ab.p, 0.0, nullptr, part, grid, 0, 0, nullptr};
+      ppcg_start_kernel<T><<<grid, kPipeBlock, 0, st>>>(a);
+      for (int k = 0; k < reps; ++k) {
+        a.k = k;
+        a.win = (k & 1) ? B.p1 : B.p0;
+        a.wout = (k & 1) ? B.p0 : B.p1;
+        a.pin = part + 3 * (size_t)grid * (k & 1);
+        a.pout = part + 3 * (size_t)grid * ((k + 1) & 1);
+        ppcg_iter_kernel<T><<<grid, kPipeBlock, 0, st>>>(a);
+      }
+    };
+    if (S->mixed()) {
+      const PcgBufs<float> B = bufs32(S);
+      run(B, S->vhat32.p, S->pp32.p, (const float*)B.r);
+    } else {
+      run(bufs64(S), S->vhat.p, S->pp.p, (const double*)S->g.p);
+    }
+    SK_HIP_CHECK(hipGetLastError());
+    return SK_OK;
+  }
   if (S->mixed()) {
     const PcgBufs<float> B = bufs32(S);
     BlockPcgArgs<float> a = block_pcg_args<float>(S, B, B.r, 0.0);
@@ -1779,7 +2223,7 @@ int sk_sim_hess_reps(sk_sim* S, int reps, sk_stream_t s) {
   const double* lam = S->lam.p;
   for (int k = 0; k < reps; ++k)
     SK_TRY(launch_elem_hess(S->mesh, S->prm.material, S->x.p, nullptr, S->mu.p, lam, S->vol.p, S->prm.psd,
-                            S->mixed(), st));
+                            S->mixed(), st, nullptr, nullptr, nullptr, S->prm.psd_jacobi == 0));
   return SK_OK;
 }
 

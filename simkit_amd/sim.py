@@ -35,7 +35,7 @@ class BackwardEulerSim:
                  pcg_max_iter=100000, ls_alpha=0.01, ls_beta=0.5, ls_max_iter=100, ls_threshold=1e-12,
                  timing=False, precision="fp64", inner_rtol=1e-2, max_refine=50, spmv_blocks=0,
                  update_blocks=0, preconditioner="bjacobi", coarse_vertices=216, pcg_chunk=0, renumber=True,
-                 max_inverted=-1):
+                 max_inverted=-1, pcg="classic", psd_solver="closed_form"):
         """precision: "fp64" (every stage FP64) or "mixed" (FP32 element Hessian, PSD projection,
         assembled values and inner PCG to inner_rtol, wrapped in FP64 iterative refinement to
         pcg_rtol; gradient, line-search energies and the polar SVD of ARAP elements with
@@ -52,7 +52,11 @@ class BackwardEulerSim:
         states passed in and returned keep the caller's vertex order (the library permutes them in the
         step's first and last kernels, sk_sim_set_order).
         max_inverted: < 0 no limit; otherwise a step that ends with more elements of det F <= 0 reports
-        status SK_ERR_INVERTED (info["inverted_elems"] always holds the count)."""
+        status SK_ERR_INVERTED (info["inverted_elems"] always holds the count).
+        pcg: "classic" (two launches per block-Jacobi PCG iteration) or "pipelined" (one launch per
+        iteration: Ghysels-Vanroose recurrences on A B^-1); the two-level preconditioner is classic.
+        psd_solver: "closed_form" (stable neo-Hookean's Hessian eigensystem from the SVD of F, the 9x9
+        Jacobi for other densities and near-singular F) or "jacobi" (the 9x9 Jacobi for every element)."""
         if preconditioner not in ("bjacobi", "two_level"):
             raise ValueError("preconditioner must be 'bjacobi' or 'two_level'")
         X = np.asarray(X, dtype=np.float64)
@@ -143,6 +147,12 @@ class BackwardEulerSim:
         p.update_blocks = update_blocks
         p.pcg_chunk = pcg_chunk
         p.max_inverted = int(max_inverted)
+        if pcg not in ("classic", "pipelined"):
+            raise ValueError("pcg must be 'classic' or 'pipelined'")
+        p.pcg_variant = 1 if pcg == "pipelined" else 0
+        if psd_solver not in ("closed_form", "jacobi"):
+            raise ValueError("psd_solver must be 'closed_form' or 'jacobi'")
+        p.psd_jacobi = 1 if psd_solver == "jacobi" else 0
         self.params = p
         self.precision = precision
         h_ = ctypes.c_void_p()
