@@ -17,10 +17,14 @@
 //                      refreshed after every update.
 // Rounding to BF16 is round-to-nearest-even.
 //
-// Every GEMM is one wave per 16 x 16 output tile with
-// v_mfma_f32_16x16x16_bf16 (lane l supplies A[l % 16][4 (l / 16) + 0..3] and
-// B[4 (l / 16) + 0..3][l % 16] and receives D[4 (l / 16) + 0..3][l % 16]).
-// Feature and batch dimensions are padded to multiples of 16 with zeros.
+// Every GEMM runs 64 x 64 output tiles per workgroup (4 waves, 32 x 32 each:
+// 2 x 2 tiles of v_mfma_f32_16x16x32_bf16), stepping K by 32: the workgroup
+// stages the A and B tiles in LDS with 16-byte loads along whichever operand
+// dimension is contiguous in memory (so transposed operands cost the same),
+// storing both k-contiguous; lane l then reads its fragments with one 16-byte
+// LDS load each: A[l % 16][8 (l / 16) + 0..7] and B[8 (l / 16) + 0..7][l % 16],
+// and receives D[4 (l / 16) + 0..3][l % 16]. Feature and batch dimensions are
+// padded to multiples of 16 with zeros; a K tail of 16 is zero-filled in LDS.
 #include "sk_internal.h"
 
 #include <algorithm>
@@ -28,7 +32,6 @@
 #include <cstring>
 #include <vector>
 
-typedef short sk_v4s __attribute__((ext_vector_type(4)));
 typedef float sk_v4f __attribute__((ext_vector_type(4)));
 
 namespace sk {
@@ -71,42 +74,96 @@ struct GemmArgs {
   int gelu_act;                     // 1: Cbf holds GELU(C)
 };
 
-__global__ __launch_bounds__(kBlock) void gemm_bf16_kernel(GemmArgs g) {
-  const int tiles_n = g.N / kTile;
-  const int64_t tile = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
-  if (tile >= (int64_t)(g.M / kTile) * tiles_n) return;
-  const int tm = (int)(tile / tiles_n), tn = (int)(tile % tiles_n);
-  const int lane = threadIdx.x % kWave;
-  const int i = lane % kTile, kq = lane / kTile;
-  sk_v4f acc = {0.f, 0.f, 0.f, 0.f};
-  const uint16_t* Ar = g.A + (int64_t)(tm * kTile + i) * g.sam;
-  const uint16_t* Bc = g.B + (int64_t)(tn * kTile + i) * g.sbn;
-  for (int k0 = 0; k0 < g.K; k0 += kTile) {
-    sk_v4s a, b;
+constexpr int kGT = 64;           // workgroup output tile, M and N
+constexpr int kGK = 32;           // K step (one 16x16x32 MFMA)
+constexpr int kGP = kGK + 8;      // LDS row pitch in BF16 (80 bytes: 16-byte aligned, rows on different banks)
+
+typedef __bf16 sk_bf16x8 __attribute__((ext_vector_type(8)));
+typedef uint32_t sk_u4 __attribute__((ext_vector_type(4)));
+
+// S[r][c] = P[(r0 + r) srow + (k0 + c) sk] for r < 64, c < 32 (zero outside
+// R x K). 256 threads, 8 elements each: 16-byte loads along k (sk == 1) or
+// along r (srow == 1) when the addresses allow, else element by element.
+// K is a multiple of 16, so an aligned run of 8 along k is all inside or all
+// outside.
+__device__ __forceinline__ void stage_tile(const uint16_t* __restrict__ P, int64_t srow, int64_t sk, int r0, int R,
+                                           int k0, int K, bool vec_k, bool vec_r, uint16_t (*S)[kGP]) {
+  const int t = threadIdx.x;
+  if (vec_k) {
+    const int r = t / 4, c = (t % 4) * 8;
+    sk_u4 v = {0u, 0u, 0u, 0u};
+    if (r0 + r < R && k0 + c < K) v = *reinterpret_cast<const sk_u4*>(P + (int64_t)(r0 + r) * srow + (k0 + c));
+    *reinterpret_cast<sk_u4*>(&S[r][c]) = v;
+  } else if (vec_r) {
+    const int c = t / 8, r = (t % 8) * 8;
+    sk_u4 v = {0u, 0u, 0u, 0u};
+    if (r0 + r < R && k0 + c < K) v = *reinterpret_cast<const sk_u4*>(P + (int64_t)(k0 + c) * sk + (r0 + r));
+    const uint16_t* h = reinterpret_cast<const uint16_t*>(&v);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int64_t k = k0 + 4 * kq + r;
-      a[r] = (short)Ar[k * g.sak];
-      b[r] = (short)Bc[k * g.sbk];
+    for (int q = 0; q < 8; ++q) S[r + q][c] = h[q];
+  } else {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int e = t * 8 + q, r = e / kGK, c = e % kGK;
+      S[r][c] = (r0 + r < R && k0 + c < K) ? P[(int64_t)(r0 + r) * srow + (int64_t)(k0 + c) * sk] : (uint16_t)0;
     }
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
   }
-  const int n = tn * kTile + i;
-  const float bn = g.bias ? g.bias[n] : 0.f;
+}
+
+__global__ __launch_bounds__(kBlock) void gemm_bf16_kernel(GemmArgs g) {
+  __shared__ __align__(16) uint16_t As[kGT][kGP];  // [m][k]
+  __shared__ __align__(16) uint16_t Bs[kGT][kGP];  // [n][k]
+  const int tiles_n = (g.N + kGT - 1) / kGT;
+  const int m0 = (blockIdx.x / tiles_n) * kGT, n0 = (blockIdx.x % tiles_n) * kGT;
+  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+  const int wm = (wid / 2) * 32, wn = (wid % 2) * 32;
+  const int i = lane % 16, kq = lane / 16;
+  // 16-byte staging needs 8-element runs at 16-byte aligned addresses
+  auto aligned = [](const uint16_t* p, int64_t s) { return ((uintptr_t)p % 16 == 0) && (s % 8 == 0); };
+  const bool a_k = g.sak == 1 && aligned(g.A, g.sam), a_r = !a_k && g.sam == 1 && aligned(g.A, g.sak);
+  const bool b_k = g.sbk == 1 && aligned(g.B, g.sbn), b_r = !b_k && g.sbn == 1 && aligned(g.B, g.sbk);
+  sk_v4f acc[2][2];
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int64_t m = tm * kTile + 4 * kq + r;
-    const float z = acc[r] + bn;
-    if (g.C) g.C[m * g.ldc + n] = z;
-    if (g.Cbf) g.Cbf[m * g.ldcb + n] = f2bf(g.gelu_act ? gelu(z) : z);
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = sk_v4f{0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < g.K; k0 += kGK) {
+    stage_tile(g.A, g.sam, g.sak, m0, g.M, k0, g.K, a_k, a_r, As);
+    stage_tile(g.B, g.sbn, g.sbk, n0, g.N, k0, g.K, b_k, b_r, Bs);
+    __syncthreads();
+    sk_bf16x8 af[2], bfr[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) af[a] = *reinterpret_cast<const sk_bf16x8*>(&As[wm + 16 * a + i][8 * kq]);
+#pragma unroll
+    for (int b = 0; b < 2; ++b) bfr[b] = *reinterpret_cast<const sk_bf16x8*>(&Bs[wn + 16 * b + i][8 * kq]);
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int b = 0; b < 2; ++b) {
+    const int n = n0 + wn + 16 * b + i;
+    if (n >= g.N) continue;
+    const float bn = g.bias ? g.bias[n] : 0.f;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t m = m0 + wm + 16 * a + 4 * kq + r;
+        if (m >= g.M) continue;
+        const float z = acc[a][b][r] + bn;
+        if (g.C) g.C[m * g.ldc + n] = z;
+        if (g.Cbf) g.Cbf[m * g.ldcb + n] = f2bf(g.gelu_act ? gelu(z) : z);
+      }
   }
 }
 
 int launch_gemm(const GemmArgs& g, hipStream_t st) {
   SK_REQUIRE(g.M % kTile == 0 && g.N % kTile == 0 && g.K % kTile == 0 && g.M > 0 && g.N > 0 && g.K > 0,
              "bf16 GEMM: dimensions must be positive multiples of 16");
-  const int64_t tiles = (int64_t)(g.M / kTile) * (g.N / kTile);
-  const int64_t blocks = (tiles + kBlock / kWave - 1) / (kBlock / kWave);
+  const int64_t blocks = (int64_t)((g.M + kGT - 1) / kGT) * ((g.N + kGT - 1) / kGT);
   gemm_bf16_kernel<<<(unsigned)blocks, kBlock, 0, st>>>(g);
   SK_HIP_CHECK(hipGetLastError());
   return SK_OK;
