@@ -298,6 +298,11 @@ __device__ __forceinline__ void load_staged_block(const R* __restrict__ s, R (&B
   }
 }
 
+// Contributions loaded per pass of the gather loop. Measured at 48^3 as the
+// FP64 Hessian phase per Newton iteration (element pass + gather): 1 per pass
+// 0.780 ms, 2 per pass 0.710 ms, 4 per pass 0.725 ms.
+constexpr int kGatherU = 2;
+
 // One lane per block: sums the staged contributions in list order (precision
 // R), adds the lumped mass / h^2 on diagonal blocks, writes the block in CSR,
 // BSR or a sliced order and, for diagonal blocks, the inverse of the
@@ -312,17 +317,14 @@ __global__ __launch_bounds__(kBlock) void gather_hess_kernel(GatherHArgs a) {
   R acc[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) acc[k] = R(0);
-  const int32_t k0 = a.cptr[b], k1 = a.cptr[b + 1];
-  for (int32_t k = k0; k < k1; ++k) {
-    const int32_t pk = a.clist[k];
+  // the staged upper block (min(i,j), max(i,j)) of contribution pk, used
+  // transposed when i > j: one load path for every lane, the transpose by selects
+  auto source = [&](int32_t pk, bool& tr) {
     const int i = (pk >> 2) & 3, j = pk & 3;
-    const int64_t e = pk >> 4;
-    // the staged upper block (min(i,j), max(i,j)), used transposed when i > j:
-    // one load path for every lane, the transpose by selects
-    const bool tr = i > j;
-    const R* s = Kst + kKstR<R> * e + kBlkStride<R> * ublock(tr ? j : i, tr ? i : j);
-    R B[kBlkStride<R>];
-    load_staged_block<R>(s, B);
+    tr = i > j;
+    return Kst + kKstR<R> * (int64_t)(pk >> 4) + kBlkStride<R> * ublock(tr ? j : i, tr ? i : j);
+  };
+  auto accumulate = [&](const R (&B)[kBlkStride<R>], bool tr) {
     acc[0] += B[0];
     acc[4] += B[4];
     acc[8] += B[8];
@@ -332,6 +334,24 @@ __global__ __launch_bounds__(kBlock) void gather_hess_kernel(GatherHArgs a) {
     acc[6] += tr ? B[2] : B[6];
     acc[5] += tr ? B[7] : B[5];
     acc[7] += tr ? B[5] : B[7];
+  };
+  const int32_t k0 = a.cptr[b], k1 = a.cptr[b + 1];
+  int32_t k = k0;
+  // kGatherU contributions per pass: their blocks' loads in flight together, summed in list order
+  constexpr int U = kGatherU;
+  for (; k + U <= k1; k += U) {
+    bool t[U];
+    R B[U][kBlkStride<R>];
+#pragma unroll
+    for (int u = 0; u < U; ++u) load_staged_block<R>(source(a.clist[k + u], t[u]), B[u]);
+#pragma unroll
+    for (int u = 0; u < U; ++u) accumulate(B[u], t[u]);
+  }
+  for (; k < k1; ++k) {
+    bool t0;
+    R B0[kBlkStride<R>];
+    load_staged_block<R>(source(a.clist[k], t0), B0);
+    accumulate(B0, t0);
   }
   const int32_t v = a.brow[b], w = a.bcol[b];
   const int32_t b0 = a.bptr[v], deg = a.bptr[v + 1] - b0, r = (int32_t)(b - b0);
