@@ -10,7 +10,8 @@
 //
 // Layout (n_agg aggregates, nc coarse unknowns):
 //   agg_ptr[n_agg + 1], agg_vert[nv]: the vertices of aggregate I are
-//     agg_vert[agg_ptr[I] .. agg_ptr[I + 1]), ascending;
+//     agg_vert[agg_ptr[I] .. agg_ptr[I + 1]), ascending (a view's agg_vert
+//     is null when that is the identity: vertex t at position t);
 //   vagg[nv]: aggregate of each vertex;
 //   dv[3 * v]: offset of vertex v from its aggregate's centroid;
 //   dva[3 * t]: the same offset for vertex agg_vert[t] (aggregate order);
@@ -34,6 +35,8 @@ constexpr int kMaxCoarse = 4096;  // coarse unknowns at most (E^-1: 64 MiB in FP
 
 template <typename T>
 struct CoarseView {
+  // vertex at aggregate-order position t
+  __device__ __forceinline__ int64_t vert(int64_t t) const { return agg_vert ? (int64_t)agg_vert[t] : t; }
   int n_agg;
   int nc;
   int ld;
@@ -50,6 +53,7 @@ struct CoarseView {
 struct CoarseSpace {
   int n_agg = 0, nc = 0, ld = 0;
   bool built = false;
+  bool ident = false;  // agg_vert[t] == t (aggregates are contiguous vertex ranges): views pass no list
   double build_ms = 0.0;
   DevBuf<int32_t> agg_ptr, agg_vert, agg_col, vagg, adj_ptr, adj;
   DevBuf<double> tm, dv64, dva64, E, pivot, c;
@@ -60,11 +64,11 @@ struct CoarseSpace {
 
 template <>
 inline CoarseView<double> CoarseSpace::view<double>() const {
-  return {n_agg, nc, ld, agg_ptr.p, agg_vert.p, agg_col.p, dv64.p, dva64.p, tm.p, einv.p, c.p};
+  return {n_agg, nc, ld, agg_ptr.p, ident ? nullptr : agg_vert.p, agg_col.p, dv64.p, dva64.p, tm.p, einv.p, c.p};
 }
 template <>
 inline CoarseView<float> CoarseSpace::view<float>() const {
-  return {n_agg, nc, ld, agg_ptr.p, agg_vert.p, agg_col.p, dv32.p, dva32.p, tm.p, einv.p, c.p};
+  return {n_agg, nc, ld, agg_ptr.p, ident ? nullptr : agg_vert.p, agg_col.p, dv32.p, dva32.p, tm.p, einv.p, c.p};
 }
 
 // Orthonormal basis of an aggregate's rigid motions restricted to its free

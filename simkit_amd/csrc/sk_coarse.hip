@@ -305,6 +305,8 @@ int coarse_setup(CoarseSpace& C, sk_mesh* m, const double* X, const uint8_t* fix
   C.nc = nc;
   C.ld = ld;
   C.built = false;
+  C.ident = true;
+  for (int64_t t = 0; t < nv && C.ident; ++t) C.ident = agg_vert[t] == (int32_t)t;
   SK_TRY(C.agg_ptr.upload(agg_ptr));
   SK_TRY(C.agg_vert.upload(agg_vert));
   SK_TRY(C.agg_col.upload(agg_col));

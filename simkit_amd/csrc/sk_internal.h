@@ -54,7 +54,8 @@ struct DevBuf {
   }
 };
 
-// Upper-triangle 3x3-block index of local pair (i <= j) of a tet: 0..9.
+// Upper-triangle 3x3-block index of local pair (i <= j) of a tet: 0..9. The
+// blocks of one local row i are consecutive.
 __host__ __device__ constexpr int ublock(int i, int j) { return i * (7 - i) / 2 + j; }
 
 // Staged element stiffness: the 10 upper 3x3 blocks of K_e. Block u (ublock)
@@ -62,6 +63,8 @@ __host__ __device__ constexpr int ublock(int i, int j) { return i * (7 - i) / 2 
 // blocks are padded to 10 doubles (the padding is never written) so that the
 // gather reads them with five 16-byte loads; FP32 blocks stay at 9 floats
 // (padded to 12 the extra bytes cost more than the wider loads saved).
+// Records are not padded to whole cache lines: at 48^3 that wrote 5% more
+// bytes and took 6% longer in FP64.
 template <typename R>
 constexpr int kBlkStride = sizeof(R) == 8 ? 10 : 9;
 template <typename R>
@@ -124,7 +127,7 @@ struct sk_mesh {
   // slot count and offset, so that addresses formed from them stay valid).
   sk::DevBuf<int4> cmeta;
   int64_t lpad = 0;
-  sk::DevBuf<double> Kst;     // [nt][90] staged element stiffness blocks
+  sk::DevBuf<double> Kst;     // [nt][kKst] staged element stiffness blocks
   sk::DevBuf<double> fst;     // [nt][12] staged element forces
   sk::DevBuf<uint8_t> eflag;  // [nt] element pass: 1 where the closed-form eigensystem was not used
   sk::Reducer red;

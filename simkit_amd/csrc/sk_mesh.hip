@@ -188,24 +188,24 @@ __global__ __launch_bounds__(kBlock) void elem_hess_kernel(ElemArgs a) {
   R D[12];
 #pragma unroll
   for (int k = 0; k < 12; ++k) D[k] = (R)Dd[k];
-  // K_e[3i+p][3j+q] = sum_{c,c'} D[c][i] H[3p+c][3q+c'] D[c'][j], upper blocks i <= j.
+  // K_e[3i+p][3j+q] = sum_{c,c'} D[c][i] H[3p+c][3q+c'] D[c'][j], upper blocks i <= j,
+  // written one local row i at a time: row i's blocks are consecutive in the
+  // record, so its cache lines are complete before the next row starts
   R* out = reinterpret_cast<R*>(a.Kst) + (int64_t)kKstR<R> * e;
 #pragma unroll
-  for (int p = 0; p < 3; ++p)
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      R G[3][4];
+    for (int p = 0; p < 3; ++p) {
+      R w[9];  // w[3q+c'] = sum_c D[c][i] H[3p+c][3q+c']
 #pragma unroll
-      for (int c = 0; c < 3; ++c)
+      for (int m = 0; m < 9; ++m)
+        w[m] = D[i] * H[Sym<9>::idx(3 * p, m)] + D[4 + i] * H[Sym<9>::idx(3 * p + 1, m)] +
+               D[8 + i] * H[Sym<9>::idx(3 * p + 2, m)];
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-          G[c][j] = H[Sym<9>::idx(3 * p + c, 3 * q + 0)] * D[j] + H[Sym<9>::idx(3 * p + c, 3 * q + 1)] * D[4 + j] +
-                    H[Sym<9>::idx(3 * p + c, 3 * q + 2)] * D[8 + j];
+      for (int j = i; j < 4; ++j)
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = i; j < 4; ++j)
-          out[ublock(i, j) * kBlkStride<R> + 3 * p + q] = D[i] * G[0][j] + D[4 + i] * G[1][j] + D[8 + i] * G[2][j];
+        for (int q = 0; q < 3; ++q)
+          out[ublock(i, j) * kBlkStride<R> + 3 * p + q] = w[3 * q] * D[j] + w[3 * q + 1] * D[4 + j] + w[3 * q + 2] * D[8 + j];
     }
 }
 

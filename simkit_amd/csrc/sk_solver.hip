@@ -292,7 +292,7 @@ template <bool kAgg, typename T, typename F>
 __device__ __forceinline__ void for_vertices(const BlockPcgArgs<T>& a, F f) {
   if (kAgg) {
     const int32_t t1 = a.cv.agg_ptr[blockIdx.x + 1];
-    for (int32_t t = a.cv.agg_ptr[blockIdx.x] + threadIdx.x; t < t1; t += blockDim.x) f((int64_t)a.cv.agg_vert[t]);
+    for (int32_t t = a.cv.agg_ptr[blockIdx.x] + threadIdx.x; t < t1; t += blockDim.x) f(a.cv.vert(t));
   } else {
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < a.nv; v += (int64_t)gridDim.x * blockDim.x)
       f(v);
@@ -710,7 +710,7 @@ __global__ __launch_bounds__(kBlock) void bpcg_update_kernel(BlockPcgArgs<T> a) 
     t = a.cv.agg_ptr[blockIdx.x] + threadIdx.x;
     t_end = a.cv.agg_ptr[blockIdx.x + 1];
     stride = blockDim.x;
-    if (t < t_end) v = a.cv.agg_vert[t];
+    if (t < t_end) v = a.cv.vert(t);
   } else {
     t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     t_end = a.nv;
@@ -725,7 +725,7 @@ __global__ __launch_bounds__(kBlock) void bpcg_update_kernel(BlockPcgArgs<T> a) 
   double rz = 0.0, rr = 0.0, raw[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (v >= 0) apply_update<T, kAgg>(a, v, u, alpha, rz, rr, raw);
   for (t += stride; t < t_end; t += stride) {
-    const int64_t w = kAgg ? (int64_t)a.cv.agg_vert[t] : t;
+    const int64_t w = kAgg ? a.cv.vert(t) : t;
     UpdateIn<T> uw;
     load_update(a, w, uw);
     apply_update<T, kAgg>(a, w, uw, alpha, rz, rr, raw);
@@ -766,7 +766,7 @@ __global__ __launch_bounds__(kBlock) void bpcg_coarse_kernel(BlockPcgArgs<T> a) 
   T pz[3] = {T(0), T(0), T(0)};
   bool pfree[3] = {false, false, false};
   if (has_p) {
-    pv = cv.agg_vert[tp];
+    pv = cv.vert(tp);
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       pd[q] = (double)cv.dva[3 * (int64_t)tp + q];
@@ -774,15 +774,32 @@ __global__ __launch_bounds__(kBlock) void bpcg_coarse_kernel(BlockPcgArgs<T> a) 
       pfree[q] = !(a.fixed && a.fixed[3 * pv + q]);
     }
   }
-  // y_q = sum_j E^-1[c0 + q][j] c_j: four columns per float4 load, up to kU loads per row in
-  // flight per thread
-  constexpr int kU = 4;
+  // y_q = sum_j E^-1[c0 + q][j] c_j: four columns per float4 load. Every load
+  // of a pass (6 rows x kU) is issued before the first FMA, so the rows'
+  // latencies overlap instead of adding up; rows past the aggregate's k
+  // re-read an in-bounds row and are not accumulated.
+  constexpr int kU = 3;  // one pass for up to 3072 coarse unknowns
   const int ld4 = cv.ld >> 2;
   const float4* __restrict__ E4 = reinterpret_cast<const float4*>(cv.einv);
   const double2* __restrict__ c2 = reinterpret_cast<const double2*>(cv.c);
+  const float4* rows[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    int rq = c0 + (q < k ? q : k - 1);
+    rq = rq < 0 ? 0 : (rq >= cv.nc ? cv.nc - 1 : rq);
+    rows[q] = E4 + (int64_t)rq * ld4;
+  }
   double y[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (int base = threadIdx.x; base < ld4; base += kU * kBlock) {
     double2 ca[kU], cb[kU];
+    float4 e[6][kU];
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int j4 = base + u * kBlock;
+        e[q][u] = j4 < ld4 ? rows[q][j4] : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
       const int j4 = base + u * kBlock;
@@ -792,17 +809,10 @@ __global__ __launch_bounds__(kBlock) void bpcg_coarse_kernel(BlockPcgArgs<T> a) 
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
       if (q < k) {
-        const float4* row = E4 + (int64_t)(c0 + q) * ld4;
-        float4 e[kU];
-#pragma unroll
-        for (int u = 0; u < kU; ++u) {
-          const int j4 = base + u * kBlock;
-          e[u] = j4 < ld4 ? row[j4] : make_float4(0.f, 0.f, 0.f, 0.f);
-        }
 #pragma unroll
         for (int u = 0; u < kU; ++u)
-          y[q] += (double)e[u].x * ca[u].x + (double)e[u].y * ca[u].y + (double)e[u].z * cb[u].x +
-                  (double)e[u].w * cb[u].y;
+          y[q] += (double)e[q][u].x * ca[u].x + (double)e[q][u].y * ca[u].y + (double)e[q][u].z * cb[u].x +
+                  (double)e[q][u].w * cb[u].y;
       }
     }
   }
@@ -830,7 +840,7 @@ __global__ __launch_bounds__(kBlock) void bpcg_coarse_kernel(BlockPcgArgs<T> a) 
       if (pfree[q]) a.z[3 * pv + q] = (T)((double)pz[q] + corr[q]);
   }
   for (int32_t t = tp + kBlock; t < t1; t += kBlock) {  // aggregates larger than the workgroup
-    const int64_t v = cv.agg_vert[t];
+    const int64_t v = cv.vert(t);
     const double d0 = (double)cv.dva[3 * (int64_t)t], d1 = (double)cv.dva[3 * (int64_t)t + 1],
                  d2 = (double)cv.dva[3 * (int64_t)t + 2];
     const double corr[3] = {u0 + w1 * d2 - w2 * d1, u1 + w2 * d0 - w0 * d2, u2 + w0 * d1 - w1 * d0};
