@@ -20,10 +20,15 @@
 //   agg_col[n_agg + 1]: the coarse unknowns of aggregate I are
 //     agg_col[I] .. agg_col[I + 1) (at most 6; fewer where pinned DOFs or a
 //     degenerate vertex set leave fewer independent motions);
-//   einv[nc * ld]: E^-1, symmetric, FP32 (applied with FP64 accumulation),
-//     rows padded with zeros to ld = nc rounded up to a multiple of 4;
-//   c[ld]: the restricted residual W^T r of the current iteration (zero
-//     past nc).
+//   dsc[ld]: D = diag(E^-1)^(1/2) (zero past nc);
+//   einv[nc * ld]: S = D^-1 E^-1 D^-1 (unit diagonal, |S_ij| <= 1), symmetrised
+//     and rounded to FP16, rows padded with zeros to ld = nc rounded up to a
+//     multiple of 8; shift = max_i sum_j |S16_ij - S_ij|. The coarse solve
+//     applies D (S16 + shift I) D: by Gershgorin the rounding moves no
+//     eigenvalue of S by more than the shift, so the applied operator is
+//     symmetric positive definite whenever E is, and M^-1 stays SPD;
+//   c[ld]: the scaled restricted residual D W^T r of the current iteration
+//     (zero past nc).
 // A vertex's raw rigid motion (u, w) displaces it by u + w x d.
 #pragma once
 
@@ -31,7 +36,7 @@
 
 namespace sk {
 
-constexpr int kMaxCoarse = 4096;  // coarse unknowns at most (E^-1: 64 MiB in FP32)
+constexpr int kMaxCoarse = 4096;  // coarse unknowns at most (the FP64 E^-1 of the build: 128 MiB)
 
 template <typename T>
 struct CoarseView {
@@ -46,7 +51,9 @@ struct CoarseView {
   const T* dv;
   const T* dva;
   const double* tm;
-  const float* einv;
+  const _Float16* einv;
+  const double* dsc;
+  double shift;
   double* c;
 };
 
@@ -57,18 +64,21 @@ struct CoarseSpace {
   double build_ms = 0.0;
   DevBuf<int32_t> agg_ptr, agg_vert, agg_col, vagg, adj_ptr, adj;
   DevBuf<double> tm, dv64, dva64, E, pivot, c;
-  DevBuf<float> dv32, dva32, einv;
+  DevBuf<float> dv32, dva32;
+  DevBuf<_Float16> einv;
+  DevBuf<double> dsc;
+  double shift = 0.0;
   template <typename T>
   CoarseView<T> view() const;
 };
 
 template <>
 inline CoarseView<double> CoarseSpace::view<double>() const {
-  return {n_agg, nc, ld, agg_ptr.p, ident ? nullptr : agg_vert.p, agg_col.p, dv64.p, dva64.p, tm.p, einv.p, c.p};
+  return {n_agg, nc, ld, agg_ptr.p, ident ? nullptr : agg_vert.p, agg_col.p, dv64.p, dva64.p, tm.p, einv.p, dsc.p, shift, c.p};
 }
 template <>
 inline CoarseView<float> CoarseSpace::view<float>() const {
-  return {n_agg, nc, ld, agg_ptr.p, ident ? nullptr : agg_vert.p, agg_col.p, dv32.p, dva32.p, tm.p, einv.p, c.p};
+  return {n_agg, nc, ld, agg_ptr.p, ident ? nullptr : agg_vert.p, agg_col.p, dv32.p, dva32.p, tm.p, einv.p, dsc.p, shift, c.p};
 }
 
 // Orthonormal basis of an aggregate's rigid motions restricted to its free

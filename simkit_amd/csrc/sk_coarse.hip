@@ -167,12 +167,29 @@ __global__ __launch_bounds__(kBlock) void gj_rowcol_kernel(double* A, int n, int
   }
 }
 
-// out[i * ld + j] = (A[i][j] + A[j][i]) / 2 in FP32; the padding columns were zeroed.
-__global__ __launch_bounds__(kBlock) void symmetrize_f32_kernel(const double* A, int n, int ld, float* out) {
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (int64_t)n * n) return;
-  const int i = (int)(idx / n), j = (int)(idx % n);
-  out[(int64_t)i * ld + j] = (float)(0.5 * (A[idx] + A[(int64_t)j * n + i]));
+// dsc[i] = sqrt(E^-1_ii).
+__global__ __launch_bounds__(kBlock) void coarse_scale_kernel(const double* A, int n, double* dsc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dsc[i] = sqrt(A[(int64_t)i * n + i]);
+}
+
+// out[i * ld + j] = FP16 of S_ij = (A_ij + A_ji) / 2 / (dsc_i dsc_j), one workgroup
+// per row i; dev[i] = sum_j |S16_ij - S_ij| in a fixed order (per-thread
+// strided sums, then the workgroup sum). The padding columns were zeroed.
+__global__ __launch_bounds__(kBlock) void coarse_f16_kernel(const double* A, int n, int ld, const double* dsc,
+                                                           _Float16* out, double* dev) {
+  __shared__ double lds[kBlock / kWave];
+  const int i = blockIdx.x;
+  const double di = dsc[i];
+  double acc[1] = {0.0};
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    const double sij = 0.5 * (A[(int64_t)i * n + j] + A[(int64_t)j * n + i]) / (di * dsc[j]);
+    const _Float16 h = (_Float16)sij;
+    out[(int64_t)i * ld + j] = h;
+    acc[0] += fabs((double)h - sij);
+  }
+  block_sum<1>(acc, lds);
+  if (threadIdx.x == 0) dev[i] = acc[0];
 }
 
 }  // namespace
@@ -300,7 +317,7 @@ int coarse_setup(CoarseSpace& C, sk_mesh* m, const double* X, const uint8_t* fix
   for (int64_t t = 0; t < nv; ++t)
     for (int a = 0; a < 3; ++a) dva[3 * t + a] = dv[3 * (int64_t)agg_vert[t] + a];
   std::vector<float> dva32(dva.begin(), dva.end());
-  const int ld = (nc + 3) / 4 * 4;
+  const int ld = (nc + 7) / 8 * 8;
   C.n_agg = n_agg;
   C.nc = nc;
   C.ld = ld;
@@ -323,6 +340,8 @@ int coarse_setup(CoarseSpace& C, sk_mesh* m, const double* X, const uint8_t* fix
   SK_TRY(C.pivot.alloc(nc));
   SK_TRY(C.E.alloc((size_t)nc * nc));
   SK_TRY(C.einv.alloc((size_t)nc * ld));
+  SK_TRY(C.dsc.alloc(ld));
+  SK_HIP_CHECK(hipMemset(C.dsc.p, 0, ld * sizeof(double)));
   return SK_OK;
 }
 
@@ -348,10 +367,7 @@ int coarse_build(CoarseSpace& C, sk_mesh* m, const void* vals, bool fp32, const 
     gj_rowcol_kernel<<<gr, kBlock, 0, st>>>(C.E.p, nc, k, C.pivot.p);
   }
   SK_HIP_CHECK(hipGetLastError());
-  SK_HIP_CHECK(hipMemsetAsync(C.einv.p, 0, (size_t)nc * C.ld * sizeof(float), st));
-  symmetrize_f32_kernel<<<ge, kBlock, 0, st>>>(C.E.p, nc, C.ld, C.einv.p);
-  SK_HIP_CHECK(hipGetLastError());
-  std::vector<double> piv(nc);
+  std::vector<double> piv(nc), dev(nc);
   SK_HIP_CHECK(hipMemcpyAsync(piv.data(), C.pivot.p, nc * sizeof(double), hipMemcpyDeviceToHost, st));
   SK_HIP_CHECK(hipStreamSynchronize(st));
   for (int k = 0; k < nc; ++k)
@@ -359,6 +375,18 @@ int coarse_build(CoarseSpace& C, sk_mesh* m, const void* vals, bool fp32, const 
       set_error("coarse build: the coarse matrix is not positive definite (pivot " + std::to_string(k) + ")");
       return SK_ERR_PCG_BREAKDOWN;
     }
+  SK_HIP_CHECK(hipMemsetAsync(C.einv.p, 0, (size_t)nc * C.ld * sizeof(_Float16), st));
+  coarse_scale_kernel<<<(nc + kBlock - 1) / kBlock, kBlock, 0, st>>>(C.E.p, nc, C.dsc.p);
+  // the pivots have been checked: their buffer takes the rows' rounding deviations
+  coarse_f16_kernel<<<nc, kBlock, 0, st>>>(C.E.p, nc, C.ld, C.dsc.p, C.einv.p, C.pivot.p);
+  SK_HIP_CHECK(hipGetLastError());
+  SK_HIP_CHECK(hipMemcpyAsync(dev.data(), C.pivot.p, nc * sizeof(double), hipMemcpyDeviceToHost, st));
+  SK_HIP_CHECK(hipStreamSynchronize(st));
+  C.shift = 0.0;
+  for (int k = 0; k < nc; ++k) {
+    SK_REQUIRE(std::isfinite(dev[k]), "coarse build: non-finite coarse inverse");
+    C.shift = std::max(C.shift, dev[k]);
+  }
   C.built = true;
   C.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return SK_OK;

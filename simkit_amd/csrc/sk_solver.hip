@@ -299,10 +299,11 @@ __device__ __forceinline__ void for_vertices(const BlockPcgArgs<T>& a, F f) {
   }
 }
 
-// raw[0..2] += r, raw[3..5] += d x r: the vertex's term of R_v^T r.
+// raw[0..2] += r, raw[3..5] += d x r: the vertex's term of R_v^T r (d: its
+// offset from the aggregate's centroid).
 template <typename T>
-__device__ __forceinline__ void restrict_add(const CoarseView<T>& cv, int64_t v, const T (&r)[3], double (&raw)[6]) {
-  const double d0 = (double)cv.dv[3 * v], d1 = (double)cv.dv[3 * v + 1], d2 = (double)cv.dv[3 * v + 2];
+__device__ __forceinline__ void restrict_add(const T (&d)[3], const T (&r)[3], double (&raw)[6]) {
+  const double d0 = (double)d[0], d1 = (double)d[1], d2 = (double)d[2];
   const double r0 = (double)r[0], r1 = (double)r[1], r2 = (double)r[2];
   raw[0] += r0;
   raw[1] += r1;
@@ -317,6 +318,7 @@ __device__ __forceinline__ void restrict_add(const CoarseView<T>& cv, int64_t v,
 // does not wait for them.
 struct AggPre {
   double col[6];
+  double dsc;  // thread j < k: the scale of coarse unknown c0 + j
   int c0, k;
 };
 
@@ -326,9 +328,11 @@ __device__ __forceinline__ AggPre agg_prefetch(const BlockPcgArgs<T>& a) {
 #pragma unroll
   for (int q = 0; q < 6; ++q) pre.col[q] = 0.0;
   pre.c0 = pre.k = 0;
+  pre.dsc = 0.0;
   if (kAgg) {
     pre.c0 = a.cv.agg_col[blockIdx.x];
     pre.k = a.cv.agg_col[blockIdx.x + 1] - pre.c0;
+    if ((int)threadIdx.x < pre.k) pre.dsc = a.cv.dsc[pre.c0 + threadIdx.x];
     if (threadIdx.x < 6) {
       const double* tm = a.cv.tm + 36 * (int64_t)blockIdx.x + threadIdx.x;
 #pragma unroll
@@ -339,7 +343,7 @@ __device__ __forceinline__ AggPre agg_prefetch(const BlockPcgArgs<T>& a) {
 }
 
 // Per-workgroup partials of r.z and r.r; with kAgg also the aggregate's
-// restricted residual c_I = T_I^T raw (thread j < k writes c_I[j]).
+// scaled restricted residual c_I = D_I T_I^T raw (thread j < k writes c_I[j]).
 template <bool kAgg, typename T>
 __device__ __forceinline__ void finish_vertex_pass(const BlockPcgArgs<T>& a, double rz, double rr,
                                                    const double (&raw)[6], const AggPre& pre) {
@@ -363,7 +367,7 @@ __device__ __forceinline__ void finish_vertex_pass(const BlockPcgArgs<T>& a, dou
     double c = 0.0;
 #pragma unroll
     for (int q = 0; q < 6; ++q) c += pre.col[q] * s_raw[q];
-    a.cv.c[pre.c0 + threadIdx.x] = c;
+    a.cv.c[pre.c0 + threadIdx.x] = c * pre.dsc;
   }
 }
 
@@ -389,7 +393,10 @@ __global__ __launch_bounds__(kBlock) void bpcg_init_kernel(BlockPcgArgs<T> a) {
       rz += (double)r[p] * (double)z[p];
       rr += (double)r[p] * (double)r[p];
     }
-    if (kAgg) restrict_add(a.cv, v, r, raw);
+    if (kAgg) {
+      const T d[3] = {a.cv.dv[3 * v], a.cv.dv[3 * v + 1], a.cv.dv[3 * v + 2]};
+      restrict_add(d, r, raw);
+    }
   });
   finish_vertex_pass<kAgg>(a, rz, rr, raw, pre);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -421,7 +428,10 @@ __global__ __launch_bounds__(kBlock) void bpcg_restart_kernel(BlockPcgArgs<T> a)
       rz += (double)r[p] * (double)z[p];
       rr += (double)r[p] * (double)r[p];
     }
-    if (kAgg) restrict_add(a.cv, v, r, raw);
+    if (kAgg) {
+      const T d[3] = {a.cv.dv[3 * v], a.cv.dv[3 * v + 1], a.cv.dv[3 * v + 2]};
+      restrict_add(d, r, raw);
+    }
   });
   finish_vertex_pass<kAgg>(a, rz, rr, raw, pre);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -661,10 +671,10 @@ __global__ __launch_bounds__(kBlock) void bpcg_spmv_kernel(BlockPcgArgs<T> a) {
 // prologue's reductions so that the loads overlap them.
 template <typename T>
 struct UpdateIn {
-  T x[3], p[3], r[3], ap[3], B[9];
+  T x[3], p[3], r[3], ap[3], B[9], d[3];  // d: centroid offset (two-level restriction)
 };
 
-template <typename T>
+template <typename T, bool kAgg>
 __device__ __forceinline__ void load_update(const BlockPcgArgs<T>& a, int64_t v, UpdateIn<T>& u) {
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
@@ -673,6 +683,7 @@ __device__ __forceinline__ void load_update(const BlockPcgArgs<T>& a, int64_t v,
     u.p[q] = a.pout[k];
     u.r[q] = a.r[k];
     u.ap[q] = a.Ap[k];
+    u.d[q] = kAgg ? a.cv.dv[k] : T(0);
   }
 #pragma unroll
   for (int m = 0; m < 9; ++m) u.B[m] = a.binv[9 * v + m];
@@ -697,7 +708,7 @@ __device__ __forceinline__ void apply_update(const BlockPcgArgs<T>& a, int64_t v
     rz += (double)r[q] * (double)z[q];
     rr += (double)r[q] * (double)r[q];
   }
-  if (kAgg) restrict_add(a.cv, v, r, raw);
+  if (kAgg) restrict_add(u.d, r, raw);
 }
 
 template <typename T, bool kAgg>
@@ -718,33 +729,89 @@ __global__ __launch_bounds__(kBlock) void bpcg_update_kernel(BlockPcgArgs<T> a) 
     if (t < t_end) v = t;
   }
   UpdateIn<T> u;
-  if (v >= 0) load_update(a, v, u);
+  if (v >= 0) load_update<T, kAgg>(a, v, u);
+  // aggregates larger than the workgroup: the thread's second vertex is loaded
+  // with the first, so the largest aggregates do not add a dependent round
+  int64_t v2 = -1;
+  UpdateIn<T> u2;
+  if (kAgg && t + stride < t_end) {
+    v2 = a.cv.vert(t + stride);
+    load_update<T, kAgg>(a, v2, u2);
+  }
   double alpha_d;
   if (!update_prologue_finish(L, a.st, a.spart, a.gs, a.k, alpha_d, a.hst)) return;
   const T alpha = (T)alpha_d;
   double rz = 0.0, rr = 0.0, raw[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (v >= 0) apply_update<T, kAgg>(a, v, u, alpha, rz, rr, raw);
+  if (kAgg) {
+    if (v2 >= 0) apply_update<T, kAgg>(a, v2, u2, alpha, rz, rr, raw);
+    t += stride;
+  }
   for (t += stride; t < t_end; t += stride) {
     const int64_t w = kAgg ? a.cv.vert(t) : t;
     UpdateIn<T> uw;
-    load_update(a, w, uw);
+    load_update<T, kAgg>(a, w, uw);
     apply_update<T, kAgg>(a, w, uw, alpha, rz, rr, raw);
   }
   finish_vertex_pass<kAgg>(a, rz, rr, raw, pre);
 }
 
+// A vertex of the coarse kernel's prolongation: aggregate-order position t
+// (none when t >= t1), its vertex, centroid offset, z and free mask.
+template <typename T>
+struct ProlongIn {
+  bool has;
+  int64_t v;
+  double d[3];
+  T z[3];
+  bool free[3];
+};
+
+template <typename T>
+__device__ __forceinline__ void load_prolong(const BlockPcgArgs<T>& a, int32_t t, int32_t t1, ProlongIn<T>& p) {
+  p.has = t < t1;
+  p.v = 0;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    p.d[q] = 0.0;
+    p.z[q] = T(0);
+    p.free[q] = false;
+  }
+  if (!p.has) return;
+  p.v = a.cv.vert(t);
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    p.d[q] = (double)a.cv.dva[3 * (int64_t)t + q];
+    p.z[q] = a.z[3 * p.v + q];
+    p.free[q] = !(a.fixed && a.fixed[3 * p.v + q]);
+  }
+}
+
+// z += u + w x d on the vertex's free DOFs (uw = (u, w)).
+template <typename T>
+__device__ __forceinline__ void apply_prolong(const BlockPcgArgs<T>& a, const ProlongIn<T>& p, const double (&uw)[6]) {
+  if (!p.has) return;
+  const double corr[3] = {uw[0] + uw[4] * p.d[2] - uw[5] * p.d[1], uw[1] + uw[5] * p.d[0] - uw[3] * p.d[2],
+                          uw[2] + uw[3] * p.d[1] - uw[4] * p.d[0]};
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+    if (p.free[q]) a.z[3 * p.v + q] = (T)((double)p.z[q] + corr[q]);
+}
+
 // Coarse correction of the two-level preconditioner, one workgroup per
-// aggregate I (launched after a kAgg vertex pass of the same grid): y_I =
-// (E^-1 c)_I, the aggregate's rigid motion (u, w) = T_I y_I is added to z on
-// its free DOFs, and c_I . y_I is added to the workgroup's r.z partial, so
-// the next launch's r.z is r . (B^-1 r + W E^-1 W^T r). No block of this
+// aggregate I (launched after a kAgg vertex pass of the same grid): with the
+// scaled residual c = D W^T r, y'_I = ((S16 + shift I) c)_I and y_I = D_I y'_I
+// (so y = D (S16 + shift I) D W^T r, the coarse solve), the aggregate's rigid
+// motion (u, w) = T_I y_I is added to z on its free DOFs, and c_I . y'_I is
+// added to the workgroup's r.z partial, so the next launch's r.z is
+// r . (B^-1 r + W E^-1 W^T r) with the FP16 coarse inverse. No block of this
 // launch writes the stop flag (only the SpMV and update launches before it
 // do), so every thread reads the same final value at entry: a launch past
-// convergence returns before streaming E^-1 and writes nothing.
+// convergence returns before streaming S16 and writes nothing.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void bpcg_coarse_kernel(BlockPcgArgs<T> a) {
   __shared__ double lds[6 * kBlock / kWave];
-  __shared__ double s_y[6], s_uw[6], s_cy[6];
+  __shared__ double s_y[6], s_uw[6];
   const CoarseView<T>& cv = a.cv;
   const int I = blockIdx.x;
   if (a.st->done) return;
@@ -755,100 +822,81 @@ __global__ __launch_bounds__(kBlock) void bpcg_coarse_kernel(BlockPcgArgs<T> a) 
 #pragma unroll
     for (int j = 0; j < 6; ++j) trow[j] = cv.tm[36 * (int64_t)I + 6 * threadIdx.x + j];
   }
-  const double cq = ((int)threadIdx.x < k) ? cv.c[c0 + threadIdx.x] : 0.0;
-  const double part0 = threadIdx.x == 0 ? a.upart[I] : 0.0;
-  // The first vertex this thread prolongs into, loaded now: z is final (the
-  // previous launch wrote it) and only this thread writes it below.
-  const int32_t tp = t0 + (int32_t)threadIdx.x;
-  const bool has_p = tp < t1;
-  int64_t pv = 0;
-  double pd[3] = {0.0, 0.0, 0.0};
-  T pz[3] = {T(0), T(0), T(0)};
-  bool pfree[3] = {false, false, false};
-  if (has_p) {
-    pv = cv.vert(tp);
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      pd[q] = (double)cv.dva[3 * (int64_t)tp + q];
-      pz[q] = a.z[3 * pv + q];
-      pfree[q] = !(a.fixed && a.fixed[3 * pv + q]);
-    }
-  }
-  // y_q = sum_j E^-1[c0 + q][j] c_j: four columns per float4 load. Every load
-  // of a pass (6 rows x kU) is issued before the first FMA, so the rows'
-  // latencies overlap instead of adding up; rows past the aggregate's k
-  // re-read an in-bounds row and are not accumulated.
-  constexpr int kU = 3;  // one pass for up to 3072 coarse unknowns
-  const int ld4 = cv.ld >> 2;
-  const float4* __restrict__ E4 = reinterpret_cast<const float4*>(cv.einv);
-  const double2* __restrict__ c2 = reinterpret_cast<const double2*>(cv.c);
-  const float4* rows[6];
+  double cs[6], ds[6];  // the aggregate's scaled residual and scales (the same for every thread)
 #pragma unroll
   for (int q = 0; q < 6; ++q) {
-    int rq = c0 + (q < k ? q : k - 1);
-    rq = rq < 0 ? 0 : (rq >= cv.nc ? cv.nc - 1 : rq);
-    rows[q] = E4 + (int64_t)rq * ld4;
+    cs[q] = q < k ? cv.c[c0 + q] : 0.0;
+    ds[q] = q < k ? cv.dsc[c0 + q] : 0.0;
   }
+  const double part0 = threadIdx.x == 0 ? a.upart[I] : 0.0;
+  // The first two vertices this thread prolongs into (the second in aggregates
+  // larger than the workgroup), loaded now: z is final (the previous launch
+  // wrote it) and only this thread writes them below.
+  const int32_t tp = t0 + (int32_t)threadIdx.x;
+  ProlongIn<T> p0, p1;
+  load_prolong(a, tp, t1, p0);
+  load_prolong(a, tp + kBlock, t1, p1);
+  // y'_q = sum_j S16[c0 + q][j] c_j: eight FP16 columns per 16-byte load, up
+  // to kU loads per row in flight per thread, FP64 accumulation
+  constexpr int kU = 2;
+  const int ld8 = cv.ld >> 3;
+  const uint4* __restrict__ E8 = reinterpret_cast<const uint4*>(cv.einv);
+  const double2* __restrict__ c2 = reinterpret_cast<const double2*>(cv.c);
   double y[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  for (int base = threadIdx.x; base < ld4; base += kU * kBlock) {
-    double2 ca[kU], cb[kU];
-    float4 e[6][kU];
-#pragma unroll
-    for (int q = 0; q < 6; ++q)
-#pragma unroll
-      for (int u = 0; u < kU; ++u) {
-        const int j4 = base + u * kBlock;
-        e[q][u] = j4 < ld4 ? rows[q][j4] : make_float4(0.f, 0.f, 0.f, 0.f);
-      }
+  for (int base = threadIdx.x; base < ld8; base += kU * kBlock) {
+    double2 cc[kU][4];
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
-      const int j4 = base + u * kBlock;
-      ca[u] = j4 < ld4 ? c2[2 * j4] : make_double2(0.0, 0.0);
-      cb[u] = j4 < ld4 ? c2[2 * j4 + 1] : make_double2(0.0, 0.0);
+      const int j8 = base + u * kBlock;
+#pragma unroll
+      for (int h = 0; h < 4; ++h) cc[u][h] = j8 < ld8 ? c2[4 * j8 + h] : make_double2(0.0, 0.0);
     }
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
       if (q < k) {
+        const uint4* row = E8 + (int64_t)(c0 + q) * ld8;
+        uint4 e[kU];
 #pragma unroll
-        for (int u = 0; u < kU; ++u)
-          y[q] += (double)e[q][u].x * ca[u].x + (double)e[q][u].y * ca[u].y + (double)e[q][u].z * cb[u].x +
-                  (double)e[q][u].w * cb[u].y;
+        for (int u = 0; u < kU; ++u) {
+          const int j8 = base + u * kBlock;
+          e[u] = j8 < ld8 ? row[j8] : make_uint4(0u, 0u, 0u, 0u);
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+          const _Float16* hv = reinterpret_cast<const _Float16*>(&e[u]);
+#pragma unroll
+          for (int h = 0; h < 4; ++h)
+            y[q] += (double)hv[2 * h] * cc[u][h].x + (double)hv[2 * h + 1] * cc[u][h].y;
+        }
       }
     }
   }
   block_sum<6>(y, lds);
   if (threadIdx.x == 0) {
+    double cy = 0.0;
 #pragma unroll
-    for (int q = 0; q < 6; ++q) s_y[q] = y[q];
+    for (int q = 0; q < 6; ++q) {
+      const double yq = y[q] + cv.shift * cs[q];
+      s_y[q] = ds[q] * yq;
+      cy += cs[q] * yq;
+    }
+    a.upart[I] = part0 + cy;
   }
   __syncthreads();
-  if (threadIdx.x < 6) {
+  if (threadIdx.x < 6) {  // (u, w) = T_I y_I
     double u = 0.0;
 #pragma unroll
     for (int j = 0; j < 6; ++j) u += trow[j] * s_y[j];
     s_uw[threadIdx.x] = u;
-    s_cy[threadIdx.x] = cq * s_y[threadIdx.x];
   }
   __syncthreads();
-  if (threadIdx.x == 0) a.upart[I] = part0 + (s_cy[0] + s_cy[1] + s_cy[2] + s_cy[3] + s_cy[4] + s_cy[5]);
-  const double u0 = s_uw[0], u1 = s_uw[1], u2 = s_uw[2], w0 = s_uw[3], w1 = s_uw[4], w2 = s_uw[5];
-  if (has_p) {
-    const double corr[3] = {u0 + w1 * pd[2] - w2 * pd[1], u1 + w2 * pd[0] - w0 * pd[2],
-                            u2 + w0 * pd[1] - w1 * pd[0]};
-#pragma unroll
-    for (int q = 0; q < 3; ++q)
-      if (pfree[q]) a.z[3 * pv + q] = (T)((double)pz[q] + corr[q]);
-  }
-  for (int32_t t = tp + kBlock; t < t1; t += kBlock) {  // aggregates larger than the workgroup
-    const int64_t v = cv.vert(t);
-    const double d0 = (double)cv.dva[3 * (int64_t)t], d1 = (double)cv.dva[3 * (int64_t)t + 1],
-                 d2 = (double)cv.dva[3 * (int64_t)t + 2];
-    const double corr[3] = {u0 + w1 * d2 - w2 * d1, u1 + w2 * d0 - w0 * d2, u2 + w0 * d1 - w1 * d0};
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      const int64_t kk = 3 * v + q;
-      if (!(a.fixed && a.fixed[kk])) a.z[kk] = (T)((double)a.z[kk] + corr[q]);
-    }
+  const double uw[6] = {s_uw[0], s_uw[1], s_uw[2], s_uw[3], s_uw[4], s_uw[5]};
+  apply_prolong(a, p0, uw);
+  apply_prolong(a, p1, uw);
+  for (int32_t t = tp + 2 * kBlock; t < t1; t += kBlock) {  // aggregates above twice the workgroup
+    ProlongIn<T> pt;
+    load_prolong(a, t, t1, pt);
+    apply_prolong(a, pt, uw);
   }
 }
 
