@@ -255,13 +255,15 @@ typedef struct sk_step_info {
   double ms_pcg;
   double ms_line_search;
   int refine_steps;      /* SK_PREC_MIXED: FP64 residual replacements summed over the Newton iterations */
-  int guess_used;        /* sk_sim_step_guess: 1 if Newton started from the supplied guess */
-  double guess_dE;       /* sk_sim_step_guess: E(x_guess) - E(x~), FP64 */
-  double guess_g_norm;   /* sk_sim_step_guess: ||g_free(x_guess)|| when the guess was taken, else 0 */
+  int guess_used;        /* sk_sim_step_guess(es): k if Newton started from the k-th supplied guess (1-based),
+                            0 if from x~ */
+  double guess_dE;       /* sk_sim_step_guess(es): E(first guess) - E(x~), FP64 */
+  double guess_g_norm;   /* sk_sim_step_guess(es): ||g_free|| at the guess taken, else 0 */
   int stalled;           /* 1: stopped because the update fell below position resolution (see sk_sim_step) */
   int inverted_elems;    /* elements with det F <= 0 at x_next */
   int64_t polar_fp64_elems; /* SK_PREC_MIXED, ARAP/FCR: element Hessians evaluated in FP64 because
                                det F < 1e-3, summed over the step's Newton iterations */
+  double start_dE;       /* sk_sim_step_guess(es): E(start) - E(x~) of the start taken (0 for x~) */
 } sk_step_info;
 
 /* mu, lam, vol: per element; mass: per DOF (3nv, M = kron(M_lumped, I3));
@@ -293,6 +295,17 @@ int sk_sim_step(sk_sim* sim, const double* x_curr, const double* x_prev, const d
  * x~); it is the hook for a learned warm start. */
 int sk_sim_step_guess(sk_sim* sim, const double* x_curr, const double* x_prev, const double* x_bc,
                       const double* x_guess, double* x_next, sk_step_info* info, sk_stream_t stream);
+
+/* sk_sim_step_guess with up to 8 candidate start points (guesses: host array
+ * of n_guess device pointers, caller's numbering): Newton starts from the
+ * candidate with the lowest incremental potential when that is below E(x~)
+ * (ties go to the earlier candidate), else from x~; info.guess_used is the
+ * 1-based index of the start taken. For a learned warm start with a
+ * non-learned fallback: a prediction that is worse than the fallback is
+ * rejected in favour of it rather than of x~. No counterpart in the
+ * reference. */
+int sk_sim_step_guesses(sk_sim* sim, const double* x_curr, const double* x_prev, const double* x_bc, int n_guess,
+                        const double* const* guesses, double* x_next, sk_step_info* info, sk_stream_t stream);
 
 /* One BDF2 step (bdf2.py:31-101 with kinetic.py bdf2_target / velocity_bdf2,
  * c = 9/4): Newton on V(x) + c/(2h^2) |x - x~|_M^2 - f_ext.x from

@@ -97,6 +97,7 @@ class SkStepInfo(ctypes.Structure):
         ("stalled", ctypes.c_int),
         ("inverted_elems", ctypes.c_int),
         ("polar_fp64_elems", ctypes.c_int64),
+        ("start_dE", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -209,6 +210,7 @@ SIGNATURES = {
     "sk_mlp_backward": [_P, _P, _P, _INT, ctypes.POINTER(_D), _P],
     "sk_mlp_adam": [_P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _P],
     "sk_sim_step_guess": [_P, _P, _P, _P, _P, _P, ctypes.POINTER(SkStepInfo), _P],
+    "sk_sim_step_guesses": [_P, _P, _P, _P, ctypes.c_int, _P, _P, ctypes.POINTER(SkStepInfo), _P],
     "sk_sim_step_bdf2": [_P, _P, _P, _P, _P, _P, _P, ctypes.POINTER(SkStepInfo), _P],
     "sk_sim_spmv_reps": [_P, _INT, _P],
     "sk_sim_hess_reps": [_P, _INT, _P],

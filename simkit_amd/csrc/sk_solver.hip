@@ -2043,11 +2043,13 @@ namespace {
 
 // Newton on E(x) = V(x) + c/(2h^2) |x - x~|_M^2 - f_ext.x for one implicit
 // step; S->x and S->xt hold the start point and x~ (written by the caller's
-// init kernel). With x_guess, the candidate start point is taken instead when
-// its incremental potential is lower (E(x_guess) - E(x~) < 0, evaluated in
-// FP64 in difference form); the stopping test stays relative to the gradient
-// norm at x~, so both start points converge to the same absolute tolerance.
-int step_core(sk_sim* S, double c_mass, const double* x_guess, double* x_next, sk_step_info* info, hipStream_t st) {
+// init kernel). With n_guess candidate start points (device arrays in the
+// caller's numbering), Newton starts from the one with the lowest incremental
+// potential if that is below E(x~) (E(guess) - E(x~), FP64, difference form;
+// ties go to the earlier candidate); the stopping test stays relative to the
+// gradient norm at x~, so every start converges to the same absolute tolerance.
+int step_core(sk_sim* S, double c_mass, const double* const* guesses, int n_guess, double* x_next,
+              sk_step_info* info, hipStream_t st) {
   sk_mesh* m = S->mesh;
   const sk_sim_params& P = S->prm;
   const bool mixed = S->mixed();
@@ -2068,14 +2070,26 @@ int step_core(sk_sim* S, double c_mass, const double* x_guess, double* x_next, s
   bool stalled = false;
   inf.g0_norm = g0;
   if (!std::isfinite(g0)) inf.status = SK_ERR_NONFINITE;
-  if (x_guess && inf.status == SK_OK) {
-    guess_dx_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, x_guess, S->x.p, fixed, S->order.p, S->dx.p);
-    SK_TRY(launch_elem_energy(m, P.material, S->x.p, nullptr, S->dx.p, 1.0, S->mu.p, lam, S->vol.p, nullptr,
-                              S->mass.p, S->xt.p, fext, fixed, inv_h2, st));
-    SK_TRY(read_scalars(m, m->red.result.p, 1, st));
-    inf.guess_dE = m->pinned[0];
-    if (inf.guess_dE < 0.0) {
-      inf.guess_used = 1;
+  if (n_guess > 0 && inf.status == SK_OK) {
+    int best = -1;
+    double best_dE = 0.0;
+    for (int j = 0; j < n_guess; ++j) {
+      guess_dx_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, guesses[j], S->x.p, fixed, S->order.p, S->dx.p);
+      SK_TRY(launch_elem_energy(m, P.material, S->x.p, nullptr, S->dx.p, 1.0, S->mu.p, lam, S->vol.p, nullptr,
+                                S->mass.p, S->xt.p, fext, fixed, inv_h2, st));
+      SK_TRY(read_scalars(m, m->red.result.p, 1, st));
+      const double dE = m->pinned[0];
+      if (j == 0) inf.guess_dE = dE;
+      if (dE < best_dE) {
+        best = j;
+        best_dE = dE;
+      }
+    }
+    if (best >= 0) {
+      inf.guess_used = best + 1;
+      inf.start_dE = best_dE;
+      if (best != n_guess - 1)  // S->dx holds the last candidate's offset
+        guess_dx_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, guesses[best], S->x.p, fixed, S->order.p, S->dx.p);
       axpy_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, 1.0, S->dx.p, S->x.p);
       SK_TRY(launch_elem_grad(m, P.material, S->x.p, nullptr, S->mu.p, lam, S->vol.p, st));
       SK_TRY(launch_gather_grad(m, S->g.p, S->mass.p, S->x.p, S->xt.p, fext, fixed, inv_h2, st));
@@ -2195,7 +2209,7 @@ int sk_sim_step(sk_sim* S, const double* x_curr, const double* x_prev, const dou
   be_init_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, x_curr, x_prev, x_bc, S->fixed.p, S->prm.h, S->order.p,
                                                    S->x.p, S->xt.p);
   SK_HIP_CHECK(hipGetLastError());
-  return step_core(S, 1.0, nullptr, x_next, info, st);
+  return step_core(S, 1.0, nullptr, 0, x_next, info, st);
 }
 
 int sk_sim_step_guess(sk_sim* S, const double* x_curr, const double* x_prev, const double* x_bc,
@@ -2206,7 +2220,20 @@ int sk_sim_step_guess(sk_sim* S, const double* x_curr, const double* x_prev, con
   be_init_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, x_curr, x_prev, x_bc, S->fixed.p, S->prm.h, S->order.p,
                                                    S->x.p, S->xt.p);
   SK_HIP_CHECK(hipGetLastError());
-  return step_core(S, 1.0, x_guess, x_next, info, st);
+  return step_core(S, 1.0, &x_guess, x_guess ? 1 : 0, x_next, info, st);
+}
+
+int sk_sim_step_guesses(sk_sim* S, const double* x_curr, const double* x_prev, const double* x_bc, int n_guess,
+                        const double* const* guesses, double* x_next, sk_step_info* info, sk_stream_t s) {
+  SK_REQUIRE(S && x_curr && x_prev && x_next, "sk_sim_step_guesses: null argument");
+  SK_REQUIRE(n_guess >= 0 && n_guess <= 8 && (n_guess == 0 || guesses), "sk_sim_step_guesses: 0..8 guesses");
+  for (int j = 0; j < n_guess; ++j) SK_REQUIRE(guesses[j], "sk_sim_step_guesses: null guess");
+  hipStream_t st = as_stream(s);
+  const int64_t n3 = 3 * S->mesh->nv;
+  be_init_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, x_curr, x_prev, x_bc, S->fixed.p, S->prm.h, S->order.p,
+                                                   S->x.p, S->xt.p);
+  SK_HIP_CHECK(hipGetLastError());
+  return step_core(S, 1.0, guesses, n_guess, x_next, info, st);
 }
 
 int sk_sim_step_bdf2(sk_sim* S, const double* x_curr, const double* x_prev, const double* x_prev2,
@@ -2217,7 +2244,7 @@ int sk_sim_step_bdf2(sk_sim* S, const double* x_curr, const double* x_prev, cons
   bdf2_init_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, x_curr, x_prev, x_prev2, x_prev3, x_bc, S->fixed.p,
                                                      S->prm.h, S->order.p, S->x.p, S->xt.p);
   SK_HIP_CHECK(hipGetLastError());
-  return step_core(S, 9.0 / 4.0, nullptr, x_next, info, st);
+  return step_core(S, 9.0 / 4.0, nullptr, 0, x_next, info, st);
 }
 
 int sk_sim_spmv_reps(sk_sim* S, int reps, sk_stream_t s) {

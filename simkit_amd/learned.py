@@ -28,7 +28,10 @@ The warm start has two parts:
 
 sk_sim_step_guess takes the guess only when its incremental potential is
 lower than x~'s, and Newton still stops at ||g|| <= rtol ||g(x~)||, so the
-answer meets the same tolerance either way. Rollouts run with start "plain"
+answer meets the same tolerance either way. The learned start passes the
+prediction and the lift to sk_sim_step_guesses, which starts from the lower
+of the two (if below x~): a prediction that is worse than the lift is
+rejected in favour of the lift, the method without the network. Rollouts run with start "plain"
 (x~, the reference's start), "lift" (x^) or "learned" (x^ + Phi dq), so that
 the network's own share of the gain is measured apart from the lift's.
 
@@ -243,11 +246,12 @@ class Rollout:
                 x_next, info = sim.step(x_curr, x_prev, self.lift_next)
             else:
                 if self.start == "learned":
+                    # the prediction first, the lift as the fallback: a prediction with a higher incremental
+                    # potential than the lift is rejected in favour of the lift, not of x~
                     c.add_modes(self.xhat, self.model.predict(self.feat), out=self.guess)
-                    g = self.guess
+                    x_next, info = sim.step_guesses(x_curr, x_prev, [self.guess, self.xhat], self.lift_next)
                 else:
-                    g = self.xhat
-                x_next, info = sim.step_guess(x_curr, x_prev, g, self.lift_next)
+                    x_next, info = sim.step_guess(x_curr, x_prev, self.xhat, self.lift_next)
             if info["status"] != 0:
                 raise RuntimeError(f"step {n} failed: {info}")
             q_next = c.project(x_next, self.lift_next)
@@ -346,6 +350,9 @@ def evaluate(sim, coords, model, episodes, steps, seed, starts=("plain", "lift",
         if name != "plain":
             used = [i for i in infos if i["guess_used"]]
             res[name]["guess_accepted"] = len(used) / n
+            if name == "learned":  # the prediction itself, or its fallback (the lift)
+                res[name]["prediction_taken"] = sum(1 for i in infos if i["guess_used"] == 1) / n
+                res[name]["fallback_taken"] = sum(1 for i in infos if i["guess_used"] == 2) / n
             # gradient norm at the taken guess relative to the plain start x~ (geometric mean)
             res[name]["guess_g_ratio"] = float(np.exp(np.mean([np.log(i["guess_g_norm"] / i["g0_norm"])
                                                                for i in used]))) if used else None

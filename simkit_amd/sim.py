@@ -199,6 +199,19 @@ class BackwardEulerSim:
                   ctypes.byref(info), stream())
         return x_next, info.as_dict()
 
+    def step_guesses(self, x_curr, x_prev, guesses, x_bc=None, out=None):
+        """BE step that starts Newton from the candidate of `guesses` (device tensors, in order) with the
+        lowest incremental potential when that is below E(x~) (info["guess_used"]: its 1-based index, 0
+        for x~), with the stopping test still relative to ||g(x~)||."""
+        if not 0 <= len(guesses) <= 8:
+            raise ValueError("step_guesses takes 0..8 guesses")
+        info = _lib.SkStepInfo()
+        x_next = self._out(x_curr, out)
+        arr = (ctypes.c_void_p * max(1, len(guesses)))(*[ptr(g) for g in guesses])
+        _lib.call("sk_sim_step_guesses", self._h, ptr(x_curr), ptr(x_prev), ptr(x_bc), len(guesses),
+                  ctypes.cast(arr, ctypes.c_void_p), ptr(x_next), ctypes.byref(info), stream())
+        return x_next, info.as_dict()
+
     def step_bdf2(self, x_curr, x_prev, x_prev2, x_prev3, x_bc=None, out=None):
         """One BDF2 step (c = 9/4, BDF2 target) on device tensors."""
         info = _lib.SkStepInfo()

@@ -205,6 +205,23 @@ def test_step_guess_accepts_good_and_rejects_bad_guesses():
     xb, ib = sim.step_guess(x1, x0, bad)
     assert ib["guess_used"] == 0 and ib["guess_dE"] > 0
     assert np.array_equal(xb.cpu().numpy(), x2.cpu().numpy()) and ib["newton_iters"] == i2["newton_iters"]
+    # several candidates: the lowest incremental potential below E(x~) is taken (1-based index), and the
+    # step from it is the single-guess step from the same point, bit for bit
+    good = x2.clone()
+    mid = 0.5 * (x2 + (2.0 * x1 - x0))      # halfway from x~ to the answer: E(x2) < E(mid) < E(x~)
+    xs, is_ = sim.step_guesses(x1, x0, [bad, mid, good])
+    xm, im = sim.step_guess(x1, x0, good)
+    assert is_["guess_used"] == 3 and is_["guess_dE"] == ib["guess_dE"] and is_["start_dE"] == im["guess_dE"]
+    assert np.array_equal(xs.cpu().numpy(), xm.cpu().numpy()) and is_["newton_iters"] == im["newton_iters"]
+    _, i_mid = sim.step_guess(x1, x0, mid)
+    assert i_mid["guess_used"] == 1 and im["guess_dE"] < i_mid["guess_dE"] < 0
+    _, iw = sim.step_guesses(x1, x0, [mid, bad])
+    assert iw["guess_used"] == 1 and iw["start_dE"] == i_mid["guess_dE"]
+    xn, i_none = sim.step_guesses(x1, x0, [bad, bad])
+    assert i_none["guess_used"] == 0 and i_none["start_dE"] == 0.0
+    assert np.array_equal(xn.cpu().numpy(), x2.cpu().numpy())
+    x0g, i0g = sim.step_guesses(x1, x0, [])
+    assert i0g["guess_used"] == 0 and np.array_equal(x0g.cpu().numpy(), x2.cpu().numpy())
 
 
 def torch_randn_like(x, seed):
