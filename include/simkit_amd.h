@@ -457,6 +457,12 @@ int sk_mlp_set_params(sk_mlp* net, const float* params, sk_stream_t stream);   /
 int sk_mlp_get_params(sk_mlp* net, float* params, sk_stream_t stream);
 /* Y (device, batch x dims[L], FP32) = net(X) (device, batch x dims[0], FP32). */
 int sk_mlp_forward(sk_mlp* net, const float* X, int batch, float* Y, sk_stream_t stream);
+/* A normalized predictor in one call, FP64 in and out (device arrays): Y = net((X - in_mu) / in_sd)
+ * * out_sd + out_mu, with per-column in_mu, in_sd (dims[0]) and out_mu, out_sd (dims[L]); the
+ * normalized input is rounded to FP32, then to BF16 as in sk_mlp_forward. No counterpart in the
+ * reference (the learned warm start's inference). */
+int sk_mlp_predict(sk_mlp* net, const double* X, int batch, const double* in_mu, const double* in_sd,
+                   const double* out_mu, const double* out_sd, double* Y, sk_stream_t stream);
 /* Forward + backward of the mean squared error against T (device, batch x
  * dims[L]); keeps the gradient for sk_mlp_adam / sk_mlp_get_grad; *loss_host
  * (may be NULL: no synchronization) receives the loss. */

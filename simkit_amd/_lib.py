@@ -207,6 +207,7 @@ SIGNATURES = {
     "sk_mlp_get_params": [_P, _P, _P],
     "sk_mlp_get_grad": [_P, _P, _P],
     "sk_mlp_forward": [_P, _P, _INT, _P, _P],
+    "sk_mlp_predict": [_P, _P, _INT, _P, _P, _P, _P, _P, _P],
     "sk_mlp_backward": [_P, _P, _P, _INT, ctypes.POINTER(_D), _P],
     "sk_mlp_adam": [_P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _P],
     "sk_sim_step_guess": [_P, _P, _P, _P, _P, _P, ctypes.POINTER(SkStepInfo), _P],

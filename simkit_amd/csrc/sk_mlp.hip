@@ -178,6 +178,26 @@ __global__ void pack_bf16_kernel(int rows, int cols, int prow, int pcol, const f
   dst[e] = (b < rows && k < cols) ? f2bf(src[(int64_t)b * cols + k]) : (uint16_t)0;
 }
 
+// dst = BF16 of the FP32 rounding of (X - mu) / sd (FP64 input, per-column
+// mu and sd), zero in the padding: the input normalization of a predictor.
+__global__ void pack_affine_bf16_kernel(int rows, int cols, int prow, int pcol, const double* X, const double* mu,
+                                        const double* sd, uint16_t* dst) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)prow * pcol) return;
+  const int b = (int)(e / pcol), k = (int)(e % pcol);
+  dst[e] = (b < rows && k < cols) ? f2bf((float)((X[(int64_t)b * cols + k] - mu[k]) / sd[k])) : (uint16_t)0;
+}
+
+// out[b][n] = Y[b][n] sd[n] + mu[n] in FP64 for the unpadded block (the output
+// de-normalization of a predictor).
+__global__ void unpack_affine_kernel(int rows, int cols, int pcol, const float* Y, const double* sd,
+                                     const double* mu, double* out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)rows * cols) return;
+  const int b = (int)(e / cols), n = (int)(e % cols);
+  out[e] = (double)Y[(int64_t)b * pcol + n] * sd[n] + mu[n];
+}
+
 // out[b][n] = Y[b][n] for the unpadded block.
 __global__ void unpack_kernel(int rows, int cols, int pcol, const float* Y, float* out) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -272,11 +292,9 @@ namespace {
 
 hipStream_t as_stream(sk_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
 
-int mlp_forward(sk_mlp* n, const float* X, int batch, hipStream_t st) {
+// The layers on the packed input Abf[0] (batch rows, padded to pad16(batch)).
+int mlp_layers(sk_mlp* n, int batch, hipStream_t st) {
   const int pb = pad16(batch);
-  const int64_t e0 = (int64_t)pb * n->pdims[0];
-  pack_bf16_kernel<<<grid_for(e0), kBlock, 0, st>>>(batch, n->dims[0], pb, n->pdims[0], X, n->Abf[0].p);
-  SK_HIP_CHECK(hipGetLastError());
   for (int l = 0; l < n->L; ++l) {
     GemmArgs g{};
     g.M = pb;
@@ -299,6 +317,14 @@ int mlp_forward(sk_mlp* n, const float* X, int batch, hipStream_t st) {
     SK_TRY(launch_gemm(g, st));
   }
   return SK_OK;
+}
+
+int mlp_forward(sk_mlp* n, const float* X, int batch, hipStream_t st) {
+  const int pb = pad16(batch);
+  const int64_t e0 = (int64_t)pb * n->pdims[0];
+  pack_bf16_kernel<<<grid_for(e0), kBlock, 0, st>>>(batch, n->dims[0], pb, n->pdims[0], X, n->Abf[0].p);
+  SK_HIP_CHECK(hipGetLastError());
+  return mlp_layers(n, batch, st);
 }
 
 }  // namespace
@@ -436,6 +462,24 @@ int sk_mlp_get_grad(sk_mlp* n, float* grad, sk_stream_t s) {
   SK_REQUIRE(n && grad, "sk_mlp_get_grad: null argument");
   SK_REQUIRE(n->have_grad, "sk_mlp_get_grad: no gradient computed yet");
   return mlp_export(n, 1, grad, as_stream(s));
+}
+
+int sk_mlp_predict(sk_mlp* n, const double* X, int batch, const double* in_mu, const double* in_sd,
+                   const double* out_mu, const double* out_sd, double* Y, sk_stream_t s) {
+  SK_REQUIRE(n && X && Y && in_mu && in_sd && out_mu && out_sd, "sk_mlp_predict: null argument");
+  SK_REQUIRE(batch >= 1 && batch <= n->max_batch, "sk_mlp_predict: batch outside [1, max_batch]");
+  hipStream_t st = as_stream(s);
+  const int pb = pad16(batch);
+  const int64_t e0 = (int64_t)pb * n->pdims[0];
+  pack_affine_bf16_kernel<<<grid_for(e0), kBlock, 0, st>>>(batch, n->dims[0], pb, n->pdims[0], X, in_mu, in_sd,
+                                                            n->Abf[0].p);
+  SK_HIP_CHECK(hipGetLastError());
+  SK_TRY(mlp_layers(n, batch, st));
+  const int64_t e = (int64_t)batch * n->dims[n->L];
+  unpack_affine_kernel<<<grid_for(e), kBlock, 0, st>>>(batch, n->dims[n->L], n->pdims[n->L], n->Z[n->L - 1].p,
+                                                        out_sd, out_mu, Y);
+  SK_HIP_CHECK(hipGetLastError());
+  return SK_OK;
 }
 
 int sk_mlp_forward(sk_mlp* n, const float* X, int batch, float* Y, sk_stream_t s) {

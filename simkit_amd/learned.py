@@ -155,8 +155,6 @@ class WarmStartModel:
         self.in_sd = torch.ones(self.dims[0], dtype=torch.float64, device=dev_)
         self.out_mu = torch.zeros(self.k, dtype=torch.float64, device=dev_)
         self.out_sd = torch.ones(self.k, dtype=torch.float64, device=dev_)
-        self._x32 = torch.empty((1, self.dims[0]), dtype=torch.float32, device=dev_)
-        self._y32 = torch.empty((1, self.k), dtype=torch.float32, device=dev_)
 
     def set_normalization(self, F, D):
         self.in_mu = F.mean(0)
@@ -172,14 +170,12 @@ class WarmStartModel:
         return self.net.fit(Xn, Tn, epochs=epochs, batch=batch, lr=lr, seed=seed, lr_decay=lr_decay, log=log)
 
     def predict(self, feat, out=None):
-        """dq (device k, float64) for one feature vector (device 2k+12, float64)."""
-        self._x32[0].copy_((feat - self.in_mu) / self.in_sd)
-        self.net.forward(self._x32, out=self._y32)
-        y = self._y32[0].double() * self.out_sd + self.out_mu
-        if out is not None:
-            out.copy_(y)
-            return out
-        return y
+        """dq (device k, float64) for one feature vector (device 2k+12, float64): normalization, the
+        network and de-normalization in the library (sk_mlp_predict)."""
+        out = torch.empty(self.k, dtype=torch.float64, device=feat.device) if out is None else out
+        _lib.call("sk_mlp_predict", self.net._h, ptr(feat), 1, ptr(self.in_mu), ptr(self.in_sd), ptr(self.out_mu),
+                  ptr(self.out_sd), ptr(out), stream())
+        return out
 
     def state_dict(self):
         return {"dims": np.array(self.dims), "params": self.net.get_params(),
@@ -257,7 +253,7 @@ class Rollout:
             q_next = c.project(x_next, self.lift_next)
             if record:
                 feats.append(self.feat.clone())
-                targets.append(q_next - self.qhat)
+                targets.append(_axpby(1.0, q_next, -1.0, self.qhat, torch.empty_like(q_next)))
             infos.append(info)
             x_prev, x_curr = x_curr, x_next
             q_prev, q_curr = q_curr, q_next

@@ -100,3 +100,28 @@ def test_training_is_deterministic_and_fits():
     losses = runs[0][0]
     # the same schedule in the oracle (FP64 sums, BF16 rounding) reaches 0.0009 from 0.18
     assert losses[-1] < 0.05 * losses[0], losses
+
+
+def test_normalized_predict_matches_forward_and_oracle():
+    """sk_mlp_predict (FP64 in and out, input and output normalization in the library) against the FP32
+    forward of the same normalized input (the same BF16 input rows, so the network's output bit for bit
+    before the FP64 de-normalization) and against the oracle MLP."""
+    from simkit_amd import _lib
+    from simkit_amd._device import ptr, stream
+    dims = [140, 256, 256, 256, 64]
+    net, _, _ = _setup(dims, 4, 3)
+    rng = np.random.default_rng(7)
+    X = rng.standard_normal((5, dims[0])) * 3.0 + 1.0
+    mu_i, sd_i = rng.standard_normal(dims[0]), rng.uniform(0.5, 4.0, dims[0])
+    mu_o, sd_o = rng.standard_normal(dims[-1]), rng.uniform(1e-3, 2.0, dims[-1])
+    d = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.float64)).cuda()
+    Xd, mi, si, mo, so = d(X), d(mu_i), d(sd_i), d(mu_o), d(sd_o)
+    Y = torch.empty((5, dims[-1]), dtype=torch.float64, device="cuda")
+    _lib.call("sk_mlp_predict", net._h, ptr(Xd), 5, ptr(mi), ptr(si), ptr(mo), ptr(so), ptr(Y), stream())
+    Xn = ((X - mu_i) / sd_i).astype(np.float32)
+    Yf = net.forward(torch.as_tensor(Xn).cuda()).cpu().numpy().astype(np.float64)
+    Yp = Y.cpu().numpy()
+    # the network outputs are the same FP32 values; only the FP64 affine map can round differently
+    assert np.abs(Yp - (Yf * sd_o + mu_o)).max() <= 1e-15 * np.abs(Yp).max()
+    Yo, _, _ = omlp.forward(omlp.unflatten(net.get_params(), dims), Xn)
+    assert _rel((Yp - mu_o) / sd_o, Yo) < 1e-3
