@@ -64,6 +64,7 @@ struct CoarseSpace {
   double build_ms = 0.0;
   DevBuf<int32_t> agg_ptr, agg_vert, agg_col, vagg, adj_ptr, adj;
   DevBuf<double> tm, dv64, dva64, E, pivot, c;
+  DevBuf<double> gjP, gjC, gjR;  // blocked Gauss-Jordan: panel inverse, panel columns, new panel rows
   DevBuf<float> dv32, dva32;
   DevBuf<_Float16> einv;
   DevBuf<double> dsc;

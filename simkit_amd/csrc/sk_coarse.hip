@@ -141,29 +141,119 @@ __global__ __launch_bounds__(kBlock) void coarse_galerkin_kernel(GalerkinArgs<R>
   }
 }
 
-// Gauss-Jordan inversion in place, pivot k (E symmetric positive definite,
-// no pivoting): every entry off row and column k is eliminated; the pivot is
-// recorded for the row/column launch and for the host's definiteness check.
-__global__ __launch_bounds__(kBlock) void gj_eliminate_kernel(double* A, int n, int k, double* piv) {
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (int64_t)n * n) return;
-  const int i = (int)(idx / n), j = (int)(idx % n);
-  const double p = A[(int64_t)k * n + k];
-  if (idx == 0) piv[k] = p;
-  if (i == k || j == k) return;
-  A[idx] -= A[(int64_t)i * n + k] * A[(int64_t)k * n + j] / p;
+// Blocked Gauss-Jordan inversion in place (E symmetric positive definite, no
+// pivoting), kGJ pivots per panel P. With A11 = A[P][P], A12 = A[P][rest],
+// A21 = A[rest][P], sweeping the panel's pivots turns A into
+//   A11^-1,  A11^-1 A12,  -A21 A11^-1,  A22 - A21 A11^-1 A12
+// (the same matrix as sweeping the pivots one at a time), so after every
+// panel A holds E^-1. Per panel: the panel's inverse in LDS (its pivots, the
+// Schur complements' diagonals, recorded for the host's definiteness check),
+// the panel's columns copied out, its new rows, the rank-kGJ update of every
+// entry outside the panel, then the panel's rows, columns and block.
+constexpr int kGJ = 48;    // pivots per panel
+constexpr int kGJT = 64;   // rank-update output tile (rows and columns)
+
+__global__ __launch_bounds__(kBlock) void gj_panel_inverse_kernel(const double* A, int n, int k0, int kb,
+                                                                  double* Pinv, double* piv) {
+  __shared__ double M[kGJ][kGJ + 1];
+  for (int e = threadIdx.x; e < kb * kb; e += blockDim.x) M[e / kb][e % kb] = A[(int64_t)(k0 + e / kb) * n + k0 + e % kb];
+  __syncthreads();
+  for (int j = 0; j < kb; ++j) {
+    const double p = M[j][j];
+    if (threadIdx.x == 0) piv[k0 + j] = p;
+    for (int e = threadIdx.x; e < kb * kb; e += blockDim.x) {
+      const int r = e / kb, c = e % kb;
+      if (r != j && c != j) M[r][c] -= M[r][j] * M[j][c] / p;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < 2 * kb; e += blockDim.x) {
+      if (e < kb) M[j][e] = (e == j) ? 1.0 / p : M[j][e] / p;
+      else if (e - kb != j) M[e - kb][j] = -M[e - kb][j] / p;
+    }
+    __syncthreads();
+  }
+  for (int e = threadIdx.x; e < kb * kb; e += blockDim.x) Pinv[(e / kb) * kGJ + e % kb] = M[e / kb][e % kb];
 }
 
-__global__ __launch_bounds__(kBlock) void gj_rowcol_kernel(double* A, int n, int k, const double* piv) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= 2 * n) return;
-  const double p = piv[k];
-  if (idx < n) {
-    const int j = idx;
-    A[(int64_t)k * n + j] = (j == k) ? 1.0 / p : A[(int64_t)k * n + j] / p;
-  } else {
-    const int i = idx - n;
-    if (i != k) A[(int64_t)i * n + k] = -A[(int64_t)i * n + k] / p;
+// Cb[i][t] = A[i][k0 + t] (the panel's columns, all rows); Rb[t][j] =
+// (A11^-1 A[P][j])_t for the columns outside the panel.
+__global__ __launch_bounds__(kBlock) void gj_panel_copy_kernel(const double* A, int n, int k0, int kb,
+                                                               const double* Pinv, double* Cb, double* Rb) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nc = (int64_t)n * kb;
+  if (idx < nc) {
+    const int i = (int)(idx / kb), t = (int)(idx % kb);
+    Cb[(int64_t)i * kGJ + t] = A[(int64_t)i * n + k0 + t];
+  } else if (idx < 2 * nc) {
+    const int t = (int)((idx - nc) / n), j = (int)((idx - nc) % n);
+    if (j >= k0 && j < k0 + kb) return;
+    double acc = 0.0;
+    for (int q = 0; q < kb; ++q) acc += Pinv[t * kGJ + q] * A[(int64_t)(k0 + q) * n + j];
+    Rb[(int64_t)t * n + j] = acc;
+  }
+}
+
+// A[i][j] -= sum_t Cb[i][t] Rb[t][j] for i, j outside the panel: one
+// kGJT x kGJT tile per workgroup, each thread 4 x 4 outputs, operands in LDS.
+__global__ __launch_bounds__(kBlock) void gj_update_kernel(double* A, int n, int k0, int kb, const double* Cb,
+                                                           const double* Rb) {
+  __shared__ double Cs[kGJT][kGJ + 1];
+  __shared__ double Rs[kGJ][kGJT];
+  const int tiles = (n + kGJT - 1) / kGJT;
+  const int i0 = (blockIdx.x / tiles) * kGJT, j0 = (blockIdx.x % tiles) * kGJT;
+  for (int e = threadIdx.x; e < kGJT * kGJ; e += blockDim.x) {
+    const int r = e / kGJ, t = e % kGJ;
+    Cs[r][t] = (i0 + r < n && t < kb) ? Cb[(int64_t)(i0 + r) * kGJ + t] : 0.0;
+  }
+  for (int e = threadIdx.x; e < kGJ * kGJT; e += blockDim.x) {
+    const int t = e / kGJT, c = e % kGJT;
+    const int j = j0 + c;
+    Rs[t][c] = (t < kb && j < n && !(j >= k0 && j < k0 + kb)) ? Rb[(int64_t)t * n + j] : 0.0;
+  }
+  __syncthreads();
+  const int ty = threadIdx.x / 16, tx = threadIdx.x % 16;
+  double acc[4][4] = {};
+  for (int t = 0; t < kb; ++t) {
+    double c[4], r[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) c[a] = Cs[ty * 4 + a][t];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) r[b] = Rs[t][tx * 4 + b];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] += c[a] * r[b];
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int i = i0 + ty * 4 + a;
+    if (i >= n || (i >= k0 && i < k0 + kb)) continue;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int j = j0 + tx * 4 + b;
+      if (j < n && !(j >= k0 && j < k0 + kb)) A[(int64_t)i * n + j] -= acc[a][b];
+    }
+  }
+}
+
+// The panel's rows (Rb), columns (-Cb A11^-1) and block (A11^-1).
+__global__ __launch_bounds__(kBlock) void gj_panel_finish_kernel(double* A, int n, int k0, int kb, const double* Pinv,
+                                                                 const double* Cb, const double* Rb) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nr = (int64_t)n * kb;
+  if (idx < nr) {
+    const int t = (int)(idx / n), j = (int)(idx % n);
+    if (j >= k0 && j < k0 + kb) return;
+    A[(int64_t)(k0 + t) * n + j] = Rb[(int64_t)t * n + j];
+  } else if (idx < 2 * nr) {
+    const int i = (int)((idx - nr) / kb), t = (int)((idx - nr) % kb);
+    if (i >= k0 && i < k0 + kb) return;
+    double acc = 0.0;
+    for (int q = 0; q < kb; ++q) acc += Cb[(int64_t)i * kGJ + q] * Pinv[q * kGJ + t];
+    A[(int64_t)i * n + k0 + t] = -acc;
+  } else if (idx < 2 * nr + (int64_t)kb * kb) {
+    const int e = (int)(idx - 2 * nr), s = e / kb, t = e % kb;
+    A[(int64_t)(k0 + s) * n + k0 + t] = Pinv[s * kGJ + t];
   }
 }
 
@@ -339,6 +429,9 @@ int coarse_setup(CoarseSpace& C, sk_mesh* m, const double* X, const uint8_t* fix
   SK_HIP_CHECK(hipMemset(C.c.p, 0, ld * sizeof(double)));
   SK_TRY(C.pivot.alloc(nc));
   SK_TRY(C.E.alloc((size_t)nc * nc));
+  SK_TRY(C.gjP.alloc((size_t)kGJ * kGJ));
+  SK_TRY(C.gjC.alloc((size_t)nc * kGJ));
+  SK_TRY(C.gjR.alloc((size_t)kGJ * nc));
   SK_TRY(C.einv.alloc((size_t)nc * ld));
   SK_TRY(C.dsc.alloc(ld));
   SK_HIP_CHECK(hipMemset(C.dsc.p, 0, ld * sizeof(double)));
@@ -361,10 +454,16 @@ int coarse_build(CoarseSpace& C, sk_mesh* m, const void* vals, bool fp32, const 
     coarse_galerkin_kernel<double><<<C.n_agg, kBlock, 0, st>>>(a);
   }
   SK_HIP_CHECK(hipGetLastError());
-  const int ge = (int)((nn + kBlock - 1) / kBlock), gr = (2 * nc + kBlock - 1) / kBlock;
-  for (int k = 0; k < nc; ++k) {
-    gj_eliminate_kernel<<<ge, kBlock, 0, st>>>(C.E.p, nc, k, C.pivot.p);
-    gj_rowcol_kernel<<<gr, kBlock, 0, st>>>(C.E.p, nc, k, C.pivot.p);
+  const int tiles = (nc + kGJT - 1) / kGJT;
+  for (int k0 = 0; k0 < nc; k0 += kGJ) {
+    const int kb = std::min(kGJ, nc - k0);
+    const int64_t nk = (int64_t)nc * kb;
+    gj_panel_inverse_kernel<<<1, kBlock, 0, st>>>(C.E.p, nc, k0, kb, C.gjP.p, C.pivot.p);
+    gj_panel_copy_kernel<<<(int)((2 * nk + kBlock - 1) / kBlock), kBlock, 0, st>>>(C.E.p, nc, k0, kb, C.gjP.p,
+                                                                                   C.gjC.p, C.gjR.p);
+    gj_update_kernel<<<tiles * tiles, kBlock, 0, st>>>(C.E.p, nc, k0, kb, C.gjC.p, C.gjR.p);
+    gj_panel_finish_kernel<<<(int)((2 * nk + (int64_t)kb * kb + kBlock - 1) / kBlock), kBlock, 0, st>>>(
+        C.E.p, nc, k0, kb, C.gjP.p, C.gjC.p, C.gjR.p);
   }
   SK_HIP_CHECK(hipGetLastError());
   std::vector<double> piv(nc), dev(nc);
