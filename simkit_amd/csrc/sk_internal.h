@@ -3,6 +3,7 @@
 
 #include "sk_common.h"
 
+#include <type_traits>
 #include <vector>
 
 namespace sk {
@@ -57,6 +58,25 @@ struct DevBuf {
 // Upper-triangle 3x3-block index of local pair (i <= j) of a tet: 0..9. The
 // blocks of one local row i are consecutive.
 __host__ __device__ constexpr int ublock(int i, int j) { return i * (7 - i) / 2 + j; }
+
+// The inverse of a vertex's masked diagonal block (the block-Jacobi
+// preconditioner) is symmetric: the simulator stores its upper triangle
+// (00, 01, 02, 11, 12, 22) at binv[kBinv v], read as three 2-wide loads.
+constexpr int kBinv = 6;
+constexpr int kBinvPQ[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+
+template <typename T>
+__device__ __forceinline__ void binv_load(const T* __restrict__ binv, int64_t v, T (&B)[9]) {
+  using T2 = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
+  const T2* b2 = reinterpret_cast<const T2*>(binv + kBinv * v);
+  const T2 u0 = b2[0], u1 = b2[1], u2 = b2[2];
+  B[0] = u0.x;
+  B[1] = B[3] = u0.y;
+  B[2] = B[6] = u1.x;
+  B[4] = u1.y;
+  B[5] = B[7] = u2.x;
+  B[8] = u2.y;
+}
 
 // Staged element stiffness: the 10 upper 3x3 blocks of K_e. Block u (ublock)
 // of element e sits at R-unit offset kKstR<R> e + kBlkStride<R> u. FP64
@@ -166,7 +186,7 @@ int launch_gather_grad(sk_mesh* m, double* g, const double* mass, const double* 
                        const double* fext, const uint8_t* fixed, double inv_h2, hipStream_t st);
 // Gathers staged element blocks into CSR values; with mass != nullptr adds
 // mass * inv_h2 on the diagonal and writes the fixed-DOF-masked inverse of each
-// diagonal 3x3 block into binv (9 per vertex).
+// diagonal 3x3 block into binv (its upper triangle, kBinv per vertex).
 // layout: kLayoutCSR (scalar CSR of the sorted pattern), kLayoutSELL (the
 // simulator's private SELL matrix) or kLayoutBSR (block b's 3x3 row-major at
 // values[9 b]). fp32: values / binv are float arrays summed in FP32, otherwise

@@ -390,14 +390,19 @@ __global__ __launch_bounds__(kBlock) void gather_hess_kernel(GatherHArgs a) {
     const R det = A[0] * c[0] + A[1] * c[1] + A[2] * c[2];
     const R inv = R(1) / det;
     R* binv = reinterpret_cast<R*>(a.binv);
+    if (a.binv_pos) {  // quad layout: all 9 values
 #pragma unroll
-    for (int p = 0; p < 3; ++p)
+      for (int p = 0; p < 3; ++p)
 #pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        const int64_t at =
-            a.binv_pos ? slot_index<kQuadC, kQuadVW>(a.binv_pos[v], 3 * p + q) : 9 * (int64_t)v + 3 * p + q;
-        binv[at] = f[p] * f[q] * c[3 * q + p] * inv;
+        for (int q = 0; q < 3; ++q)
+          binv[slot_index<kQuadC, kQuadVW>(a.binv_pos[v], 3 * p + q)] = f[p] * f[q] * c[3 * q + p] * inv;
+    } else {  // the upper triangle (the inverse of a symmetric block)
+#pragma unroll
+      for (int u = 0; u < kBinv; ++u) {
+        const int p = kBinvPQ[u][0], q = kBinvPQ[u][1];
+        binv[kBinv * (int64_t)v + u] = f[p] * f[q] * c[3 * q + p] * inv;
       }
+    }
   }
 }
 

@@ -280,7 +280,8 @@ struct BlockPcgArgs {
 
 template <typename T>
 __device__ __forceinline__ void apply_binv(const T* __restrict__ binv, int64_t v, const T (&r)[3], T (&z)[3]) {
-  const T* B = binv + 9 * v;
+  T B[9];
+  binv_load(binv, v, B);
 #pragma unroll
   for (int p = 0; p < 3; ++p) z[p] = B[3 * p] * r[0] + B[3 * p + 1] * r[1] + B[3 * p + 2] * r[2];
 }
@@ -685,8 +686,7 @@ __device__ __forceinline__ void load_update(const BlockPcgArgs<T>& a, int64_t v,
     u.ap[q] = a.Ap[k];
     u.d[q] = kAgg ? a.cv.dv[k] : T(0);
   }
-#pragma unroll
-  for (int m = 0; m < 9; ++m) u.B[m] = a.binv[9 * v + m];
+  binv_load(a.binv, v, u.B);
 }
 
 template <typename T, bool kAgg>
@@ -755,6 +755,10 @@ __global__ __launch_bounds__(kBlock) void bpcg_update_kernel(BlockPcgArgs<T> a) 
   }
   finish_vertex_pass<kAgg>(a, rz, rr, raw, pre);
 }
+
+// The FP16 values in the low and high halves of a 32-bit word.
+__device__ __forceinline__ float half_lo(uint32_t w) { return (float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xFFFFu)); }
+__device__ __forceinline__ float half_hi(uint32_t w) { return (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16)); }
 
 // A vertex of the coarse kernel's prolongation: aggregate-order position t
 // (none when t >= t1), its vertex, centroid offset, z and free mask.
@@ -863,10 +867,10 @@ __global__ __launch_bounds__(kBlock) void bpcg_coarse_kernel(BlockPcgArgs<T> a) 
         }
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
-          const _Float16* hv = reinterpret_cast<const _Float16*>(&e[u]);
+          const uint32_t wd[4] = {e[u].x, e[u].y, e[u].z, e[u].w};  // two FP16 columns per word, low first
 #pragma unroll
           for (int h = 0; h < 4; ++h)
-            y[q] += (double)hv[2 * h] * cc[u][h].x + (double)hv[2 * h + 1] * cc[u][h].y;
+            y[q] += (double)half_lo(wd[h]) * cc[u][h].x + (double)half_hi(wd[h]) * cc[u][h].y;
         }
       }
     }
@@ -1061,7 +1065,8 @@ __device__ __forceinline__ void pipe_row_product(const SellView& sv, const T* __
 
 template <typename T>
 __device__ __forceinline__ void pipe_apply_binv(const T* __restrict__ binv, int64_t v, const T (&r)[3], T (&u)[3]) {
-  const T* B = binv + 9 * v;
+  T B[9];
+  binv_load(binv, v, B);
 #pragma unroll
   for (int q = 0; q < 3; ++q) u[q] = B[3 * q] * r[0] + B[3 * q + 1] * r[1] + B[3 * q + 2] * r[2];
 }
@@ -1263,8 +1268,11 @@ __global__ __launch_bounds__(kBlock) void precond_scale_kernel(int64_t nslot, co
   T a[9];
   load_slot<T, kSellC, 1>(A, g, i, a);
   T B[9];
+  binv_load(binv, w >= 0 ? (int64_t)w : 0, B);
+  if (w < 0) {
 #pragma unroll
-  for (int m = 0; m < 9; ++m) B[m] = w >= 0 ? binv[9 * (int64_t)w + m] : T(0);
+    for (int m = 0; m < 9; ++m) B[m] = T(0);
+  }
   T* o = Ah + 9 * g + i;
 #pragma unroll
   for (int pq = 0; pq < 3; ++pq)
@@ -1947,13 +1955,13 @@ int sk_sim_create(sk_mesh* mesh, const sk_sim_params* p, const double* mu, const
         SK_TRY(b->alloc(n3));
       SK_TRY(S->vals32.alloc(9 * (size_t)mesh->nslot));
       SK_HIP_CHECK(hipMemset(S->vals32.p, 0, 9 * (size_t)mesh->nslot * sizeof(float)));
-      SK_TRY(S->binv32.alloc(9 * (size_t)mesh->nv));
+      SK_TRY(S->binv32.alloc(kBinv * (size_t)mesh->nv));
     } else {
       for (DevBuf<double>* b : {&S->z, &S->p0, &S->p1, &S->Ap})
         SK_TRY(b->alloc(n3));
       SK_TRY(S->vals.alloc(9 * (size_t)mesh->nslot));
       SK_HIP_CHECK(hipMemset(S->vals.p, 0, 9 * (size_t)mesh->nslot * sizeof(double)));
-      SK_TRY(S->binv.alloc(9 * (size_t)mesh->nv));
+      SK_TRY(S->binv.alloc(kBinv * (size_t)mesh->nv));
     }
     if (p->pcg_variant == 1) {
       if (S->mixed()) {
