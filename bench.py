@@ -368,7 +368,18 @@ def mixed_jittered_leg(cells, steps):
             del sim
         (k64, i64, t64), (k32, i32, t32) = runs["fp64"], runs["mixed"]
         err = [float(np.abs(k32[s] - k64[s]).max() / np.abs(k64[s]).max()) for s in range(1, steps + 1)]
-        out[mat] = {"per_step_rel_err_vs_fp64": err, "max_rel_err": max(err),
+        # against the FP64 oracle's trajectory of the same workload (sampled vertices; the fixture
+        # tests/golden/make_oracle_golden.py writes), where it exists for this size
+        vs_oracle = None
+        fx = os.path.join(ROOT, "tests", "golden", f"oracle_c3_jittered{cells}_{mat}.npz")
+        if os.path.exists(fx):
+            with np.load(fx) as d:
+                idx = d["idx"]
+                vs_oracle = {"steps": [int(s) for s in d["steps"] if s <= steps], "sampled_vertices": int(idx.size)}
+                for prec, kk in (("mixed", k32), ("fp64", k64)):
+                    vs_oracle[prec] = [float(np.abs(kk[s].reshape(-1, 3)[idx] - d[f"x_{s}"]).max() /
+                                             np.abs(d[f"x_{s}"]).max()) for s in vs_oracle["steps"]]
+        out[mat] = {"per_step_rel_err_vs_fp64": err, "max_rel_err": max(err), "rel_err_vs_oracle": vs_oracle,
                     "fp64_steps_per_s": round(steps / t64, 4), "mixed_steps_per_s": round(steps / t32, 4),
                     "newton_fp64": sum(i["newton_iters"] for i in i64),
                     "newton_mixed": sum(i["newton_iters"] for i in i32),
