@@ -274,10 +274,11 @@ def run_path(args, X, T, bottom, top, mu, lam, name):
 
 
 def learned_leg(cells, steps, warmup, preconditioner="bjacobi"):
-    """Learned warm start (boundary lift + the network's modal correction) on the FP32 + FP64-refinement
-    path, on the same trajectory as the headline, with the committed model for this mesh (trained on
-    random top-face sinusoids, so this trajectory is held out); also the lift without the network, to
-    isolate the network's share. None when no model exists for `cells`."""
+    """Learned warm start (boundary lift + extrapolation of its error + the network's modal correction) on
+    the FP32 + FP64-refinement path, on the same trajectory as the headline, with the committed model for
+    this mesh (trained on random top-face sinusoids, so this trajectory is held out); also the same start
+    without the network and the lift alone, to isolate each part's share. None when no model exists for
+    `cells`."""
     import torch
     from simkit_amd import learned
     model = learned.load_model(cells)
@@ -288,7 +289,7 @@ def learned_leg(cells, steps, warmup, preconditioner="bjacobi"):
     setup = time.perf_counter() - t0
     A, f = np.array([0.1, 0.0, 0.0]), np.array([1.0, 1.0, 1.0])   # d(t) = (0.1 sin 2 pi t, 0, 0)
     res = {}
-    for start in ("learned", "lift"):
+    for start in ("learned", "extrapolated", "lift"):
         ro = learned.Rollout(sim, coords, model if start == "learned" else None, start=start)
         ro.run(A, f, warmup)
         torch.cuda.synchronize()
@@ -303,6 +304,7 @@ def learned_leg(cells, steps, warmup, preconditioner="bjacobi"):
                       "pcg_iters": sum(i["pcg_iters"] for i in infos),
                       "guess_accepted": len(used) / len(infos),
                       "prediction_taken": sum(1 for i in infos if i["guess_used"] == 1) / len(infos),
+                      "start_used": (np.bincount([int(i["guess_used"]) for i in infos]) / len(infos)).round(4).tolist(),
                       "guess_g_ratio": float(np.exp(np.mean([np.log(i["guess_g_norm"] / i["g0_norm"])
                                                              for i in used]))) if used else None,
                       "x_final": out["x"].cpu().numpy()}
@@ -311,14 +313,17 @@ def learned_leg(cells, steps, warmup, preconditioner="bjacobi"):
     out = res["learned"]
     out["setup_s_modes_and_lift"] = round(setup, 2)
     out["modes"] = coords.modal_info
+    out["extrapolated_only"] = {k: v for k, v in res["extrapolated"].items() if k != "x_final"}
     out["lift_only"] = {k: v for k, v in res["lift"].items() if k != "x_final"}
-    out["network_speedup_over_lift"] = round(out["steps_per_s"] / res["lift"]["steps_per_s"], 3)
+    out["network_speedup_over_extrapolated"] = round(out["steps_per_s"] / res["extrapolated"]["steps_per_s"], 3)
+    out["extrapolation_speedup_over_lift"] = round(res["extrapolated"]["steps_per_s"] / res["lift"]["steps_per_s"], 3)
     return out
 
 
 def warmstart_eval_leg(cells, episodes, steps):
     """Learned warm start on its own workload: held-out random sinusoids of the driven cube from the plain
-    start x~, the boundary lift alone and the lift plus the network, with the committed model."""
+    start x~, the boundary lift alone, the lift plus the extrapolation of its error, and that plus the
+    network, with the committed model."""
     from simkit_amd import learned
     model = learned.load_model(cells)
     if model is None:
@@ -329,7 +334,8 @@ def warmstart_eval_leg(cells, episodes, steps):
         if isinstance(v, dict):
             v.pop("newton_per_step_list", None)
     log(f"bench[warmstart{cells}]: speedup {res['speedup']:.3f}, newton/step {res['plain']['newton_per_step']:.2f}"
-        f" -> lift {res['lift']['newton_per_step']:.2f} -> learned {res['learned']['newton_per_step']:.2f}")
+        f" -> lift {res['lift']['newton_per_step']:.2f} -> extrapolated {res['extrapolated']['newton_per_step']:.2f}"
+        f" -> learned {res['learned']['newton_per_step']:.2f}")
     return {"workload": f"driven cube {cells}^3 cells, random top-face sinusoids (held out), stable neo-Hookean, "
                         f"FP32 + FP64 refinement, block-Jacobi PCG", **res}
 

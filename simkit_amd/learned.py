@@ -13,7 +13,7 @@ mu = lam = 1, lumped mass with rho = 1, K + 1e-6 M) and L (3nv x 3) the static
 linear-elastic responses to a unit displacement of the driven face along
 each axis (the "lift", so that X + L d meets the boundary values).
 
-The warm start has two parts:
+The warm start has three parts:
 
 * the boundary lift: x~ = 2 x_curr - x_prev extrapolates the driven face to
   2 d_n - d_{n-1}, while the step prescribes d_{n+1}; the jump
@@ -21,19 +21,23 @@ The warm start has two parts:
   strain spike in the layer of elements under the face). x^ = x~ + L dd
   spreads it through the mesh with the static response; this is linear
   algebra, not learned;
+* the extrapolation: the lift's error e_n = x_n - x^_n (the dynamic,
+  nonlinear response it leaves out) changes smoothly from step to step, so
+  x^ + 2 e_n - e_{n-1} extrapolates it linearly from the last two steps
+  (x^ + e_n is kept as a fallback); also not learned;
 * the network: an MLP (simkit_amd.mlp, BF16 matrix cores) maps
   [q_curr, q_prev, 12 boundary-motion parameters] to the modal correction
-  dq = q_next - q^, where q^ are the modal coordinates of x^. The guess is
-  x^ + Phi dq.
+  dq = q_next - q_b, where q_b are the modal coordinates of the extrapolated
+  start. The guess is that start plus Phi dq.
 
-sk_sim_step_guess takes the guess only when its incremental potential is
-lower than x~'s, and Newton still stops at ||g|| <= rtol ||g(x~)||, so the
-answer meets the same tolerance either way. The learned start passes the
-prediction and the lift to sk_sim_step_guesses, which starts from the lower
-of the two (if below x~): a prediction that is worse than the lift is
-rejected in favour of the lift, the method without the network. Rollouts run with start "plain"
-(x~, the reference's start), "lift" (x^) or "learned" (x^ + Phi dq), so that
-the network's own share of the gain is measured apart from the lift's.
+sk_sim_step_guesses starts Newton from the candidate of lowest incremental
+potential when that is lower than x~'s, and Newton still stops at
+||g|| <= rtol ||g(x~)||, so the answer meets the same tolerance from any
+start. The learned start passes the prediction, then the extrapolated start,
+x^ + e_n and the lift: a prediction worse than the extrapolation is rejected
+in favour of it, not of x~. Rollouts run with start "plain" (x~, the
+reference's start), "lift", "extrapolated" or "learned", so that each part's
+share of the gain is measured apart from the others'.
 
 The 12 parameters of a top-face motion d(t) = A sin(2 pi f t) (per axis) at
 the step's end time t are [A (3), f (3), sin(2 pi f t) (3), cos(2 pi f t) (3)].
@@ -143,6 +147,11 @@ def motion_d(A, f, t):
     return np.asarray(A, np.float64) * np.sin(2.0 * np.pi * np.asarray(f, np.float64) * t)
 
 
+# The start the network's target dq is measured from: the extrapolated start of Rollout. A model
+# trained against another base would add its correction to the wrong state.
+TARGET_BASE = "extrapolated"
+
+
 class WarmStartModel:
     """The MLP and its input/output normalization."""
 
@@ -178,7 +187,7 @@ class WarmStartModel:
         return out
 
     def state_dict(self):
-        return {"dims": np.array(self.dims), "params": self.net.get_params(),
+        return {"dims": np.array(self.dims), "params": self.net.get_params(), "target_base": np.array(TARGET_BASE),
                 "in_mu": self.in_mu.cpu().numpy(), "in_sd": self.in_sd.cpu().numpy(),
                 "out_mu": self.out_mu.cpu().numpy(), "out_sd": self.out_sd.cpu().numpy()}
 
@@ -189,22 +198,34 @@ class WarmStartModel:
             setattr(self, name, dev(np.asarray(d[name], np.float64)))
 
 
+STARTS = ("plain", "lift", "extrapolated", "learned")
+
+
 class Rollout:
     """Steps a BackwardEulerSim through a driven-face sinusoid, tracking modal
-    coordinates. start: "plain" (Newton from x~), "lift" (from x^ = x~ + L dd when its
-    potential is lower) or "learned" (from x^ + Phi dq, dq predicted by the model); the default
-    is "learned" with a model, else "plain"."""
+    coordinates. start:
+      "plain":        Newton from x~;
+      "lift":         from x^ = x~ + L dd (the boundary lift) when its potential is lower;
+      "extrapolated": from x^ + 2 e_n - e_{n-1}, the lift's error e_n = x_n - x^_n of the last
+                      two steps extrapolated linearly, falling back to x^ + e_n, then to x^;
+      "learned":      from the extrapolated start plus Phi dq, dq the model's modal correction,
+                      falling back as "extrapolated" does.
+    Each step starts from the candidate of lowest incremental potential (if below x~'s). The
+    default is "learned" with a model, else "plain"."""
 
     def __init__(self, sim, coords, model=None, start=None):
         self.sim, self.c, self.model = sim, coords, model
         self.start = start or ("learned" if model is not None else "plain")
-        if self.start not in ("plain", "lift", "learned"):
-            raise ValueError("start must be 'plain', 'lift' or 'learned'")
+        if self.start not in STARTS:
+            raise ValueError("start must be one of " + ", ".join(repr(s) for s in STARTS))
         if self.start == "learned" and model is None:
             raise ValueError("start='learned' needs a model")
         n3 = 3 * coords.nv
         z = lambda: torch.empty(n3, dtype=torch.float64, device="cuda")
         self.xt, self.xhat, self.guess, self.lift_next = z(), z(), z(), z()
+        # the extrapolated start: x^ + 2 e_n - e_{n-1} (xb) and x^ + e_n (xe); e holds e_n, e_{n-1}
+        self.xb, self.xe, self.tmp = z(), z(), z()
+        self.e = [z(), z()]
         self.qhat = torch.empty(coords.k, dtype=torch.float64, device="cuda")
         self.feat = torch.empty(2 * coords.k + 12, dtype=torch.float64, device="cuda")
         # per-step host scalars (12 motion parameters, the face jump dd, the face displacement d) in
@@ -221,6 +242,11 @@ class Rollout:
         q_curr = torch.zeros_like(q_prev)
         feats, targets, infos = [], [], []
         d_prev = d_curr = motion_d(A, f, 0.0)          # from rest: x_prev = x_curr, face at d(0)
+        ext = self.start in ("extrapolated", "learned") or record
+        e1, e0 = self.e
+        if ext:  # no error history at rest
+            _axpby(0.0, c.X, 0.0, c.X, e1)
+            _axpby(0.0, c.X, 0.0, c.X, e0)
         for n in range(steps):
             t = (n + 1) * h
             d = motion_d(A, f, t)
@@ -232,22 +258,28 @@ class Rollout:
             if self.start != "plain" or record:
                 _axpby(2.0, x_curr, -1.0, x_prev, self.xt)
                 _expand(c.L, self._dev[12:15], self.xt, self.xhat)  # x^ = x~ + L dd
+            if ext:
+                _axpby(2.0, e1, -1.0, e0, self.tmp)
+                _axpby(1.0, self.xhat, 1.0, self.tmp, self.xb)  # x^ + 2 e_n - e_{n-1}
+                _axpby(1.0, self.xhat, 1.0, e1, self.xe)         # x^ + e_n
             if self.start == "learned" or record:
                 self.feat[:c.k].copy_(q_curr)
                 self.feat[c.k:2 * c.k].copy_(q_prev)
                 self.feat[2 * c.k:].copy_(self._dev[:12])
-            if record:
-                c.project(self.xhat, self.lift_next, out=self.qhat)
+            if record:  # the network's target is the modal correction to the extrapolated start
+                c.project(self.xb, self.lift_next, out=self.qhat)
             if self.start == "plain":
                 x_next, info = sim.step(x_curr, x_prev, self.lift_next)
+            elif self.start == "lift":
+                x_next, info = sim.step_guess(x_curr, x_prev, self.xhat, self.lift_next)
             else:
+                # candidates in order; the library starts from the one of lowest potential, so a
+                # prediction worse than the extrapolation falls back to it, not to x~
+                cands = [self.xb, self.xe, self.xhat]
                 if self.start == "learned":
-                    # the prediction first, the lift as the fallback: a prediction with a higher incremental
-                    # potential than the lift is rejected in favour of the lift, not of x~
-                    c.add_modes(self.xhat, self.model.predict(self.feat), out=self.guess)
-                    x_next, info = sim.step_guesses(x_curr, x_prev, [self.guess, self.xhat], self.lift_next)
-                else:
-                    x_next, info = sim.step_guess(x_curr, x_prev, self.xhat, self.lift_next)
+                    c.add_modes(self.xb, self.model.predict(self.feat), out=self.guess)
+                    cands = [self.guess] + cands
+                x_next, info = sim.step_guesses(x_curr, x_prev, cands, self.lift_next)
             if info["status"] != 0:
                 raise RuntimeError(f"step {n} failed: {info}")
             q_next = c.project(x_next, self.lift_next)
@@ -255,6 +287,9 @@ class Rollout:
                 feats.append(self.feat.clone())
                 targets.append(_axpby(1.0, q_next, -1.0, self.qhat, torch.empty_like(q_next)))
             infos.append(info)
+            if ext:  # e_{n+1} = x_{n+1} - x^_{n+1}
+                e1, e0 = e0, e1
+                _axpby(1.0, x_next, -1.0, self.xhat, e1)
             x_prev, x_curr = x_curr, x_next
             q_prev, q_curr = q_curr, q_next
             d_prev, d_curr = d_curr, d
@@ -277,6 +312,9 @@ def load_model(cells, k=64):
     if not os.path.exists(path):
         return None
     d = np.load(path, allow_pickle=False)
+    base = str(d["target_base"]) if "target_base" in d else "lift"
+    if base != TARGET_BASE:
+        raise ValueError(f"{path}: trained against the {base!r} start, not {TARGET_BASE!r}; retrain it")
     model = WarmStartModel(k, hidden=tuple(int(h) for h in d["dims"][1:-1]))
     model.load_state_dict(d)
     return model
@@ -304,7 +342,7 @@ def cube_problem(n, E=1e6, nu=0.45, rho=1000.0, h=1.0 / 60.0, precision="mixed",
 def generate(sim, coords, episodes, steps, seed, log=None):
     """Solver-generated training data: `episodes` random top-face sinusoids from rest."""
     rng = np.random.default_rng(seed)
-    ro = Rollout(sim, coords, start="lift")
+    ro = Rollout(sim, coords, start="extrapolated")
     F, D = [], []
     for ep in range(episodes):
         A, f = random_motion(rng)
@@ -318,9 +356,10 @@ def generate(sim, coords, episodes, steps, seed, log=None):
     return torch.cat(F), torch.cat(D)
 
 
-def evaluate(sim, coords, model, episodes, steps, seed, starts=("plain", "lift", "learned")):
-    """Held-out episodes run from each start ("plain": x~; "lift": x~ + L dd; "learned": the lift plus
-    the network's modal correction): Newton/PCG counts, wall time and agreement with the plain run."""
+def evaluate(sim, coords, model, episodes, steps, seed, starts=STARTS):
+    """Held-out episodes run from each start (Rollout's "plain", "lift", "extrapolated" and "learned"):
+    Newton/PCG counts, wall time, which candidate each step started from, and agreement with the plain
+    run."""
     rng = np.random.default_rng(seed)
     motions = [random_motion(rng) for _ in range(episodes)]
     res = {}
@@ -346,9 +385,11 @@ def evaluate(sim, coords, model, episodes, steps, seed, starts=("plain", "lift",
         if name != "plain":
             used = [i for i in infos if i["guess_used"]]
             res[name]["guess_accepted"] = len(used) / n
-            if name == "learned":  # the prediction itself, or its fallback (the lift)
+            # fraction of steps started from each candidate (index 1, 2, ...; 0: x~)
+            res[name]["start_used"] = (np.bincount([int(i["guess_used"]) for i in infos]) / n).round(4).tolist()
+            if name == "learned":  # the prediction itself, or one of its fallbacks
                 res[name]["prediction_taken"] = sum(1 for i in infos if i["guess_used"] == 1) / n
-                res[name]["fallback_taken"] = sum(1 for i in infos if i["guess_used"] == 2) / n
+                res[name]["fallback_taken"] = sum(1 for i in infos if i["guess_used"] >= 2) / n
             # gradient norm at the taken guess relative to the plain start x~ (geometric mean)
             res[name]["guess_g_ratio"] = float(np.exp(np.mean([np.log(i["guess_g_norm"] / i["g0_norm"])
                                                                for i in used]))) if used else None
@@ -361,11 +402,14 @@ def evaluate(sim, coords, model, episodes, steps, seed, starts=("plain", "lift",
         res["speedup"] = res["learned"]["steps_per_s"] / res["plain"]["steps_per_s"]
         res["newton_reduction"] = res["plain"]["newton_per_step"] / res["learned"]["newton_per_step"]
         res["final_position_rel_dev"] = res["learned"]["final_position_rel_dev"]
-    if "lift" in res and "learned" in res:
-        # the network's own share: learned against the lifted start without it
-        res["network_speedup_over_lift"] = res["learned"]["steps_per_s"] / res["lift"]["steps_per_s"]
-        res["network_newton_reduction_over_lift"] = (res["lift"]["newton_per_step"]
-                                                     / res["learned"]["newton_per_step"])
+    if "extrapolated" in res and "learned" in res:
+        # the network's own share: learned against the same start without it
+        res["network_speedup_over_extrapolated"] = (res["learned"]["steps_per_s"]
+                                                    / res["extrapolated"]["steps_per_s"])
+        res["network_newton_reduction_over_extrapolated"] = (res["extrapolated"]["newton_per_step"]
+                                                             / res["learned"]["newton_per_step"])
+    if "lift" in res and "extrapolated" in res:
+        res["extrapolation_speedup_over_lift"] = res["extrapolated"]["steps_per_s"] / res["lift"]["steps_per_s"]
     return res
 
 

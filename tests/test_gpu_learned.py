@@ -3,7 +3,8 @@ coordinates (basis M-orthonormal, lift exact on the driven face, project /
 expand round trip), training on solver-generated steps by the library's
 BF16 network, and held-out steps where the warm-started Newton reaches the
 plain path's answer (1e-6 relative) in fewer Newton iterations per step than
-the plain start x~, both with the boundary lift alone and with the network."""
+the plain start x~: with the boundary lift alone, with the extrapolation of
+its error added (no more iterations than the lift), and with the network."""
 
 import numpy as np
 import pytest
@@ -35,10 +36,13 @@ def test_modal_coordinates_and_warm_start_small_cube():
     res = learned.evaluate(sim, c, model, episodes=2, steps=30, seed=2)
     print({k: (v["newton_per_step"], v.get("guess_g_ratio")) for k, v in res.items() if isinstance(v, dict)},
           flush=True)
-    assert res["final_position_rel_dev"] < 1e-6 and res["lift"]["final_position_rel_dev"] < 1e-6
+    for start in ("lift", "extrapolated", "learned"):
+        assert res[start]["final_position_rel_dev"] < 1e-6, start
+        assert res[start]["steps"] == res["plain"]["steps"] == 60
     assert res["learned"]["guess_accepted"] >= 0.5
-    assert res["plain"]["steps"] == res["learned"]["steps"] == res["lift"]["steps"] == 60
     # the gain: fewer Newton iterations per held-out step than from x~ (same tolerance)
     assert res["lift"]["newton_per_step"] < res["plain"]["newton_per_step"]
+    assert res["extrapolated"]["newton_per_step"] <= res["lift"]["newton_per_step"]
     assert res["learned"]["newton_per_step"] < res["plain"]["newton_per_step"]
+    assert res["extrapolated"]["guess_g_ratio"] < res["lift"]["guess_g_ratio"]
     assert res["learned"]["guess_g_ratio"] < 0.5
