@@ -23,19 +23,20 @@ The warm start has three parts:
   algebra, not learned;
 * the extrapolation: the lift's error e_n = x_n - x^_n (the dynamic,
   nonlinear response it leaves out) changes smoothly from step to step, so
-  x^ + 2 e_n - e_{n-1} extrapolates it linearly from the last two steps
-  (x^ + e_n is kept as a fallback); also not learned;
+  the cubic through its last four values, x^ + 4 e_n - 6 e_{n-1} + 4 e_{n-2}
+  - e_{n-3}, extrapolates it (the quadratic and linear extrapolations are
+  kept as fallbacks); also not learned;
 * the network: an MLP (simkit_amd.mlp, BF16 matrix cores) maps
   [q_curr, q_prev, 12 boundary-motion parameters] to the modal correction
-  dq = q_next - q_b, where q_b are the modal coordinates of the extrapolated
-  start. The guess is that start plus Phi dq.
+  dq = q_next - q_b, where q_b are the modal coordinates of the cubic
+  extrapolated start. The guess is that start plus Phi dq.
 
 sk_sim_step_guesses starts Newton from the candidate of lowest incremental
 potential when that is lower than x~'s, and Newton still stops at
 ||g|| <= rtol ||g(x~)||, so the answer meets the same tolerance from any
-start. The learned start passes the prediction, then the extrapolated start,
-x^ + e_n and the lift: a prediction worse than the extrapolation is rejected
-in favour of it, not of x~. Rollouts run with start "plain" (x~, the
+start. The learned start passes the prediction, then the cubic, quadratic and
+linear extrapolations and the lift: a prediction worse than the extrapolation
+is rejected in favour of it, not of x~. Rollouts run with start "plain" (x~, the
 reference's start), "lift", "extrapolated" or "learned", so that each part's
 share of the gain is measured apart from the others'.
 
@@ -147,9 +148,14 @@ def motion_d(A, f, t):
     return np.asarray(A, np.float64) * np.sin(2.0 * np.pi * np.asarray(f, np.float64) * t)
 
 
-# The start the network's target dq is measured from: the extrapolated start of Rollout. A model
+# The start the network's target dq is measured from: Rollout's cubic extrapolated start. A model
 # trained against another base would add its correction to the wrong state.
-TARGET_BASE = "extrapolated"
+TARGET_BASE = "extrapolated-cubic"
+
+# Extrapolations of the lift's error, highest order first: order p takes
+# sum_j EXTRAP[p][j] e_{n-j}, the degree-p polynomial through the last p + 1
+# errors evaluated one step ahead (coefficients (-1)^j C(p+1, j+1)).
+EXTRAP = {3: (4.0, -6.0, 4.0, -1.0), 2: (3.0, -3.0, 1.0), 1: (2.0, -1.0)}
 
 
 class WarmStartModel:
@@ -206,8 +212,9 @@ class Rollout:
     coordinates. start:
       "plain":        Newton from x~;
       "lift":         from x^ = x~ + L dd (the boundary lift) when its potential is lower;
-      "extrapolated": from x^ + 2 e_n - e_{n-1}, the lift's error e_n = x_n - x^_n of the last
-                      two steps extrapolated linearly, falling back to x^ + e_n, then to x^;
+      "extrapolated": from x^ plus the lift's error e_n = x_n - x^_n extrapolated from the last
+                      four steps by the cubic through them (EXTRAP), falling back to the
+                      quadratic and linear extrapolations, then to x^;
       "learned":      from the extrapolated start plus Phi dq, dq the model's modal correction,
                       falling back as "extrapolated" does.
     Each step starts from the candidate of lowest incremental potential (if below x~'s). The
@@ -223,9 +230,10 @@ class Rollout:
         n3 = 3 * coords.nv
         z = lambda: torch.empty(n3, dtype=torch.float64, device="cuda")
         self.xt, self.xhat, self.guess, self.lift_next = z(), z(), z(), z()
-        # the extrapolated start: x^ + 2 e_n - e_{n-1} (xb) and x^ + e_n (xe); e holds e_n, e_{n-1}
-        self.xb, self.xe, self.tmp = z(), z(), z()
-        self.e = [z(), z()]
+        # the extrapolated starts, cubic first (xb is the cubic one); e holds e_n, ..., e_{n-3}
+        self.xext = [z() for _ in EXTRAP]
+        self.xb = self.xext[0]
+        self.e = [z() for _ in range(max(EXTRAP) + 1)]
         self.qhat = torch.empty(coords.k, dtype=torch.float64, device="cuda")
         self.feat = torch.empty(2 * coords.k + 12, dtype=torch.float64, device="cuda")
         # per-step host scalars (12 motion parameters, the face jump dd, the face displacement d) in
@@ -243,10 +251,10 @@ class Rollout:
         feats, targets, infos = [], [], []
         d_prev = d_curr = motion_d(A, f, 0.0)          # from rest: x_prev = x_curr, face at d(0)
         ext = self.start in ("extrapolated", "learned") or record
-        e1, e0 = self.e
+        e = list(self.e)  # e[j] = e_{n-j}
         if ext:  # no error history at rest
-            _axpby(0.0, c.X, 0.0, c.X, e1)
-            _axpby(0.0, c.X, 0.0, c.X, e0)
+            for v in e:
+                _axpby(0.0, c.X, 0.0, c.X, v)
         for n in range(steps):
             t = (n + 1) * h
             d = motion_d(A, f, t)
@@ -259,9 +267,11 @@ class Rollout:
                 _axpby(2.0, x_curr, -1.0, x_prev, self.xt)
                 _expand(c.L, self._dev[12:15], self.xt, self.xhat)  # x^ = x~ + L dd
             if ext:
-                _axpby(2.0, e1, -1.0, e0, self.tmp)
-                _axpby(1.0, self.xhat, 1.0, self.tmp, self.xb)  # x^ + 2 e_n - e_{n-1}
-                _axpby(1.0, self.xhat, 1.0, e1, self.xe)         # x^ + e_n
+                for out, p in zip(self.xext, sorted(EXTRAP, reverse=True)):
+                    w = EXTRAP[p]  # out = x^ + sum_j w_j e_{n-j}
+                    _axpby(1.0, self.xhat, w[0], e[0], out)
+                    for j in range(1, len(w)):
+                        _axpby(1.0, out, w[j], e[j], out)
             if self.start == "learned" or record:
                 self.feat[:c.k].copy_(q_curr)
                 self.feat[c.k:2 * c.k].copy_(q_prev)
@@ -275,7 +285,7 @@ class Rollout:
             else:
                 # candidates in order; the library starts from the one of lowest potential, so a
                 # prediction worse than the extrapolation falls back to it, not to x~
-                cands = [self.xb, self.xe, self.xhat]
+                cands = self.xext + [self.xhat]
                 if self.start == "learned":
                     c.add_modes(self.xb, self.model.predict(self.feat), out=self.guess)
                     cands = [self.guess] + cands
@@ -288,8 +298,8 @@ class Rollout:
                 targets.append(_axpby(1.0, q_next, -1.0, self.qhat, torch.empty_like(q_next)))
             infos.append(info)
             if ext:  # e_{n+1} = x_{n+1} - x^_{n+1}
-                e1, e0 = e0, e1
-                _axpby(1.0, x_next, -1.0, self.xhat, e1)
+                e = [e[-1]] + e[:-1]
+                _axpby(1.0, x_next, -1.0, self.xhat, e[0])
             x_prev, x_curr = x_curr, x_next
             q_prev, q_curr = q_curr, q_next
             d_prev, d_curr = d_curr, d
