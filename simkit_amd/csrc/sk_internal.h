@@ -97,12 +97,13 @@ constexpr int kFst = 12;  // staged doubles per element: f_e = vol P D
 // Grid-reduction scratch: per-block partial sums and result slots.
 struct Reducer {
   DevBuf<double> partials;  // up to 8 values x kMaxBlocks
-  DevBuf<double> result;    // 8 doubles
+  DevBuf<double> result;    // kResults doubles; [kCandidateSlot, kResults): a step's candidate starts
   static constexpr int kMaxBlocks = 65536;
+  static constexpr int kResults = 16, kCandidateSlot = 8;
   int init() {
     SK_TRY(partials.alloc(8 * (size_t)kMaxBlocks));
-    SK_TRY(result.alloc(8));
-    SK_HIP_CHECK(hipMemset(result.p, 0, 8 * sizeof(double)));
+    SK_TRY(result.alloc(kResults));
+    SK_HIP_CHECK(hipMemset(result.p, 0, kResults * sizeof(double)));
     return SK_OK;
   }
 };
@@ -161,7 +162,7 @@ int launch_elem_energy(sk_mesh* m, int material, const double* x, const double* 
                        const double* mu,
                        const double* lam, const double* vol, double* elem_energy, const double* mass,
                        const double* xt, const double* fext, const uint8_t* fixed, double inv_h2,
-                       hipStream_t st);
+                       hipStream_t st, double* result = nullptr);  // result: default red.result[0]
 int launch_elem_grad(sk_mesh* m, int material, const double* x, const double* F0, const double* mu,
                      const double* lam, const double* vol, hipStream_t st);
 // fp32: element Hessian, PSD projection and staged blocks in FP32 (staging

@@ -479,7 +479,7 @@ int launch_elem_energy(sk_mesh* m, int material, const double* x, const double* 
                        const double* mu,
                        const double* lam, const double* vol, double* elem_energy, const double* mass,
                        const double* xt, const double* fext, const uint8_t* fixed, double inv_h2,
-                       hipStream_t st) {
+                       hipStream_t st, double* result) {
   EnergyArgs a{view(m), m->nv, x, F0, dx, t, mu, lam, vol, elem_energy, mass, xt, fext, fixed, inv_h2,
                m->red.partials.p};
   const int64_t n = std::max<int64_t>(m->nt, mass ? m->nv : 0);
@@ -489,7 +489,7 @@ int launch_elem_energy(sk_mesh* m, int material, const double* x, const double* 
                elem_energy_kernel<typename decltype(tag)::template type<double>><<<g, kBlock, 0, st>>>(a);
              }),
              "unknown material");
-  finalize_sum_kernel<1><<<1, kBlock, 0, st>>>(m->red.partials.p, g, m->red.result.p);
+  finalize_sum_kernel<1><<<1, kBlock, 0, st>>>(m->red.partials.p, g, result ? result : m->red.result.p);
   SK_HIP_CHECK(hipGetLastError());
   return SK_OK;
 }

@@ -2081,12 +2081,18 @@ int step_core(sk_sim* S, double c_mass, const double* const* guesses, int n_gues
   if (n_guess > 0 && inf.status == SK_OK) {
     int best = -1;
     double best_dE = 0.0;
+    // every candidate's potential difference into its own result slot, read back together (the
+    // launches reuse dx and the partials in stream order)
+    double* cres = m->red.result.p + Reducer::kCandidateSlot;
+    static_assert(Reducer::kResults - Reducer::kCandidateSlot >= 8, "room for 8 candidates");
     for (int j = 0; j < n_guess; ++j) {
       guess_dx_kernel<<<grid_for(n3), kBlock, 0, st>>>(n3, guesses[j], S->x.p, fixed, S->order.p, S->dx.p);
       SK_TRY(launch_elem_energy(m, P.material, S->x.p, nullptr, S->dx.p, 1.0, S->mu.p, lam, S->vol.p, nullptr,
-                                S->mass.p, S->xt.p, fext, fixed, inv_h2, st));
-      SK_TRY(read_scalars(m, m->red.result.p, 1, st));
-      const double dE = m->pinned[0];
+                                S->mass.p, S->xt.p, fext, fixed, inv_h2, st, cres + j));
+    }
+    SK_TRY(read_scalars(m, cres, n_guess, st));
+    for (int j = 0; j < n_guess; ++j) {
+      const double dE = m->pinned[j];
       if (j == 0) inf.guess_dE = dE;
       if (dE < best_dE) {
         best = j;
